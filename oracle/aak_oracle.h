@@ -1,0 +1,136 @@
+/*
+ * aak_oracle.h -- CPU restatement of the reference AAK waveform path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (emri_kludge_suite_amd/)
+ * includes, links or calls this code.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load liboracle.so, as the checker.
+ *
+ * The functions restate, in plain C, the behaviour of the reference CPU AAK
+ * (AAKwrapper.wave -> AAKwave -> GenWave -> waveform -> PNevolution) and of
+ * the sparse-trajectory code it calls.  Each function cites the reference
+ * file:line it follows.  Where the reference links a GSL routine, the
+ * published method is restated instead (natural cubic spline, Powell hybrid,
+ * least squares, Miller recurrence for J_n); those endpoints are pinned by
+ * scipy/numpy in tests/, not by the reference itself (GSL is absent here).
+ */
+#ifndef AAK_ORACLE_H
+#define AAK_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One point of the sparse inspiral (Globals.h:185-191). */
+typedef struct {
+  double t, p, ecc, cosiota, E, Lz, Q, p3, p4, beta_wilkins, zedminus, betazedplus;
+} o_trajdata;
+
+/* Geodesic constants and frequencies (IEKG.cc:19-226).  out[0..15]:
+ * E, Lz, Q, Stable, zedminus, zedplus, betazedplus, r1, r2, r3, r4, p3, p4,
+ * Om_r, Om_theta, Om_phi. */
+int o_iekg(double p, double e, double cosi, double a, int want_freq, double *out);
+
+/* Radiation-reaction derivatives (GKTrajFast.cc:677-710): out = pdot, edot, cosidot. */
+int o_param_derivs(double p, double e, double cosi, double a, double *out);
+
+/* Circular Kerr constants at fixed cos(iota) (CKG.cc:9-74): out = E, Lz, Q, Lovercosi. */
+int o_ckg(double cosi, double r, double a, double *out);
+
+/* LSO table p_lso[csi][e] (GKTrajFast.cc:20-130): out = p[1][1], p[1][2], p[2][1], p[2][2]. */
+int o_plso_table(double cosi, double a, double *out);
+
+/* Sparse inspiral (GKTrajFast.cc:145-318).  traj has room for maxsteps+1
+ * records (index 0 unused, as in the reference).  Returns the reference's
+ * return flag; *steps receives the number of stored points. */
+int o_eccentric(double p, double e, double cosi, double a, double dtstart,
+                int maxsteps, o_trajdata *traj, int *steps);
+
+/* Carlson integrals and Legendre forms (behaviour of src/numrec NRr*.cc, NREll*.cc). */
+double o_rf(double x, double y, double z);
+double o_rd(double x, double y, double z);
+double o_rj(double x, double y, double z, double p);
+double o_rc(double x, double y);
+double o_ellf(double phi, double k);
+double o_elle(double phi, double k);
+double o_ellpi(double phi, double n, double k);
+
+/* Quartic roots by Laguerre + deflation + polish (behaviour of NRzroot.cc/NRLaguer.cc).
+ * coef[0..4] real coefficients; roots_re/roots_im sorted by real part (ascending). */
+int o_quartic_roots(const double *coef, double *roots_re, double *roots_im);
+
+/* PN maps (KSParMap.cc:86-209). */
+double o_dvdt(double v, double e, double Y, double m, double M, double q);
+double o_dedt(double v, double e, double Y, double m, double M, double q);
+double o_dtdm(double v, double e, double Y, double q);
+double o_drdm(double v, double e, double Y, double q);
+double o_dthetadm(double v, double e, double Y, double q);
+double o_dphidm(double v, double e, double Y, double q);
+double o_omegaphi(double v, double e, double cosiota, double s, double M);
+
+/* Frequency map (KSParMap.cc:323-358,397-440).  map = v_map, M_map, s_map.
+ * Returns the number of hybrid iterations (negative on solver failure). */
+int o_parmap(double *map, const double *Omega, double p, double M, double s,
+             double e, double iota);
+void o_sol_fun(const double *x, const double *Omega, double M, double e,
+               double iota, double *f);
+
+/* Natural cubic spline (KSParMap.cc:487-498 calls GSL cspline). */
+int o_spline_natural(const double *x, const double *y, int n, double *c);
+double o_spline_eval(const double *x, const double *y, const double *c, int n, double xv);
+
+/* Two-parameter least squares anchored at x[0] (KSParMap.cc:500-530). */
+int o_polyfit(double *coeff, const double *x, const double *y, int n);
+
+/* Bessel J_k(x) for k = nmin..nmax (nmin >= 0). */
+int o_bessel_jn_array(int nmin, int nmax, double x, double *out);
+
+/* Mean anomaly from true anomaly (KSParMap.cc:25). */
+double o_parang_phi0(double e, double psi);
+
+/* LISA-frame rotation coefficients (KSParMap.cc:532-574). */
+void o_rotcoeff(double *rot, double iota, double theta_S, double phi_S,
+                double theta_K, double phi_K, double alpha);
+
+/* Sparse stage of AAKwave (AAKpy.cc:20-44).  Fills up to 100 entries of
+ * e_traj, v_map, M_map, s_map, dt_map; returns steps (<=0 on error) and Phi0. */
+int o_sparse_stage(double T_fit, int length, double dt, double iota, double s,
+                   double p, double e, double M, double mu, double psi,
+                   double *e_traj, double *v_map, double *M_map, double *s_map,
+                   double *dt_map, double *Phi0, int *ecc_flag);
+
+/* Knot construction of PNevolution (AAK.cc:27-237).  Writes the spline knots
+ * (t_in,e_in,v_in,M_in,S_in) into caller arrays of capacity cap (needs
+ * >= 2*j0); returns the knot count K, or a negative error code.
+ * info[0]=i0, info[1]=j0, info[2]=j_min, info[3]=j_max, info[4]=j_max_temp,
+ * info[5]=j_start, info[6]=j_end, info[7]=plunged (j_max != j_max_temp). */
+int o_pn_knots(double timestep, int vlength, const double *par,
+               const double *e_traj, const double *v_map, double M_phys,
+               const double *M_map, double S_phys, const double *S_map,
+               const double *dt_map, int steps, int backint,
+               double *t_in, double *e_in, double *v_in, double *M_in,
+               double *S_in, int cap, int *info);
+int o_pn_j0(double timestep, int vlength, const double *dt_map, int steps);
+
+/* Full CPU AAK (AAKpy.cc:16-51 + AAK.cc:321-591).  pars[14] in the order
+ * iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D.
+ * n_max > 0 overrides the reference harmonic rule (AAK.cc:582-583).
+ * Outputs t, hI, hII (length vlength).  If dense != NULL it receives 10
+ * arrays of vlength+1 doubles: t,e,v,M,S,gim,Phi,alp,nu,gimdot.
+ * meta[0..7]: steps, nmodes, i_plunge, i_buffer, K, i0, Phi0, zeta. */
+int o_aak_wave(const double *pars, double T_fit, int length, double dt,
+               int LISA, int backint, int n_max,
+               double *t, double *hI, double *hII, double *dense, double *meta);
+
+/* Mode sum + response + taper given dense arrays (AAK.cc:409-547). */
+int o_waveform_from_dense(const double *dense, int vlength, double timestep,
+                          int nmodes, double zeta, double M_phys, double lam,
+                          double qS, double phiS, double qK, double phiK,
+                          int mich, int i_plunge, int i_buffer,
+                          double *t, double *hI, double *hII);
+
+const char *o_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
