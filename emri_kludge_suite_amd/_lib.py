@@ -1,0 +1,72 @@
+"""Loader for the in-tree HIP library libaak_hip.so (C ABI of include/gpuAAK.h).
+
+The library is required: there is no CPU fallback for any product entry point.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaak_hip.so")
+
+_d = C.c_double
+_i = C.c_int
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """Return the loaded libaak_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(
+            "libaak_hip.so not found at %s: build it with "
+            "`make -C emri_kludge_suite_amd/csrc` (or __graft_entry__.build())" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "aak_create": (_i, [_d, _i, _i, _d, _d, _i, _i, C.POINTER(C.c_void_p)]),
+        "aak_destroy": (_i, [C.c_void_p]),
+        "aak_input_data": (_i, [C.c_void_p, _pd, _pd, _pd, _pd, _i]),
+        "aak_gpu_gen_AAK": (_i, [C.c_void_p] + [_d] * 14),
+        "aak_likelihood": (_i, [C.c_void_p, _pd]),
+        "aak_get_waveform": (_i, [C.c_void_p, _pd, _pd, _pd]),
+        "aak_OmegaPhi": (_d, [_d] * 5),
+        "aak_set_nmax": (_i, [C.c_void_p, _i]),
+        "aak_get_info": (_i, [C.c_void_p, _pi, _i]),
+        "aak_last_error": (C.c_char_p, []),
+        "aak_test_host_plan": (_i, [_pd, _d, _i, _d, _i, _i, _i, _pd, _pd, _i, _pi, _pd]),
+        "aak_test_spline_table": (_i, [_pd, _d, _i, _d, _i, _i, _pd, _i]),
+        "aak_test_inspiral_rates": (_i, [_d, _d, _d, _d, _pd]),
+        "aak_test_get_dense": (_i, [C.c_void_p, _pd]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().aak_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc < 0:
+        raise RuntimeError("%s failed (%d): %s" % (what, rc, last_error()))
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_pd)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_pi)

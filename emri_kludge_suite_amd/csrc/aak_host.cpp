@@ -1,0 +1,481 @@
+// aak_host.cpp -- host half of the AAK waveform.
+//
+//   sparse_stage   AAKpy.cc:20-44 / AAK_manager.cu:296-320: inspiral, Kerr
+//                  frequencies, mean anomaly, frequency map at <= 100 points
+//   build_knots    AAK.cc:27-226: AK evolution, quadratic fit corrections,
+//                  plunge search and the spline knots
+//   plan_waveform  knots -> natural-spline table, plunge index/buffer and the
+//                  per-waveform constants of the device mode sum
+//
+// Compiled without floating-point contraction: the host stage reproduces the
+// reference CPU AAK's rounding, so the device fold can be exact.
+#include "aak_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "aak_freqmap.hpp"
+#include "aak_inspiral.hpp"
+
+namespace aak {
+
+double omega_phi(double v, double e, double cosiota, double s, double M) { return PN::omega_phi(v, e, cosiota, s, M); }
+
+int sparse_stage(const WaveConfig &cfg, const SourceParams &sp, SparseTrajectory *out, std::string *err) {
+  const int maxsteps = 100;
+  Inspiral<HostCx> insp;
+  if (insp.init(sp.s)) {
+    *err = "LSO table failed";
+    return -1;
+  }
+  const double tr = cfg.length * cfg.dt / kMsunSec / sp.M / sp.M * sp.mu;
+  const double dt_fit = ((tr < cfg.T_fit) ? tr : cfg.T_fit) / (maxsteps - 1);
+  InspiralPoint pts[100];
+  const int steps = insp.eccentric(sp.p, sp.e, cos(sp.iota), dt_fit, maxsteps, pts);
+  if (steps < 0) {
+    *err = "inspiral integration failed";
+    return -1;
+  }
+  if (steps < 3) {
+    *err = "trajectory too short (p0 at or inside the LSO?)";
+    return -2;
+  }
+  for (int i = 0; i < steps; i++) {
+    Geodesic<HostCx> g;
+    if (g.init(pts[i].p, pts[i].e, pts[i].cosi, sp.s, 1)) {
+      *err = "geodesic constants failed";
+      return -1;
+    }
+    double Om[3];
+    g.frequencies(Om);
+    if (i == 0) {
+      const double e = sp.e, psi = sp.psi;
+      out->Phi0 = atan2(sqrt(1. - e * e) * sin(psi), e + cos(psi)) - e * sqrt(1. - e * e) * sin(psi) / (1. + e * cos(psi));
+    }
+    FreqMap fm{Om, sp.M, pts[i].e, sp.iota};
+    double x[3] = {1. / sqrt(pts[i].p), sp.M, sp.s};
+    out->map_iters[i] = fm.solve(x);
+    out->e_traj[i] = pts[i].e;
+    out->v_map[i] = x[0];
+    out->M_map[i] = x[1];
+    out->s_map[i] = x[2];
+    out->dt_map[i] = pts[i].t * kMsunSec * sp.M * sp.M / sp.mu;
+  }
+  out->steps = steps;
+  return steps;
+}
+
+// y - y0 = c0 (x - x0) + c1 (x - x0)^2 by least squares: column-scaled
+// Householder QR of the (n-1) x 2 design matrix.
+static int polyfit2(double *coeff, const double *x, const double *y, int n) {
+  const int m = n - 1;
+  if (m < 2) return -1;
+  std::vector<double> A0(m), A1(m), b(m);
+  for (int i = 0; i < m; i++) {
+    A0[i] = x[i + 1] - x[0];
+    A1[i] = pow(x[i + 1] - x[0], 2);
+    b[i] = y[i + 1] - y[0];
+  }
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = 0; i < m; i++) {
+    s0 += A0[i] * A0[i];
+    s1 += A1[i] * A1[i];
+  }
+  s0 = sqrt(s0);
+  s1 = sqrt(s1);
+  if (s0 == 0.0) s0 = 1.0;
+  if (s1 == 0.0) s1 = 1.0;
+  for (int i = 0; i < m; i++) {
+    A0[i] /= s0;
+    A1[i] /= s1;
+  }
+  double *col[2] = {A0.data(), A1.data()};
+  double diag[2];
+  for (int k = 0; k < 2; k++) {
+    double nrm = 0.0;
+    for (int i = k; i < m; i++) nrm += col[k][i] * col[k][i];
+    nrm = sqrt(nrm);
+    const double alpha = (col[k][k] > 0) ? -nrm : nrm;
+    const double vk = col[k][k] - alpha;
+    double vtv = vk * vk;
+    for (int i = k + 1; i < m; i++) vtv += col[k][i] * col[k][i];
+    if (vtv != 0.0) {
+      for (int j = k + 1; j < 2; j++) {
+        double s = vk * col[j][k];
+        for (int i = k + 1; i < m; i++) s += col[k][i] * col[j][i];
+        const double t = 2.0 * s / vtv;
+        col[j][k] -= t * vk;
+        for (int i = k + 1; i < m; i++) col[j][i] -= t * col[k][i];
+      }
+      double s = vk * b[k];
+      for (int i = k + 1; i < m; i++) s += col[k][i] * b[i];
+      const double t = 2.0 * s / vtv;
+      b[k] -= t * vk;
+      for (int i = k + 1; i < m; i++) b[i] -= t * col[k][i];
+    }
+    diag[k] = alpha;
+  }
+  const double z1 = b[1] / diag[1];
+  const double z0 = (b[0] - col[1][0] * z1) / diag[0];
+  coeff[0] = z0 / s0;
+  coeff[1] = z1 / s1;
+  return 0;
+}
+
+static double pn_p(double v, double e, double coslam, double S, double M, double Mphys) {
+  return (1. - e * e) * pow(PN::omega_phi(v, e, coslam, S, M) * Mphys * kMsunSec, -2. / 3.);
+}
+
+int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double S_phys, const SparseTrajectory &tr,
+                KnotSet *ks, std::string *err) {
+  const double timestep = cfg.dt;
+  const int vlength = cfg.length, steps = tr.steps;
+  const double mu = par[0], M0 = par[1], S0 = par[2], e0 = par[3];
+  const double coslam = cos(par[4]);
+  const double *dt_map = tr.dt_map;
+  const double dt_large = dt_map[1];
+  const int j0 = (int)((vlength * timestep - dt_map[steps - 1]) / dt_large + steps + 1);
+  if (j0 <= steps || j0 > 400000000) {
+    *err = "knot count out of range";
+    return -3;
+  }
+  int j_start = j0, j_min = j0, j_end = 2 * j0 - 1, j_max = j_end, j_max_temp = j_max;
+  double t_end = vlength * timestep;
+  const size_t L = (size_t)2 * j0;
+  std::vector<double> eA(L, 0.0), vA(L, 0.0), tf(L, 0.0), ef(L, 0.0), vf(L, 0.0), Mf(L, 0.0), Sf(L, 0.0);
+  double edot = 0, vdot = 0, edotm, vdotm;
+  eA[j0] = e0;
+  vA[j0] = tr.v_map[0];
+  auto clamp_step = [&](int j) {
+    if (eA[j + 1] < 0. || vA[j + 1] < vA[j] || std::isnan(eA[j + 1]) || std::isnan(vA[j + 1])) {
+      if (j_max == j_max_temp) j_max = j;
+      eA[j + 1] = eA[j_max];
+      vA[j + 1] = vA[j_max];
+    }
+  };
+  for (int j = j0; j < j0 + steps - 1; j++) {
+    edotm = edot;
+    vdotm = vdot;
+    edot = PN::dedt(vA[j], eA[j], coslam, mu, M0, S0);
+    vdot = PN::dvdt(vA[j], eA[j], coslam, mu, M0, S0);
+    if (j == j0) {
+      edotm = edot;
+      vdotm = vdot;
+    }
+    eA[j + 1] = eA[j] + (1.5 * edot - .5 * edotm) * (dt_map[j - j0 + 1] - dt_map[j - j0]);
+    vA[j + 1] = vA[j] + (1.5 * vdot - .5 * vdotm) * (dt_map[j - j0 + 1] - dt_map[j - j0]);
+    clamp_step(j);
+  }
+  j_max_temp = j_max;
+  int points = std::min(steps, j_max - j0 + 1);
+  while (points >= 2 && tr.e_traj[points - 1] > tr.e_traj[points - 2]) points--;
+  if (points < 3) {
+    *err = "Fitting error: t_0 too close to plunge";
+    return -4;
+  }
+  double ec[2], vc[2], Mc[2], Sc[2];
+  {
+    std::vector<double> ediff(points), vdiff(points);
+    for (int i = 0; i < points; i++) {
+      ediff[i] = tr.e_traj[i] - eA[i + j0];
+      vdiff[i] = tr.v_map[i] - vA[i + j0];
+    }
+    polyfit2(ec, dt_map, ediff.data(), points);
+    polyfit2(vc, dt_map, vdiff.data(), points);
+    polyfit2(Mc, dt_map, tr.M_map, points);
+    polyfit2(Sc, dt_map, tr.s_map, points);
+  }
+  for (int j = j0 + steps - 1; j < j_end; j++) {
+    edotm = edot;
+    vdotm = vdot;
+    edot = PN::dedt(vA[j], eA[j], coslam, mu, M0, S0);
+    vdot = PN::dvdt(vA[j], eA[j], coslam, mu, M0, S0);
+    eA[j + 1] = eA[j] + (1.5 * edot - .5 * edotm) * dt_large;
+    vA[j + 1] = vA[j] + (1.5 * vdot - .5 * vdotm) * dt_large;
+    clamp_step(j);
+  }
+  j_max_temp = j_max;
+  for (int j = j0; j <= j_end; j++) {
+    double dt;
+    if (j < j0 + steps) dt = dt_map[j - j0];
+    else dt = dt_map[steps - 1] + dt_large * (j - j0 - steps + 1);
+    const double dt2 = dt * dt;
+    tf[j] = dt;
+    ef[j] = eA[j] + ec[0] * dt + ec[1] * dt2;
+    vf[j] = vA[j] + vc[0] * dt + vc[1] * dt2;
+    Mf[j] = M0 + (Mc[0] * dt + Mc[1] * dt2) * kMsunSec;
+    Sf[j] = S0 + Sc[0] * dt + Sc[1] * dt2;
+    if (ef[j] < 0. || vf[j] < vf[std::max(j0, j - 1)]) {
+      if (j_max == j_max_temp) j_max = j - 1;
+      ef[j] = ef[j_max];
+      vf[j] = vf[j_max];
+      Mf[j] = Mf[j_max];
+      Sf[j] = Sf[j_max];
+    }
+  }
+  j_max_temp = j_max;
+  const int j_RR = (int)((kMsunSec * M_phys * kMsunSec * M_phys / mu) / dt_large) + 1;
+  const double z1 = 1. + pow(1. - S_phys * S_phys, 1. / 3.) * (pow(1. + S_phys, 1. / 3.) + pow(1. - S_phys, 1. / 3.));
+  const double z2 = sqrt(3. * S_phys * S_phys + z1 * z1);
+  const double LSO_min = 3. + z2 - sqrt((3. - z1) * (3. + z1 + 2. * z2));
+  const double LSO_max = 3. + z2 + sqrt((3. - z1) * (3. + z1 + 2. * z2));
+  auto unstable = [&](int j, int *bad) {
+    Geodesic<HostCx> g;
+    if (g.init(pn_p(vf[j], ef[j], coslam, Sf[j], Mf[j], M_phys), ef[j], coslam, S_phys, 1)) {
+      *bad = 1;
+      return false;
+    }
+    return g.stable == -1 || g.E > 1;
+  };
+  int bad = 0;
+  for (int j = j0; j <= j_max_temp; j++) {
+    if (j > j_max) break;
+    if (pn_p(vf[j], ef[j], coslam, Sf[j], Mf[j], M_phys) < LSO_min && j_max == j_max_temp) {
+      j_min = std::max(j - j_RR, j0);
+      j_max = std::min(j + 1, j_max_temp);
+    }
+    if (j > j0 && (j - j0) % j_RR == 0 && pn_p(vf[j], ef[j], coslam, Sf[j], Mf[j], M_phys) < LSO_max && j_max == j_max_temp) {
+      j_min = std::max(j - j_RR, j0);
+      if (unstable(j, &bad)) j_max = std::min(j + 1, j_max_temp);
+      if (bad) break;
+    }
+  }
+  while (!bad && j_max - j_min > 1) {
+    const int jm = (j_max + j_min) / 2;
+    if (unstable(jm, &bad)) j_max = jm;
+    else j_min = jm;
+  }
+  if (bad) {
+    *err = "geodesic constants failed in the plunge search";
+    return -5;
+  }
+  if (cfg.backint) {
+    double edotp, vdotp;
+    j_end = j_max;
+    t_end = (tf[j_end] < vlength * timestep) ? tf[j_end] : vlength * timestep;
+    j_start = j0 + floor((t_end - vlength * timestep) / dt_large);
+    for (int j = j0; j > j_start; j--) {
+      edotp = edot;
+      vdotp = vdot;
+      edot = PN::dedt(vA[j], eA[j], coslam, mu, M0, S0);
+      vdot = PN::dvdt(vA[j], eA[j], coslam, mu, M0, S0);
+      if (j == j0) {
+        edotp = edot;
+        vdotp = vdot;
+      }
+      eA[j - 1] = eA[j] - (1.5 * edot - .5 * edotp) * dt_large;
+      vA[j - 1] = vA[j] - (1.5 * vdot - .5 * vdotp) * dt_large;
+    }
+    for (int j = j_start; j < j0; j++) {
+      const double dtu = dt_large * (j - j0);
+      double dt;
+      if (j > j0 - points) dt = (dtu + dtu * (j - j0 + points) / points) / 2.;
+      else dt = -dt_large * points / 2.;
+      const double dt2 = dt * dt;
+      tf[j] = dtu;
+      ef[j] = eA[j] + ec[0] * dt + ec[1] * dt2;
+      vf[j] = vA[j] + vc[0] * dt + vc[1] * dt2;
+      Mf[j] = M0 + (Mc[0] * dt + Mc[1] * dt2) * kMsunSec;
+      Sf[j] = S0 + Sc[0] * dt + Sc[1] * dt2;
+    }
+  }
+  const int K = j_end - j_start + 1;
+  ks->K = K;
+  ks->t.assign(tf.begin() + j_start, tf.begin() + j_end + 1);
+  ks->e.assign(ef.begin() + j_start, ef.begin() + j_end + 1);
+  ks->v.assign(vf.begin() + j_start, vf.begin() + j_end + 1);
+  ks->M.assign(Mf.begin() + j_start, Mf.begin() + j_end + 1);
+  ks->S.assign(Sf.begin() + j_start, Sf.begin() + j_end + 1);
+  ks->i0 = vlength - (int)(t_end / timestep);
+  ks->j0 = j0;
+  ks->j_min = j_min;
+  ks->j_max = j_max;
+  ks->j_max_temp = j_max_temp;
+  ks->j_start = j_start;
+  ks->j_end = j_end;
+  ks->plunged = (j_max != j_max_temp);
+  ks->points = points;
+  ks->t_fit_jmin = tf[j_min];
+  return K;
+}
+
+// Natural cubic spline: symmetric tridiagonal system for c = y''/2 solved by
+// LDL^T (the scheme of GSL's cspline, which the reference links).
+int spline_natural(const double *x, const double *y, int n, double *c) {
+  if (n < 3) return -1;
+  const int sys = n - 2;
+  std::vector<double> dg(sys), od(sys), g(sys), gam(sys), al(sys), z(sys);
+  c[0] = 0.0;
+  c[n - 1] = 0.0;
+  for (int i = 0; i < sys; i++) {
+    const double hi = x[i + 1] - x[i], hip1 = x[i + 2] - x[i + 1];
+    const double ydi = y[i + 1] - y[i], ydip1 = y[i + 2] - y[i + 1];
+    const double gi = (hi != 0.0) ? 1.0 / hi : 0.0;
+    const double gip1 = (hip1 != 0.0) ? 1.0 / hip1 : 0.0;
+    od[i] = hip1;
+    dg[i] = 2.0 * (hip1 + hi);
+    g[i] = 3.0 * (ydip1 * gip1 - ydi * gi);
+  }
+  if (sys == 1) {
+    c[1] = g[0] / dg[0];
+    return 0;
+  }
+  al[0] = dg[0];
+  gam[0] = od[0] / al[0];
+  for (int i = 1; i < sys - 1; i++) {
+    al[i] = dg[i] - od[i - 1] * gam[i - 1];
+    gam[i] = od[i] / al[i];
+  }
+  al[sys - 1] = dg[sys - 1] - od[sys - 2] * gam[sys - 2];
+  z[0] = g[0];
+  for (int i = 1; i < sys; i++) z[i] = g[i] - gam[i - 1] * z[i - 1];
+  for (int i = 0; i < sys; i++) z[i] = z[i] / al[i];
+  c[sys] = z[sys - 1];
+  for (int i = sys - 2; i >= 0; i--) c[1 + i] = z[i] - gam[i] * c[i + 2];
+  return 0;
+}
+
+// Device table: t | for each of (e, v, M, S): y, b, c, d per interval, with
+// b = dy/h - h (c_{k+1} + 2 c_k)/3 and d = (c_{k+1} - c_k)/(3h).
+void pack_spline(const KnotSet &ks, std::vector<double> *packed) {
+  const int K = ks.K;
+  packed->assign((size_t)17 * K, 0.0);
+  double *P = packed->data();
+  std::copy(ks.t.begin(), ks.t.end(), P);
+  const std::vector<double> *ys[4] = {&ks.e, &ks.v, &ks.M, &ks.S};
+  std::vector<double> c(K);
+  for (int a = 0; a < 4; a++) {
+    const double *y = ys[a]->data();
+    spline_natural(ks.t.data(), y, K, c.data());
+    double *Y = P + (size_t)K * (1 + 4 * a), *Bc = Y + K, *Cc = Y + 2 * (size_t)K, *Dc = Y + 3 * (size_t)K;
+    for (int k = 0; k < K; k++) {
+      Y[k] = y[k];
+      Cc[k] = c[k];
+      if (k < K - 1) {
+        const double h = ks.t[k + 1] - ks.t[k];
+        const double dy = y[k + 1] - y[k];
+        Bc[k] = (dy / h) - h * (c[k + 1] + 2.0 * c[k]) / 3.0;
+        Dc[k] = (c[k + 1] - c[k]) / (3.0 * h);
+      }
+    }
+  }
+}
+
+double packed_eval(const std::vector<double> &packed, int K, int array, double t) {
+  const double *x = packed.data();
+  int lo = 0, hi = K - 1;
+  while (hi > lo + 1) {
+    const int m = (hi + lo) / 2;
+    if (x[m] > t) hi = m;
+    else lo = m;
+  }
+  const double *Y = packed.data() + (size_t)K * (1 + 4 * array);
+  const double dl = t - x[lo];
+  return Y[lo] + dl * (Y[K + lo] + dl * (Y[2 * (size_t)K + lo] + dl * Y[3 * (size_t)K + lo]));
+}
+
+int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *plan, std::string *err) {
+  int rc = sparse_stage(cfg, sp, &plan->traj, err);
+  if (rc < 3) return rc < 0 ? rc : -2;
+  const SparseTrajectory &tr = plan->traj;
+  double *par = plan->par;
+  par[0] = sp.mu * kMsunSec;
+  par[1] = tr.M_map[0] * kMsunSec;
+  par[2] = tr.s_map[0];
+  par[3] = tr.e_traj[0];
+  par[4] = sp.iota;
+  par[5] = sp.gamma;
+  par[6] = tr.Phi0;
+  par[7] = sp.theta_S;
+  par[8] = sp.phi_S;
+  par[9] = sp.theta_K;
+  par[10] = sp.phi_K;
+  par[11] = sp.alph;
+  int nmodes = (int)(30 * par[3]);
+  if (par[3] < 0.135) nmodes = 4;
+  if (cfg.n_max > 0) nmodes = cfg.n_max;
+  rc = build_knots(cfg, par, sp.M, sp.s, tr, &plan->knots, err);
+  if (rc < 0) return rc;
+  const KnotSet &ks = plan->knots;
+  pack_spline(ks, &plan->packed);
+  DenseParams &P = plan->P;
+  std::memset(&P, 0, sizeof(P));
+  const int N = cfg.length;
+  P.N = N;
+  P.i0 = ks.i0;
+  P.K = ks.K;
+  P.dt = cfg.dt;
+  P.mich = cfg.LISA ? 1 : 0;
+  P.nmodes = nmodes;
+  P.t_first = ks.t[0];
+  P.t_last = ks.t[ks.K - 1];
+  const double coslam = cos(par[4]);
+  P.coslam = coslam;
+  P.x0[0] = par[5];
+  P.x0[1] = par[6];
+  P.x0[2] = par[11];
+  // plunge index and taper length (AAK.cc:240-242)
+  int ip = ks.plunged ? ks.i0 + (int)(ks.t_fit_jmin / cfg.dt) : N - 1;
+  if (ip < 0 || ip > N) {
+    *err = "plunge index outside the waveform";
+    return -6;
+  }
+  auto tsample = [&](int i) { return i == 0 ? P.t_first : (i == N ? P.t_last : (double)(i - P.i0) * P.dt); };
+  const double tp = tsample(ip);
+  const double ve = packed_eval(plan->packed, ks.K, 1, tp), ee = packed_eval(plan->packed, ks.K, 0, tp);
+  const double Me = packed_eval(plan->packed, ks.K, 2, tp), Se = packed_eval(plan->packed, ks.K, 3, tp);
+  const int ib = (int)(10. / (PN::omega_phi(ve, ee, coslam, Se, Me) / 2. / kPi) / cfg.dt) + 1;
+  P.ip = ip;
+  P.ib = ib;
+  P.last_fold = std::min(N - 1, ip + ib);
+  if (P.last_fold < P.i0) P.last_fold = P.i0 - 1;
+  P.t_plunge = tp;
+  P.t_zero = tp + cfg.dt * ib;
+  // mode-sum constants (AAK.cc:323-338, 434-479; KSParMap.cc:541-549)
+  const double lam = par[4], qS = par[7], phiS = par[8], qK = par[9], phiK = par[10];
+  P.lam = lam;
+  P.sinlam = sin(lam);
+  P.qS = qS;
+  P.cosqS = cos(qS);
+  P.sinqS = sin(qS);
+  P.qK = qK;
+  P.cosqK = cos(qK);
+  P.sinqK = sin(qK);
+  P.phiS = phiS;
+  P.phiK = phiK;
+  P.cosphiK = cos(phiK);
+  P.sinphiK = sin(phiK);
+  P.halfsqrt3 = sqrt(3.) / 2.;
+  P.Sdotn = P.cosqK * P.cosqS + P.sinqK * P.sinqS * cos(phiK - phiS);
+  P.sin_dphiKS = sin(phiK - phiS);
+  P.cos_dphiKS = cos(phiK - phiS);
+  P.psi_c0 = (P.cosqK * P.cosqS + P.sinqK * P.sinqS * cos(phiK - phiS));
+  P.psi_c1 = .5 * P.sinqK * P.sinqS * sin(phiK - phiS);
+  P.psi_c2 = (P.cosqK * P.sinqS * sin(phiS) - P.cosqS * P.sinqK * sin(phiK));
+  P.psi_c3 = (P.cosqS * P.sinqK * cos(phiK) - P.cosqK * P.sinqS * cos(phiS));
+  {
+    const double up = (P.cosqS * P.sinqK * cos(phiS - phiK) - P.cosqK * P.sinqS);
+    const double dw = (P.sinqK * sin(phiS - phiK));
+    const double psi_ldc = (dw != 0.0) ? -atan2(up, dw) : 0.5 * kPi;
+    P.ldc_c2psi = cos(2. * psi_ldc);
+    P.ldc_s2psi = sin(2. * psi_ldc);
+  }
+  P.n_vec[0] = sin(qS) * cos(phiS);
+  P.n_vec[1] = sin(qS) * sin(phiS);
+  P.n_vec[2] = cos(qS);
+  P.S_vec[0] = sin(qK) * cos(phiK);
+  P.S_vec[1] = sin(qK) * sin(phiK);
+  P.S_vec[2] = cos(qK);
+  P.cosiota_L = cos(lam);
+  P.siniota_L = sin(lam);
+  P.sin_thK = sin(qK);
+  P.cos_thK = cos(qK);
+  P.sin_phK = sin(phiK);
+  P.cos_phK = cos(phiK);
+  P.zeta = par[0] / sp.D / kGpc;
+  P.Mphys = sp.M;
+  return 0;
+}
+
+}  // namespace aak

@@ -1,0 +1,802 @@
+// aak_kernels.hip -- dense stage of the AAK waveform on CDNA4 (gfx950).
+//
+//   k_rates        spline evaluation of (e, v, M, S) on the dense grid with the
+//                  block's knot coefficients staged in LDS, then the three
+//                  phase rates from the PN frequency maps (AAK.cc:233-253)
+//   fold kernels   the Adams-Bashforth phase folds (AAK.cc:244-293) computed in
+//                  parallel but reproducing the sequential FP64 rounding bit for
+//                  bit: inside one binade every value is an integer multiple of
+//                  the same ulp, so the fold becomes an exact int64 prefix sum;
+//                  binade crossings, exact ties and values near zero are
+//                  replayed serially in FP64
+//   k_modesum      per-sample harmonic sum with Bessel amplitudes, Doppler
+//                  phases, LISA response and Planck taper (AAK.cc:409-547)
+//   k_weighted_dot PSD-weighted inner products of the rFFT output
+//                  (AAK_manager.cu:352-446 semantics)
+//
+// Compiled with -ffp-contract=off: every expression that feeds the phases
+// must round exactly like the reference CPU build.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+#include "aak_dense.hpp"
+#include "aak_math.hpp"
+
+namespace aak {
+
+// ------------------------------------------------------------------ rates
+constexpr int kRatesThreads = 256;
+constexpr int kStageCap = 320;  // knot intervals staged in LDS per block
+
+__device__ inline int knot_search(const double *x, int lo, int hi, double t) {
+  // largest k in [lo, hi-1] with x[k] <= t (hi is exclusive upper knot)
+  while (hi > lo + 1) {
+    const int m = (hi + lo) >> 1;
+    if (x[m] > t) hi = m;
+    else lo = m;
+  }
+  return lo;
+}
+
+__device__ inline double sample_time(const DenseParams &P, int i) {
+  if (i == 0) return P.t_first;
+  if (i == P.N) return P.t_last;
+  return (double)(i - P.i0) * P.dt;
+}
+
+__global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuffers B) {
+  __shared__ double s_x[kStageCap + 1];
+  __shared__ double s_c[16][kStageCap];
+  __shared__ int s_lo, s_cnt;
+  const int tid = threadIdx.x;
+  const int i_lo = blockIdx.x * kRatesThreads;
+  if (i_lo > P.N) return;
+  const int i_hi = min(i_lo + kRatesThreads - 1, P.N);
+  const int K = P.K;
+  const double *kt = B.knots;
+  if (tid == 0) {
+    const int klo = knot_search(kt, 0, K - 1, sample_time(P, i_lo));
+    const int khi = knot_search(kt, 0, K - 1, sample_time(P, i_hi));
+    s_lo = klo;
+    s_cnt = khi - klo + 1;
+  }
+  __syncthreads();
+  const int klo = s_lo, cnt = s_cnt;
+  const bool staged = (cnt <= kStageCap);
+  if (staged) {
+    for (int j = tid; j < cnt; j += kRatesThreads) {
+      s_x[j] = kt[klo + j];
+#pragma unroll
+      for (int c = 0; c < 16; c++) s_c[c][j] = B.knots[(size_t)K * (1 + c) + klo + j];
+    }
+    if (tid == 0) s_x[cnt] = (klo + cnt < K) ? kt[klo + cnt] : kt[K - 1];
+  }
+  __syncthreads();
+  const int i = i_lo + tid;
+  if (i > P.N) return;
+  const double t = sample_time(P, i);
+  double y[4];
+  if (staged) {
+    const int k = knot_search(s_x, 0, cnt, t);
+    const double dl = t - s_x[k];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+      y[a] = s_c[4 * a][k] + dl * (s_c[4 * a + 1][k] + dl * (s_c[4 * a + 2][k] + dl * s_c[4 * a + 3][k]));
+  } else {
+    const int k = knot_search(kt, 0, K - 1, t);
+    const double dl = t - kt[k];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const double *c = B.knots + (size_t)K * (1 + 4 * a);
+      y[a] = c[k] + dl * (c[K + k] + dl * (c[2 * (size_t)K + k] + dl * c[3 * (size_t)K + k]));
+    }
+  }
+  const double e = y[0], v = y[1], M = y[2], S = y[3];
+  B.e[i] = e;
+  B.v[i] = v;
+  B.M[i] = M;
+  B.S[i] = S;
+  const double Y = P.coslam;
+  const double T = PN::dtdm(v, e, Y, S);
+  const double R = PN::drdm(v, e, Y, S);
+  const double Th = PN::dthetadm(v, e, Y, S);
+  const double Ph = PN::dphidm(v, e, Y, S);
+  const double phidot = R / T / M;
+  B.rate[0][i] = (Th - R) / T / M;
+  B.rate[1][i] = phidot;
+  B.rate[2][i] = (Ph - Th) / T / M;
+  if (i > P.i0 && i <= P.last_fold && !(phidot > 0.)) atomicAdd(&B.flags[0], 1);
+}
+
+void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
+  const int nb = (P.N + 1 + kRatesThreads - 1) / kRatesThreads;
+  hipLaunchKernelGGL(k_rates, dim3(nb), dim3(kRatesThreads), 0, s, P, B);
+}
+
+// Non-positive radial rates take the previous (already corrected) value
+// (AAK.cc:254).  Rare: only reached past the plunge; processed in chunks
+// staged through LDS with one lane carrying the value.
+__global__ __launch_bounds__(256) void k_phidot_guard(DenseParams P, DenseBuffers B) {
+  __shared__ double buf[4096];
+  double *r = B.rate[1];
+  double prev = r[P.i0];
+  for (int base = P.i0 + 1; base <= P.last_fold; base += 4096) {
+    const int n = min(4096, P.last_fold - base + 1);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) buf[j] = r[base + j];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int j = 0; j < n; j++) {
+        if (!(buf[j] > 0.)) buf[j] = prev;
+        prev = buf[j];
+      }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) r[base + j] = buf[j];
+    __syncthreads();
+  }
+}
+
+void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
+  hipLaunchKernelGGL(k_phidot_guard, dim3(1), dim3(256), 0, s, P, B);
+}
+
+// ------------------------------------------------------------------ folds
+struct FoldArgs {
+  const double *rate[3];
+  double *out[3];
+  double x0[3];
+  int i0, dir, nsteps, nblk;
+  double dt;
+};
+
+__device__ inline double fold_incr(const double *rate, int i0, int dir, int s, double dt) {
+  const int i = i0 + dir * s;
+  const double r = rate[i];
+  const double rm = (s == 0) ? r : rate[i - dir];
+  const double d = (1.5 * r - .5 * rm) * dt;
+  return dir > 0 ? d : -d;
+}
+
+__device__ inline int out_index(const FoldArgs &A, int s) { return A.i0 + A.dir * (s + 1); }
+
+// block-wide scans over kFoldThreads lanes through LDS (Hillis-Steele)
+template <class T>
+__device__ inline T block_excl_sum(T v, T *sh, T *total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < kFoldThreads; off <<= 1) {
+    const T add = (tid >= off) ? sh[tid - off] : T(0);
+    __syncthreads();
+    sh[tid] += add;
+    __syncthreads();
+  }
+  const T incl = sh[tid];
+  if (total) *total = sh[kFoldThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+// approximate block sums and prefix ranges
+__global__ __launch_bounds__(kFoldThreads) void k_fold_sums(FoldArgs A, FoldWork W) {
+  __shared__ double sh[kFoldThreads];
+  const int p = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+  const int base = b * kFoldBlock + tid * kFoldPerThread;
+  double run = 0.0, mn = 0.0, mx = 0.0;
+  for (int j = 0; j < kFoldPerThread; j++) {
+    const int s = base + j;
+    if (s < A.nsteps) {
+      run += fold_incr(A.rate[p], A.i0, A.dir, s, A.dt);
+      mn = fmin(mn, run);
+      mx = fmax(mx, run);
+    }
+  }
+  double tot;
+  const double off = block_excl_sum(run, sh, &tot);
+  double lmn = off + mn, lmx = off + mx;
+  sh[tid] = lmn;
+  __syncthreads();
+  for (int o = kFoldThreads / 2; o > 0; o >>= 1) {
+    if (tid < o) sh[tid] = fmin(sh[tid], sh[tid + o]);
+    __syncthreads();
+  }
+  const double bmn = sh[0];
+  __syncthreads();
+  sh[tid] = lmx;
+  __syncthreads();
+  for (int o = kFoldThreads / 2; o > 0; o >>= 1) {
+    if (tid < o) sh[tid] = fmax(sh[tid], sh[tid + o]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    W.bsum[p * A.nblk + b] = tot;
+    W.bmin[p * A.nblk + b] = bmn;
+    W.bmax[p * A.nblk + b] = sh[0];
+  }
+}
+
+// approximate block start values and the error bound E
+__global__ __launch_bounds__(1024) void k_fold_plan(FoldArgs A, FoldWork W) {
+  __shared__ double sh[1024];
+  __shared__ double shm[1024];
+  const int p = blockIdx.x, tid = threadIdx.x;
+  const int per = (A.nblk + 1023) / 1024;
+  double loc = 0.0;
+  for (int j = 0; j < per; j++) {
+    const int b = tid * per + j;
+    if (b < A.nblk) loc += W.bsum[p * A.nblk + b];
+  }
+  sh[tid] = loc;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const double add = (tid >= off) ? sh[tid - off] : 0.0;
+    __syncthreads();
+    sh[tid] += add;
+    __syncthreads();
+  }
+  double run = A.x0[p] + (sh[tid] - loc);
+  double amax = fabs(A.x0[p]);
+  for (int j = 0; j < per; j++) {
+    const int b = tid * per + j;
+    if (b < A.nblk) {
+      W.xstart[p * A.nblk + b] = run;
+      amax = fmax(amax, fmax(fabs(run + W.bmin[p * A.nblk + b]), fabs(run + W.bmax[p * A.nblk + b])));
+      run += W.bsum[p * A.nblk + b];
+    }
+  }
+  shm[tid] = amax;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (tid < o) shm[tid] = fmax(shm[tid], shm[tid + o]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    int ex = 0;
+    frexp(shm[0] > 0.0 ? shm[0] : 1.0, &ex);
+    W.E[p] = ((double)A.nsteps + 2.0 * kFoldBlock) * ldexp(1.0, ex - 52);
+    W.status[p] = 0;
+  }
+}
+
+__device__ inline int binade_key(double a, double E) {
+  const double aa = fabs(a);
+  const double lo = aa - E, hi = aa + E;
+  if (!(lo > 0.0)) return -1;
+  int klo, khi;
+  frexp(lo, &klo);
+  frexp(hi, &khi);
+  if (klo != khi) return -1;
+  return (klo + 4096) * 2 + (a > 0.0 ? 1 : 0);
+}
+
+// Classify steps, build the block's item list (PLAN) or write the exact
+// fold values of run steps (WRITE, after the chain resolved the run starts).
+template <bool WRITE>
+__global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWork W) {
+  __shared__ double shd[kFoldThreads];
+  __shared__ long long shl[kFoldThreads];
+  __shared__ int shi[kFoldThreads];
+  __shared__ int shf[kFoldThreads];
+  const int p = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+  if (W.status[p]) return;
+  const int base = b * kFoldBlock + tid * kFoldPerThread;
+  const double E = W.E[p];
+  double d[kFoldPerThread];
+  double tsum = 0.0;
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) {
+    const int s = base + j;
+    d[j] = (s < A.nsteps) ? fold_incr(A.rate[p], A.i0, A.dir, s, A.dt) : 0.0;
+    tsum += d[j];
+  }
+  double dummy;
+  double a0 = W.xstart[p * A.nblk + b] + block_excl_sum(tsum, shd, &dummy);
+  // keys: >= 0 binade grid, -1 exact replay, -2 beyond the end
+  int key[kFoldPerThread];
+  long long r[kFoldPerThread];
+  int tie_mask = 0, head_mask = 0;
+  int prev_key = -3;
+  // key of the previous lane's last step, for head detection across lanes
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) {
+    const int s = base + j;
+    if (s >= A.nsteps) {
+      key[j] = -2;
+      r[j] = 0;
+      continue;
+    }
+    const double a1 = a0 + d[j];
+    const int k0 = binade_key(a0, E);
+    const int k1 = binade_key(a1, E);
+    if (k0 >= 0 && k0 == k1) {
+      key[j] = k0;
+      const int kexp = (k0 >> 1) - 4096;
+      const double f = ldexp(d[j], 53 - kexp);
+      const double fl = floor(f);
+      const double fr = f - fl;
+      long long ri = (long long)fl;
+      if (fr > 0.5) ri += 1;
+      else if (fr == 0.5) tie_mask |= (1 << j);
+      r[j] = ri;
+    } else {
+      key[j] = -1;
+      r[j] = 0;
+    }
+    a0 = a1;
+  }
+  // head flags need the previous lane's last key
+  shi[tid] = key[kFoldPerThread - 1];
+  __syncthreads();
+  prev_key = (tid > 0) ? shi[tid - 1] : -3;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) {
+    const int kprev = (j == 0) ? prev_key : key[j - 1];
+    const bool h = (key[j] != -2) && (key[j] == -1 || kprev == -1 || kprev == -3 || key[j] != kprev);
+    if (h) head_mask |= (1 << j);
+  }
+  // does the next lane start a new item (or lie beyond the end)?
+  shi[tid] = ((head_mask & 1) || key[0] == -2) ? 1 : 0;
+  __syncthreads();
+  const bool next_starts = (tid + 1 < kFoldThreads) ? (shi[tid + 1] != 0) : true;
+  __syncthreads();
+  // segmented carry of r across lanes: summary (has_head, tail_sum)
+  long long tail = 0;
+  int has_head = 0;
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) {
+    if (head_mask & (1 << j)) {
+      has_head = 1;
+      tail = 0;
+    }
+    tail += r[j];
+  }
+  shl[tid] = tail;
+  shf[tid] = has_head;
+  __syncthreads();
+  for (int off = 1; off < kFoldThreads; off <<= 1) {
+    long long v = 0;
+    int f = 0;
+    if (tid >= off) {
+      v = shl[tid - off];
+      f = shf[tid - off];
+    }
+    __syncthreads();
+    if (tid >= off && !shf[tid]) {
+      shl[tid] += v;
+      shf[tid] |= f;
+    }
+    __syncthreads();
+  }
+  const long long carry = (tid > 0) ? shl[tid - 1] : 0;
+  __syncthreads();
+  // item and tie numbering
+  const int nh = __popc(head_mask);
+  int nheads_tot;
+  const int item_base = block_excl_sum(nh, shi, &nheads_tot);
+  const int nt = __popc(tie_mask);
+  int nties_tot;
+  const int tie_base = block_excl_sum(nt, shi, &nties_tot);
+  if (!WRITE && tid == 0) {
+    W.nitems[p * A.nblk + b] = nheads_tot;
+    W.nties[p * A.nblk + b] = nties_tot;
+    if (nheads_tot > kFoldMaxItems || nties_tot > kFoldMaxTies) atomicExch(&W.status[p], 1);
+  }
+  if (nheads_tot > kFoldMaxItems || nties_tot > kFoldMaxTies) return;
+  FoldItem *items = W.items + ((size_t)p * A.nblk + b) * kFoldMaxItems;
+  FoldTie *ties = W.ties + ((size_t)p * A.nblk + b) * kFoldMaxTies;
+  long long P = carry;
+  int item = item_base - 1, tcount = tie_base;
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) {
+    if (key[j] == -2) break;
+    const int sl = tid * kFoldPerThread + j;
+    if (head_mask & (1 << j)) {
+      item++;
+      P = 0;
+      if (!WRITE) {
+        items[item].kind = (key[j] == -1) ? 1 : 0;
+        items[item].k = (key[j] >= 0) ? (key[j] >> 1) - 4096 : 0;
+        items[item].sgn = (key[j] >= 0) ? ((key[j] & 1) ? 1 : -1) : 0;
+        items[item].first = sl;
+        items[item].tie_first = tcount;
+      }
+    }
+    P += r[j];
+    const bool last_of_run =
+        (key[j] >= 0) && ((j + 1 < kFoldPerThread) ? (((head_mask >> (j + 1)) & 1) || key[j + 1] == -2) : next_starts);
+    if (tie_mask & (1 << j)) {
+      if (!WRITE) {
+        ties[tcount].step = sl;
+        ties[tcount].P = P;
+        ties[tcount].up = 0;
+      }
+      tcount++;
+    }
+    if (!WRITE) {
+      // the run total is the inclusive prefix at the run's last step
+      if (last_of_run) items[item].R = P;
+    } else if (key[j] >= 0) {
+      const FoldItem it = items[item];
+      long long ups = 0;
+      for (int q = it.tie_first; q < tcount; q++) ups += ties[q].up;
+      const long long m = it.m_start + P + ups;
+      A.out[p][out_index(A, base + j)] = ldexp((double)m, it.k - 53);
+    }
+  }
+}
+
+// Serial replay of the item lists: exact start of every run, resolution of
+// the exact ties, FP64 replay of the steps near binade boundaries.
+__global__ __launch_bounds__(256) void k_fold_chain(FoldArgs A, FoldWork W) {
+  const int p = blockIdx.x;
+  if (W.status[p] || threadIdx.x != 0) return;
+  double x = A.x0[p];
+  A.out[p][A.i0] = x;
+  for (int b = 0; b < A.nblk; b++) {
+    const int ni = W.nitems[p * A.nblk + b];
+    const int nt = W.nties[p * A.nblk + b];
+    FoldItem *items = W.items + ((size_t)p * A.nblk + b) * kFoldMaxItems;
+    FoldTie *ties = W.ties + ((size_t)p * A.nblk + b) * kFoldMaxTies;
+    const int blen = min(kFoldBlock, A.nsteps - b * kFoldBlock);
+    for (int it = 0; it < ni; it++) {
+      const FoldItem I = items[it];
+      const int last = ((it + 1 < ni) ? items[it + 1].first : blen) - 1;
+      if (I.kind == 1) {
+        for (int sl = I.first; sl <= last; sl++) {
+          const int s = b * kFoldBlock + sl;
+          x = x + fold_incr(A.rate[p], A.i0, A.dir, s, A.dt);
+          A.out[p][out_index(A, s)] = x;
+        }
+        continue;
+      }
+      int kx;
+      frexp(x, &kx);
+      if (kx != I.k || (x > 0.0 ? 1 : -1) != I.sgn) {
+        W.status[p] = 1;
+        return;
+      }
+      const long long m = (long long)ldexp(x, 53 - I.k);
+      items[it].m_start = m;
+      const int tend = (it + 1 < ni) ? items[it + 1].tie_first : nt;
+      long long ups = 0;
+      for (int q = I.tie_first; q < tend; q++) {
+        const long long val = m + ties[q].P + ups;
+        const int up = (int)(val & 1LL);
+        ties[q].up = up;
+        ups += up;
+      }
+      const long long mend = m + I.R + ups;
+      const long long am = mend < 0 ? -mend : mend;
+      if (am < (1LL << 52) || am >= (1LL << 53)) {
+        W.status[p] = 1;
+        return;
+      }
+      x = ldexp((double)mend, I.k - 53);
+    }
+  }
+}
+
+// Fallback: plain serial replay of the whole fold (used when the item
+// bookkeeping overflows; correct for any input).
+__global__ __launch_bounds__(256) void k_fold_serial(FoldArgs A, FoldWork W) {
+  __shared__ double buf[4096];
+  const int p = blockIdx.x;
+  if (!W.status[p]) return;
+  double x = A.x0[p];
+  if (threadIdx.x == 0) A.out[p][A.i0] = x;
+  for (int base = 0; base < A.nsteps; base += 4096) {
+    const int n = min(4096, A.nsteps - base);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) buf[j] = fold_incr(A.rate[p], A.i0, A.dir, base + j, A.dt);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int j = 0; j < n; j++) {
+        x = x + buf[j];
+        buf[j] = x;
+      }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) A.out[p][out_index(A, base + j)] = buf[j];
+    __syncthreads();
+  }
+}
+
+int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, int nblk_cap, int dir, int nsteps,
+               hipStream_t s) {
+  if (nsteps <= 0) return 0;
+  FoldArgs A;
+  for (int p = 0; p < 3; p++) {
+    A.rate[p] = B.rate[p];
+    A.out[p] = B.phase[p];
+    A.x0[p] = P.x0[p];
+  }
+  A.i0 = P.i0;
+  A.dir = dir;
+  A.nsteps = nsteps;
+  A.dt = P.dt;
+  A.nblk = (nsteps + kFoldBlock - 1) / kFoldBlock;
+  if (A.nblk > nblk_cap) return -1;
+  hipLaunchKernelGGL(k_fold_sums, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  hipLaunchKernelGGL(k_fold_plan, dim3(3), dim3(1024), 0, s, A, W);
+  hipLaunchKernelGGL(k_fold_local<false>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  hipLaunchKernelGGL(k_fold_chain, dim3(3), dim3(256), 0, s, A, W);
+  hipLaunchKernelGGL(k_fold_local<true>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  hipLaunchKernelGGL(k_fold_serial, dim3(3), dim3(256), 0, s, A, W);
+  return 0;
+}
+
+// phases after the plunge buffer stay at their plunge values (AAK.cc:265-269)
+__global__ void k_fold_fill(DenseParams P, DenseBuffers B) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + P.last_fold + 2;
+  if (i > P.N) return;
+  for (int p = 0; p < 3; p++) B.phase[p][i] = B.phase[p][P.ip];
+}
+
+void launch_fold_fill(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
+  const int n = P.N - (P.last_fold + 2) + 1;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fold_fill, dim3((n + 255) / 256), dim3(256), 0, s, P, B);
+}
+
+// ------------------------------------------------------------------ mode sum
+// J_{n-2..n+2}(x) by Miller's backward recurrence normalised with
+// J_0 + 2 sum J_2k = 1 (A&S 9.12); n = 1 returns (-J_1, J_0, J_1, J_2, J_3).
+__device__ inline void bessel_five(int n, double x, double *J) {
+  if (x == 0.0) {
+    for (int q = 0; q < 5; q++) J[q] = (n - 2 + q == 0) ? 1.0 : 0.0;
+    if (n == 1) { J[0] = 0.0; J[1] = 1.0; J[2] = 0.0; J[3] = 0.0; J[4] = 0.0; }
+    return;
+  }
+  const int nmax = n + 2;
+  const int nmin = (n >= 2) ? n - 2 : 0;
+  int top = nmax + 40 + (int)x;
+  top += (top & 1);
+  const double inv = 1.0 / x;
+  double jp1 = 0.0, j = 1.0e-280, sum = 0.0;
+  double w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0;
+  for (int k = top; k >= 1; k--) {
+    const int q = k - nmin;
+    if (q == 0) w0 = j;
+    else if (q == 1) w1 = j;
+    else if (q == 2) w2 = j;
+    else if (q == 3) w3 = j;
+    else if (q == 4) w4 = j;
+    if ((k & 1) == 0) sum += 2.0 * j;
+    const double jm1 = fma((2.0 * k) * inv, j, -jp1);
+    jp1 = j;
+    j = jm1;
+    if (fabs(j) > 1.0e250) {
+      j *= 1.0e-250; jp1 *= 1.0e-250; sum *= 1.0e-250;
+      w0 *= 1.0e-250; w1 *= 1.0e-250; w2 *= 1.0e-250; w3 *= 1.0e-250; w4 *= 1.0e-250;
+    }
+  }
+  if (nmin == 0) w0 = j;
+  sum += j;
+  const double norm = 1.0 / sum;
+  if (n == 1) {
+    J[1] = w0 * norm; J[2] = w1 * norm; J[3] = w2 * norm; J[4] = w3 * norm; J[0] = -J[2];
+  } else {
+    J[0] = w0 * norm; J[1] = w1 * norm; J[2] = w2 * norm; J[3] = w3 * norm; J[4] = w4 * norm;
+  }
+}
+
+__device__ inline double nrm2_scaled(const double *v) {
+  double scale = 0.0, ssq = 1.0;
+  for (int i = 0; i < 3; i++) {
+    const double x = v[i];
+    if (x != 0.0) {
+      const double ax = fabs(x);
+      if (scale < ax) {
+        ssq = 1.0 + ssq * (scale / ax) * (scale / ax);
+        scale = ax;
+      } else {
+        ssq += (ax / scale) * (ax / scale);
+      }
+    }
+  }
+  return scale * sqrt(ssq);
+}
+
+__global__ __launch_bounds__(256) void k_modesum(DenseParams P, DenseBuffers B) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.N + 2) return;
+  double *hI = B.h, *hII = B.h + (P.N + 2);
+  if (i >= P.N) {
+    B.t[i] = 0.0;
+    hI[i] = 0.0;
+    hII[i] = 0.0;
+    return;
+  }
+  const double ti = sample_time(P, i);
+  B.t[i] = ti;
+  if (i >= P.ip + P.ib) {
+    hI[i] = 0.0;
+    hII[i] = 0.0;
+    return;
+  }
+  const double e = B.e[i], v = B.v[i], M = B.M[i], S = B.S[i];
+  const double gim = B.phase[0][i], Phi = B.phase[1][i], alp = B.phase[2][i];
+  const double gimdot = B.rate[0][i];
+  double nu;
+  if (i == 0) {
+    nu = PN::drdm(v, e, P.coslam, S) / PN::dtdm(v, e, P.coslam, S) / (2. * kPi * M);
+  } else {
+    nu = B.rate[1][i] / 2. / kPi;
+  }
+  const double coslam = P.coslam, sinlam = P.sinlam;
+  const double cosqS = P.cosqS, sinqS = P.sinqS, cosqK = P.cosqK, sinqK = P.sinqK;
+  const double cosphiK = P.cosphiK, sinphiK = P.sinphiK, halfsqrt3 = P.halfsqrt3;
+  const double cosalp = cos(alp), sinalp = sin(alp);
+  const double cosqL = cosqK * coslam + sinqK * sinlam * cosalp;
+  const double sinqL = sqrt(1. - cosqL * cosqL);
+  const double phiLup = sinqK * sinphiK * coslam - cosphiK * sinlam * sinalp - cosqK * sinphiK * sinlam * cosalp;
+  const double phiLdown = sinqK * cosphiK * coslam + sinphiK * sinlam * sinalp - cosqK * cosphiK * sinlam * cosalp;
+  const double phiL = atan2(phiLup, phiLdown);
+  const double Ldotn = cosqL * cosqS + sinqL * sinqS * cos(phiL - P.phiS);
+  const double Ldotn2 = Ldotn * Ldotn;
+  const double Sdotn = P.Sdotn;
+  const double betaup = -Sdotn + coslam * Ldotn;
+  const double betadown = sinqS * P.sin_dphiKS * sinlam * cosalp + (cosqK * Sdotn - cosqS) / sinqK * sinlam * sinalp;
+  const double beta = atan2(betaup, betadown);
+  const double gam = 2. * (gim + beta);
+  const double cos2gam = cos(gam), sin2gam = sin(gam);
+  double cosorbphs = 0.0, FplusI, FcrosI, FplusII, FcrosII;
+  if (P.mich) {
+    const double orbphs = 2. * kPi * ti / kYear;
+    cosorbphs = cos(orbphs - P.phiS);
+    const double sinorbphs = sin(orbphs - P.phiS);
+    const double cosq = .5 * cosqS - halfsqrt3 * sinqS * cosorbphs;
+    const double phiw = orbphs + atan2(halfsqrt3 * cosqS + .5 * sinqS * cosorbphs, sinqS * sinorbphs);
+    const double psiup = .5 * cosqK - halfsqrt3 * sinqK * cos(orbphs - P.phiK) - cosq * P.psi_c0;
+    const double psidown = P.psi_c1 - halfsqrt3 * cos(orbphs) * P.psi_c2 - halfsqrt3 * sin(orbphs) * P.psi_c3;
+    const double psi = atan2(psiup, psidown);
+    const double cosq1 = .5 * (1. + cosq * cosq);
+    const double cos2phi = cos(2. * phiw), sin2phi = sin(2. * phiw);
+    const double cos2psi = cos(2. * psi), sin2psi = sin(2. * psi);
+    FplusI = cosq1 * cos2phi * cos2psi - cosq * sin2phi * sin2psi;
+    FcrosI = cosq1 * cos2phi * sin2psi + cosq * sin2phi * cos2psi;
+    FplusII = cosq1 * sin2phi * cos2psi + cosq * cos2phi * sin2psi;
+    FcrosII = cosq1 * sin2phi * sin2psi - cosq * cos2phi * cos2psi;
+  } else {
+    FplusI = P.ldc_c2psi;
+    FcrosI = -P.ldc_s2psi;
+    FplusII = P.ldc_s2psi;
+    FcrosII = P.ldc_c2psi;
+  }
+  const double Amp = pow(PN::omega_phi(v, e, coslam, S, M) * P.Mphys * kMsunSec, 2. / 3.) * P.zeta;
+  // rotation to the wave frame (KSParMap.cc:532-574); n-independent
+  double L[3], nxL[3], nxS[3];
+  const double ca = cos(alp), sa = sin(alp);
+  L[0] = P.cosiota_L * P.sin_thK * P.cos_phK + P.siniota_L * (sa * P.sin_phK - ca * P.cos_thK * P.cos_phK);
+  L[1] = P.cosiota_L * P.sin_thK * P.sin_phK - P.siniota_L * (sa * P.cos_phK + ca * P.cos_thK * P.sin_phK);
+  L[2] = P.cosiota_L * P.cos_thK + P.siniota_L * ca * P.sin_thK;
+  const double *n = P.n_vec, *Sv = P.S_vec;
+  nxL[0] = n[1] * L[2] - n[2] * L[1];
+  nxL[1] = n[2] * L[0] - n[0] * L[2];
+  nxL[2] = n[0] * L[1] - n[1] * L[0];
+  nxS[0] = n[1] * Sv[2] - n[2] * Sv[1];
+  nxS[1] = n[2] * Sv[0] - n[0] * Sv[2];
+  nxS[2] = n[0] * Sv[1] - n[1] * Sv[0];
+  const double norm = nrm2_scaled(nxL) * nrm2_scaled(nxS);
+  const double cosrot = (0.0 + nxL[0] * nxS[0] + nxL[1] * nxS[1] + nxL[2] * nxS[2]) / norm;
+  double sinrot = 0.0 + L[0] * nxS[0] + L[1] * nxS[1] + L[2] * nxS[2];
+  sinrot -= 0.0 + Sv[0] * nxL[0] + Sv[1] * nxL[1] + Sv[2] * nxL[2];
+  sinrot /= norm;
+  const double rot0 = 2. * cosrot * cosrot - 1., rot1 = cosrot * sinrot;
+  const double sq1me2 = sqrt(1 - e * e);
+  double accI = 0.0, accII = 0.0;
+  double J[5];
+  for (int nn = 1; nn <= P.nmodes; nn++) {
+    double nPhi;
+    if (P.mich) {
+      const double fn = nn * nu + gimdot / kPi;
+      const double Doppler = 2. * kPi * fn * kAUsec * sinqS * cosorbphs;
+      nPhi = nn * Phi + Doppler;
+    } else {
+      nPhi = nn * Phi;
+    }
+    bessel_five(nn, nn * e, J);
+    double snp, cnp;
+    sincos(nPhi, &snp, &cnp);
+    const double a = -nn * Amp * (J[0] - 2. * e * J[1] + 2. / nn * J[2] + 2. * e * J[3] - J[4]) * cnp;
+    const double bb = -nn * Amp * sq1me2 * (J[0] - 2. * J[2] + J[4]) * snp;
+    const double c = 2. * Amp * J[2] * cnp;
+    const double Ap0 = -(1. + Ldotn2) * (a * cos2gam - bb * sin2gam) + c * (1 - Ldotn2);
+    const double Ac0 = 2. * Ldotn * (bb * cos2gam + a * sin2gam);
+    const double Ap = Ap0 * rot0 + Ac0 * rot1;
+    const double Ac = Ap0 * (-rot1) + Ac0 * rot0;
+    double hnI, hnII;
+    if (P.mich) {
+      hnI = halfsqrt3 * (FplusI * Ap + FcrosI * Ac);
+      hnII = halfsqrt3 * (FplusII * Ap + FcrosII * Ac);
+    } else {
+      hnI = FplusI * Ap + FcrosI * Ac;
+      hnII = FplusII * Ap + FcrosII * Ac;
+    }
+    accI += hnI;
+    accII += hnII;
+  }
+  if (i > P.ip) {
+    const double tp = P.t_plunge, tz = P.t_zero;
+    const double den = exp((tp - tz) / (ti - tp) + (tp - tz) / (ti - tz)) + 1.;
+    accI = accI / den;
+    accII = accII / den;
+  }
+  hI[i] = accI;
+  hII[i] = accII;
+}
+
+void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
+  const int n = P.N + 2;
+  hipLaunchKernelGGL(k_modesum, dim3((n + 255) / 256), dim3(256), 0, s, P, B);
+}
+
+// ------------------------------------------------------------------ likelihood
+// <h w, d> and <h w, h w> over two channels; the weighted spectrum is written
+// back (the reference leaves h~ w in its FFT buffer).
+constexpr int kDotThreads = 256;
+constexpr int kDotBlocks = 1024;
+
+__global__ __launch_bounds__(kDotThreads) void k_weighted_dot(const double2 *hf, const double2 *data, const double *w,
+                                                              int nbins, double2 *hw_out, double *partials) {
+  __shared__ double s_dh[kDotThreads], s_hh[kDotThreads];
+  double dh = 0.0, hh = 0.0;
+  const int total = 2 * nbins;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const double2 h = hf[q];
+    const double ww = w[q];
+    const double2 x = make_double2(h.x * ww, h.y * ww);
+    const double2 dd = data[q];
+    dh += x.x * dd.x + x.y * dd.y;
+    hh += x.x * x.x + x.y * x.y;
+    if (hw_out) hw_out[q] = x;
+  }
+  s_dh[threadIdx.x] = dh;
+  s_hh[threadIdx.x] = hh;
+  __syncthreads();
+  for (int o = kDotThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s_dh[threadIdx.x] += s_dh[threadIdx.x + o];
+      s_hh[threadIdx.x] += s_hh[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = s_dh[0];
+    partials[2 * blockIdx.x + 1] = s_hh[0];
+  }
+}
+
+__global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *partials, int nparts, double *out) {
+  __shared__ double s_dh[kDotThreads], s_hh[kDotThreads];
+  double dh = 0.0, hh = 0.0;
+  for (int q = threadIdx.x; q < nparts; q += blockDim.x) {
+    dh += partials[2 * q];
+    hh += partials[2 * q + 1];
+  }
+  s_dh[threadIdx.x] = dh;
+  s_hh[threadIdx.x] = hh;
+  __syncthreads();
+  for (int o = kDotThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s_dh[threadIdx.x] += s_dh[threadIdx.x + o];
+      s_hh[threadIdx.x] += s_hh[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = 4.0 * s_dh[0];
+    out[1] = 4.0 * s_hh[0];
+  }
+}
+
+void launch_weighted_dot(const double2 *hf, const double2 *data, const double *w, int nbins, double2 *hw_out,
+                         double *partials, double *out, hipStream_t s) {
+  const int need = (2 * nbins + kDotThreads - 1) / kDotThreads;
+  const int nb = need < kDotBlocks ? need : kDotBlocks;
+  hipLaunchKernelGGL(k_weighted_dot, dim3(nb), dim3(kDotThreads), 0, s, hf, data, w, nbins, hw_out, partials);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nb, out);
+}
+
+}  // namespace aak

@@ -1,0 +1,108 @@
+"""Python face of the GPU AAK path, mirroring the reference Cython class
+``gpuAAK.GPUAAK`` (pygpuAAK/GPUAAK.pyx:50-93): same constructor arguments,
+same methods, same return shapes.  Calls go through the C ABI of
+include/gpuAAK.h; errors raise RuntimeError instead of terminating the process.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+def OmegaPhi(v, e, cosiota, s, M):
+    """|Omega_phi| of the AK orbit (gpuAAK.h:13)."""
+    return _lib.lib().aak_OmegaPhi(float(v), float(e), float(cosiota), float(s), float(M))
+
+
+class GPUAAK:
+    """GPUAAK(T_fit, init_length, length, init_dt, dt, LISA, backint)
+
+    Owns one device waveform slot of ``length`` samples, an rFFT plan and the
+    weighted data of both TDI-like channels (GPUAAK.pyx:56-59).
+    """
+
+    def __init__(self, T_fit, init_length, length, init_dt, dt, LISA, backint):
+        self.length = int(length)
+        self._h = C.c_void_p()
+        L = _lib.lib()
+        _lib.check(L.aak_create(float(T_fit), int(init_length), self.length, float(init_dt),
+                                float(dt), int(bool(LISA)), int(bool(backint)), C.byref(self._h)),
+                   "GPUAAK()")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().aak_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def close(self):
+        self.__del__()
+
+    # ------------------------------------------------------------------ API
+    def input_data(self, hI_f_, hII_f_, channel_ASDinv1_, channel_ASDinv2_):
+        """Weighted data and inverse ASDs of both channels (GPUAAK.pyx:61-67)."""
+        a = np.ascontiguousarray(hI_f_, dtype=np.complex128)
+        b = np.ascontiguousarray(hII_f_, dtype=np.complex128)
+        w1 = np.ascontiguousarray(channel_ASDinv1_, dtype=np.float64)
+        w2 = np.ascontiguousarray(channel_ASDinv2_, dtype=np.float64)
+        n = len(a)
+        if not (len(b) == n and len(w1) == n and len(w2) == n):
+            raise ValueError("input_data: all four arrays need the same length")
+        _lib.check(_lib.lib().aak_input_data(self._h, a.view(np.float64).ctypes.data_as(_lib._pd),
+                                             b.view(np.float64).ctypes.data_as(_lib._pd),
+                                             _lib.dptr(w1), _lib.dptr(w2), n), "input_data")
+
+    def gpu_gen_AAK(self, iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D):
+        """Generate the waveform on the device (GPUAAK.pyx:69-74)."""
+        _lib.check(_lib.lib().aak_gpu_gen_AAK(self._h, *(float(x) for x in (
+            iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D))), "gpu_gen_AAK")
+
+    def GetWaveform(self, is_Fourier=False):
+        """(t, hI, hII), each length+2 doubles (GPUAAK.pyx:76-83)."""
+        n = self.length + 2
+        t = np.zeros(n)
+        hI = np.zeros(n)
+        hII = np.zeros(n)
+        _lib.check(_lib.lib().aak_get_waveform(self._h, _lib.dptr(t), _lib.dptr(hI), _lib.dptr(hII)),
+                   "GetWaveform")
+        return (t, hI, hII)
+
+    def Likelihood(self):
+        """[4<h|d>, 4<h|h>] of the last waveform (GPUAAK.pyx:85-88)."""
+        like = np.zeros(2)
+        _lib.check(_lib.lib().aak_likelihood(self._h, _lib.dptr(like)), "Likelihood")
+        return like
+
+    def WaveformThroughLikelihood(self, iota, s, p, e, M, mu, gamma, psi, alph, theta_S,
+                                  phi_S, theta_K, phi_K, D, return_waveform=False):
+        """gpu_gen_AAK then Likelihood (GPUAAK.pyx:90-93)."""
+        self.gpu_gen_AAK(iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D)
+        if return_waveform:
+            return (0.0, 0.0)
+        return self.Likelihood()
+
+    # ------------------------------------------------------------ extensions
+    def set_nmax(self, n_max):
+        """Override the harmonic count (0 restores (int)(30 e0), min 4)."""
+        _lib.check(_lib.lib().aak_set_nmax(self._h, int(n_max)), "set_nmax")
+
+    def info(self):
+        """Diagnostics of the last gpu_gen_AAK."""
+        buf = np.zeros(10, dtype=np.int32)
+        _lib.lib().aak_get_info(self._h, _lib.iptr(buf), 10)
+        keys = ("steps", "nmodes", "i_plunge", "i_buffer", "K", "i0", "fold_fallbacks",
+                "nonpositive_phidot", "j0", "plunged")
+        return dict(zip(keys, (int(x) for x in buf)))
+
+    def dense_arrays(self):
+        """Device dense arrays of the last waveform (test hook)."""
+        L = self.length + 1
+        out = np.zeros(10 * L)
+        _lib.check(_lib.lib().aak_test_get_dense(self._h, _lib.dptr(out)), "dense_arrays")
+        out = out.reshape(10, L)
+        names = ("e", "v", "M", "S", "gimdot", "Phidot", "alpdot", "gim", "Phi", "alp")
+        return dict(zip(names, out))

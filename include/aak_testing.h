@@ -1,0 +1,38 @@
+/*
+ * aak_testing.h -- host-only entry points of libaak_hip.so used by the test
+ * suite to check the host half of the waveform (sparse inspiral, frequency
+ * map, knots, spline table) without a GPU.  No counterpart in the reference;
+ * callers are tests that compare against the CPU restatement.
+ */
+#ifndef EMRI_AAK_TESTING_H
+#define EMRI_AAK_TESTING_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Runs the host stage for params[14] (reference order).  traj_out[5*100]:
+ * e_traj | v_map | M_map | s_map | dt_map.  knots_out[5*cap]: t | e | v | M | S.
+ * info[0..15]: steps, K, i0, j0, j_min, j_max, j_max_temp, j_start, j_end,
+ * plunged, points, i_plunge, i_buffer, nmodes, last_fold, map iterations sum.
+ * dparams[0..3]: Phi0, zeta, t_plunge, t_zero.  Returns K or < 0. */
+int aak_test_host_plan(const double *params, double T_fit, int length, double dt, int LISA,
+                       int backint, int n_max, double *traj_out, double *knots_out, int cap,
+                       int *info, double *dparams);
+
+/* Natural-spline table of the host stage: packed[17*K] as uploaded to the device. */
+int aak_test_spline_table(const double *params, double T_fit, int length, double dt, int LISA,
+                          int backint, double *packed, int cap17);
+
+/* One evaluation of the inspiral rates (dp/dt, de/dt, dcos(iota)/dt). */
+int aak_test_inspiral_rates(double p, double e, double cosi, double a, double *out);
+
+/* Copies the dense arrays of the last aak_gpu_gen_AAK to the host:
+ * out[10*(length+1)] = e | v | M | S | gimdot | Phidot | alpdot | gim | Phi | alp. */
+struct aak_handle_s;
+int aak_test_get_dense(struct aak_handle_s *h, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,75 @@
+/*
+ * gpuAAK.h -- C ABI of the MI355X AAK waveform + LISA likelihood library
+ * (emri_kludge_suite_amd/libaak_hip.so).
+ *
+ * The entry points are what a binding of the reference's GPU path needs: the
+ * GPUAAK class of include/AAK_manager.hh (constructor, input_data,
+ * gpu_gen_AAK, Likelihood, GetWaveform, destructor; bound by the Cython
+ * pygpuAAK/GPUAAK.pyx:50-93) and the two free functions of include/gpuAAK.h
+ * (OmegaPhi, PNevolution).  All functions are plain C: pointers, sizes and
+ * status codes; no exceptions cross the boundary.  Host arrays are copied,
+ * never retained.  Calls on one handle must not overlap; separate handles are
+ * independent.  Status: 0 = success, < 0 = error (message via aak_last_error).
+ *
+ * Semantics follow the reference CPU AAK (AAK.cc): natural spline, Kepler
+ * mean anomaly, unflipped theta_S, Phidot guard and the reference's LISA=false
+ * response.
+ */
+#ifndef EMRI_AAK_GPUAAK_H
+#define EMRI_AAK_GPUAAK_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct aak_handle_s *aak_handle;
+
+/* GPUAAK::GPUAAK(T_fit, init_length, length, init_dt, dt, LISA, backint)
+ * (AAK_manager.hh:106-112, AAK_manager.cu:42-133).  init_length/init_dt are
+ * accepted for signature compatibility; knot storage is sized per waveform. */
+int aak_create(double T_fit, int init_length, int length, double init_dt, double dt,
+               int LISA, int backint, aak_handle *out);
+
+/* GPUAAK::~GPUAAK (AAK_manager.cu:460-491). */
+int aak_destroy(aak_handle h);
+
+/* GPUAAK::input_data(hI_f, hII_f, ASDinv1, ASDinv2, len) (AAK_manager.cu:135-152).
+ * hI_f/hII_f: len complex values as interleaved (re, im) doubles, already
+ * multiplied by the weights; ASDinv*: len weights; len must be length/2+1. */
+int aak_input_data(aak_handle h, const double *hI_f, const double *hII_f,
+                   const double *ASDinv1, const double *ASDinv2, int len);
+
+/* GPUAAK::gpu_gen_AAK(iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S,
+ * theta_K, phi_K, D) (AAK_manager.cu:154-260): waveform into device memory. */
+int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, double M,
+                    double mu, double gamma, double psi, double alph, double theta_S,
+                    double phi_S, double theta_K, double phi_K, double D);
+
+/* GPUAAK::Likelihood(like_out[2]) (AAK_manager.cu:352-452):
+ * like_out = {4 <h w, d w>, 4 <h w, h w>} over both channels. */
+int aak_likelihood(aak_handle h, double *like_out);
+
+/* GPUAAK::GetWaveform(t, hI, hII) (AAK_manager.cu:454-458): length+2 doubles
+ * each; time domain after aak_gpu_gen_AAK, the weighted spectrum (interleaved)
+ * after aak_likelihood, as in the reference. */
+int aak_get_waveform(aak_handle h, double *t, double *hI, double *hII);
+
+/* OmegaPhi(v, e, cosiota, s, M) (gpuAAK.h:13, gpuAAK.cc:17-26). */
+double aak_OmegaPhi(double v, double e, double cosiota, double s, double M);
+
+/* Harmonic-count override; no counterpart in the reference, whose count is
+ * (int)(30 e0), 4 below e0 = 0.135 (AAK.cc:582-583).  n_max <= 0 restores it. */
+int aak_set_nmax(aak_handle h, int n_max);
+
+/* Diagnostics of the last aak_gpu_gen_AAK; no counterpart in the reference.
+ * info[0..9] = steps, nmodes, i_plunge, i_buffer, knots K, i0, fold fallbacks,
+ * non-positive Phidot count, j0, plunged.  Returns the number written. */
+int aak_get_info(aak_handle h, int *info, int n);
+
+/* Message of the last failed call on this thread. */
+const char *aak_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

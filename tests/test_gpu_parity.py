@@ -1,0 +1,137 @@
+"""GPU parity: the HIP path against the CPU oracle on identical inputs.
+
+Tolerances (north star): h_I/h_II relative L2 <= 1e-10 in plain FP64; inner
+product relative error <= 1e-9.  The host stage is bit-identical to the oracle
+(checked on the CPU in test_host_parity.py), so the dense arrays are compared
+bit for bit as well: the spline values and the phase rates must be identical,
+and the phase folds must reproduce the sequential FP64 rounding exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def _config1(seed=1):
+    rng = np.random.default_rng(seed)
+    return [0.5, 0.5, 8.0, 0.2, 1e6, 10.0, rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi),
+            rng.uniform(0, 2 * np.pi), np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi),
+            np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi), 1.0]
+
+
+def _config2(rng):
+    while True:
+        pars = [0.5, rng.uniform(0, 0.95), rng.uniform(7, 15), rng.uniform(1e-3, 0.5),
+                10 ** rng.uniform(5, 7), rng.uniform(5, 50), rng.uniform(0, 2 * np.pi),
+                rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi), np.arccos(rng.uniform(-1, 1)),
+                rng.uniform(0, 2 * np.pi), np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi), 1.0]
+        yield pars
+
+
+def _rel_l2(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / max(np.sum(b ** 2), 1e-300)))
+
+
+def _compare(pars, N, dt, LISA=True, backint=False, n_max=0, label=""):
+    from emri_kludge_suite_amd import GPUAAK
+    print("case %s N=%d dt=%g LISA=%d backint=%d n_max=%d pars=%s" % (label, N, dt, LISA, backint, n_max,
+                                                                     np.array2string(np.array(pars), precision=6)), flush=True)
+    ref = ol.o_aak_wave(pars, length=N, dt=dt, LISA=LISA, backint=backint, n_max=n_max, want_dense=True)
+    g = GPUAAK(1.0, N // 10, N, 100.0, dt, LISA, backint)
+    if n_max:
+        g.set_nmax(n_max)
+    g.gpu_gen_AAK(*pars)
+    info = g.info()
+    t, hI, hII = g.GetWaveform()
+    dense = g.dense_arrays()
+    out = {"info": info}
+    for name in ("e", "v", "M", "S"):
+        out["max_abs_" + name] = float(np.max(np.abs(dense[name] - ref["dense_" + name])))
+    for name in ("gim", "Phi", "alp"):
+        d = dense[name] - ref["dense_" + name]
+        out["phase_mismatch_" + name] = int(np.count_nonzero(d))
+        out["phase_maxabs_" + name] = float(np.max(np.abs(d)))
+    out["relL2_hI"] = _rel_l2(hI[:N], ref["hI"])
+    out["relL2_hII"] = _rel_l2(hII[:N], ref["hII"])
+    out["t_equal"] = bool(np.array_equal(t[:N], ref["t"]))
+    print(out, flush=True)
+    assert info["nmodes"] == ref["nmodes"] and info["i_plunge"] == ref["i_plunge"]
+    assert info["i_buffer"] == ref["i_buffer"] and info["K"] == ref["K"]
+    return out, g, ref
+
+
+def test_config1_waveform_parity():
+    out, g, ref = _compare(_config1(), 2 ** 18, 10.0, label="config1")
+    for name in ("e", "v", "M", "S"):
+        assert out["max_abs_" + name] == 0.0, name
+    assert out["t_equal"]
+    for name in ("gim", "Phi", "alp"):
+        assert out["phase_mismatch_" + name] == 0, name
+    assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
+
+
+def test_config1_nmax30_parity():
+    out, g, ref = _compare(_config1(), 2 ** 18, 10.0, n_max=30, label="config1-n30")
+    assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
+
+
+def test_no_lisa_response_parity():
+    out, g, ref = _compare(_config1(3), 2 ** 16, 10.0, LISA=False, label="ldc")
+    assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
+
+
+@pytest.mark.timeout(200)
+def test_backint_plunging_parity():
+    # mu=10, M=1e6, p0=8 plunges inside a year of dt=10 s samples (reference tutorial case)
+    pars = [np.pi / 6, 0.5, 8.0, 0.1, 1e6, 10.0, 1.0, 2.0, 3.0, np.pi / 3, 0.0, np.pi / 4, np.pi / 4, 5.0]
+    out, g, ref = _compare(pars, 2 ** 22, 8.0, backint=True, label="backint-plunge")
+    assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
+
+
+@pytest.mark.timeout(400)
+def test_config2_random_parity():
+    rng = np.random.default_rng(2)
+    gen = _config2(rng)
+    done = 0
+    while done < 2:
+        pars = next(gen)
+        try:
+            ol.o_aak_wave(pars, length=2 ** 12, dt=15.0, LISA=True)
+        except RuntimeError:
+            continue
+        out, g, ref = _compare(pars, 2 ** 21, 15.0, n_max=50, label="config2-%d" % done)
+        assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
+        done += 1
+
+
+def test_likelihood_parity():
+    from emri_kludge_suite_amd import GPUAAK
+    from emri_kludge_suite_amd.LISAnoise import LISA_Noise, generate_noise_frequencies
+    N, dt = 2 ** 18, 10.0
+    pars = _config1()
+    ref = ol.o_aak_wave(pars, length=N, dt=dt)
+    freqs = generate_noise_frequencies(N * dt, 1 / dt)
+    freqs[0] = freqs[1] / 10.0
+    w = 1.0 / np.sqrt(LISA_Noise(freqs, dur=1)) * np.sqrt(dt / N)
+    rng = np.random.default_rng(7)
+    pars2 = list(pars)
+    pars2[7] += 0.3
+    ref2 = ol.o_aak_wave(pars2, length=N, dt=dt)
+    dI = np.fft.rfft(ref2["hI"]) * w
+    dII = np.fft.rfft(ref2["hII"]) * w
+    g = GPUAAK(1.0, N // 10, N, 100.0, dt, True, False)
+    g.input_data(dI, dII, w, w)
+    like = g.WaveformThroughLikelihood(*pars)
+    hI = np.fft.rfft(ref["hI"]) * w
+    hII = np.fft.rfft(ref["hII"]) * w
+    dh = 4 * np.sum(np.real(np.conj(hI) * dI) + np.real(np.conj(hII) * dII))
+    hh = 4 * np.sum(np.abs(hI) ** 2 + np.abs(hII) ** 2)
+    print("likelihood", like, dh, hh, flush=True)
+    assert abs(like[0] - dh) <= 1e-9 * abs(hh)
+    assert abs(like[1] - hh) <= 1e-9 * abs(hh)
+    # GetWaveform after Likelihood returns the weighted spectrum
+    t, sI, sII = g.GetWaveform()
+    spec = sI[0::2] + 1j * sI[1::2]
+    assert np.max(np.abs(spec - hI)) <= 1e-9 * np.max(np.abs(hI))
