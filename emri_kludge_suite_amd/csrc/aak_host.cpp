@@ -136,7 +136,7 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
   const double *dt_map = tr.dt_map;
   const double dt_large = dt_map[1];
   const int j0 = (int)((vlength * timestep - dt_map[steps - 1]) / dt_large + steps + 1);
-  if (j0 <= steps || j0 > 400000000) {
+  if (j0 < steps || j0 > 400000000) {
     *err = "knot count out of range";
     return -3;
   }

@@ -69,7 +69,7 @@ int o_pn_knots(double timestep, int vlength, const double *par,
   const int j0 = o_pn_j0(timestep, vlength, dt_map, steps);
   int j_start = j0, j_min = j0, j_end = 2 * j0 - 1, j_max = j_end, j_max_temp = j_max;
   double t_end = vlength * timestep;
-  if (j0 <= steps || j0 > 400000000) { o_set_error("knot count out of range"); return -3; }
+  if (j0 < steps || j0 > 400000000) { o_set_error("knot count out of range"); return -3; }
   double *e_AK = calloc((size_t)2 * j0, sizeof(double));
   double *v_AK = calloc((size_t)2 * j0, sizeof(double));
   double *t_fit = calloc((size_t)2 * j0, sizeof(double));
@@ -461,7 +461,7 @@ int o_aak_wave(const double *pars, double T_fit, int length, double dt,
   if (n_max > 0) nmodes = n_max;
   const double zeta = par[0] / D / O_GPC;
   const int j0 = o_pn_j0(dt, length, dt_map, steps);
-  if (j0 <= steps || j0 > 400000000) { o_set_error("knot count out of range"); return -3; }
+  if (j0 < steps || j0 > 400000000) { o_set_error("knot count out of range"); return -3; }
   const int cap = 2 * j0 + 8;
   double *kn = malloc(sizeof(double) * 5 * (size_t)cap);
   int info[9];
