@@ -1,0 +1,182 @@
+"""Benchmark: AAK likelihoods per second on one MI355X.
+
+Workload (production-size plain-FP64 case): 1-yr-class LISA waveforms,
+N = 2^21 samples at dt = 15 s (0.997 yr), harmonics n = 1..50, parameters drawn
+from the production priors (log10 M ~ U[5,7], mu ~ U[5,50], a ~ U[0,0.95],
+e0 ~ U[1e-3,0.5], p0 ~ U[7,15], iota0 = 0.5, angles uniform on the sphere),
+noise-weighted inner products against a seeded injected signal under the
+analytic LISA instrument + confusion PSD (1 yr) on the rFFT grid.
+
+One step = one likelihood of the next template: host inspiral + knots, device
+spline/phasing/exact folds/mode sum, rFFT and the fused weighted reduction.
+The data (weighted injection) is resident in HBM before timing starts.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "AAK likelihoods/sec, 1-yr dt=15s, mixed vs plain FP64 at mismatch<=1e-6"
+N_SAMPLES = 2 ** 21
+DT = 15.0
+N_MAX = 50
+MSUN_SEC = 6.67259e-11 * 1.98892e30 / 299792458.0 ** 3
+FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
+
+
+def draw_params(rng):
+    """One draw from the production priors (14 parameters, reference order)."""
+    return [0.5, rng.uniform(0, 0.95), rng.uniform(7, 15), rng.uniform(1e-3, 0.5),
+            10 ** rng.uniform(5, 7), rng.uniform(5, 50), rng.uniform(0, 2 * np.pi),
+            rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi), np.arccos(rng.uniform(-1, 1)),
+            rng.uniform(0, 2 * np.pi), np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi), 1.0]
+
+
+def valid_draws(seed, count):
+    """First `count` draws whose host stage succeeds (p0 outside the LSO etc.)."""
+    from emri_kludge_suite_amd import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(seed)
+    out = []
+    cap = 4_000_000
+    traj = np.zeros(500)
+    kn = np.zeros(5 * cap)
+    info = np.zeros(16, dtype=np.int32)
+    dp = np.zeros(4)
+    while len(out) < count:
+        pars = draw_params(rng)
+        pv = np.array(pars)
+        K = L.aak_test_host_plan(_lib.dptr(pv), 1.0, N_SAMPLES, DT, 1, 0, N_MAX, _lib.dptr(traj),
+                                 _lib.dptr(kn), cap, _lib.iptr(info), _lib.dptr(dp))
+        if K > 0:
+            out.append(pars)
+    return out
+
+
+def weights():
+    from emri_kludge_suite_amd.LISAnoise import LISA_Noise, generate_noise_frequencies
+    freqs = generate_noise_frequencies(N_SAMPLES * DT, 1 / DT)
+    freqs[0] = freqs[1] / 10.0
+    return 1.0 / np.sqrt(LISA_Noise(freqs, dur=1)) * np.sqrt(DT / N_SAMPLES)
+
+
+def cpu_baseline(templates, w, data, budget_s):
+    """The CPU oracle (plain C restatement of the reference CPU AAK, one core)
+    timed on the same workload: waveform + numpy rFFT + weighted sums."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    t0 = time.perf_counter()
+    n = 0
+    for pars in templates:
+        r = ol.o_aak_wave(pars, length=N_SAMPLES, dt=DT, LISA=True, backint=False, n_max=N_MAX)
+        hI = np.fft.rfft(r["hI"]) * w
+        hII = np.fft.rfft(r["hII"]) * w
+        _ = 4 * np.sum(np.real(np.conj(hI) * data[0]) + np.real(np.conj(hII) * data[1]))
+        _ = 4 * np.sum(np.abs(hI) ** 2 + np.abs(hII) ** 2)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "likelihoods/s", "cores": 1, "kind": "port",
+            "sample": "%d of the template draws (full N=2^21, n=1..50), %.1f s of one core" % (n, dt)}
+
+
+def copy_probe():
+    """Plain copy of 2^27 doubles (1 GiB read + 1 GiB write), five times: the
+    reference point that scales the FETCH_SIZE/WRITE_SIZE counters to bytes."""
+    import torch
+    x = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": 2 ** 30, "launches": 5}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--templates", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+              "(waveform batches shard across GPUs without communication)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from emri_kludge_suite_amd import GPUAAK
+
+    injection = valid_draws(2, 1)[0]
+    templates = valid_draws(3, args.templates)
+    w = weights()
+    g = GPUAAK(1.0, N_SAMPLES // 100, N_SAMPLES, 100.0, DT, True, False)
+    g.set_nmax(N_MAX)
+    g.gpu_gen_AAK(*injection)
+    _, hI, hII = g.GetWaveform()
+    dI = np.fft.rfft(hI[:N_SAMPLES]) * w
+    dII = np.fft.rfft(hII[:N_SAMPLES]) * w
+    g.input_data(dI, dII, w, w)
+
+    for k in range(args.warmup):
+        g.WaveformThroughLikelihood(*templates[k % len(templates)])
+    g.set_profiling(True)
+    stage = np.zeros(6)
+    flops = 0.0
+    modesum_ms = 0.0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        pars = templates[(args.warmup + k) % len(templates)]
+        g.WaveformThroughLikelihood(*pars)
+        tm = g.timings()
+        inf = g.info()
+        stage += np.array([tm[x] for x in ("host", "rates", "fold", "modesum", "rfft", "reduce")])
+        n_active = min(N_SAMPLES, inf["i_plunge"] + inf["i_buffer"])
+        flops += n_active * (190.0 * inf["nmodes"] + 460.0)
+        modesum_ms += tm["modesum"]
+    elapsed = time.perf_counter() - t0
+    value = args.steps / elapsed
+    achieved = flops / (modesum_ms * 1e-3) / 1e12
+    stage /= args.steps
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "likelihoods/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded prior draws, injection seed 2, templates seed 3)",
+        "config": {"workload": "1-yr LISA AAK likelihood, N=2^21, dt=15 s, n=1..50, plain FP64, one waveform per call",
+                   "N": N_SAMPLES, "dt": DT, "n_max": N_MAX, "templates": len(templates), "psd": "LISA_Noise(dur=1)"},
+        "stages_ms": dict(zip(("host", "rates", "fold", "modesum", "rfft", "reduce"), [round(x, 4) for x in stage])),
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_modesum", "convention": "N_active*(190*n_modes+460) FLOP per launch"},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(templates, w, (dI, dII), args.cpu_seconds)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -45,6 +45,8 @@ def lib():
         "aak_test_spline_table": (_i, [_pd, _d, _i, _d, _i, _i, _pd, _i]),
         "aak_test_inspiral_rates": (_i, [_d, _d, _d, _d, _pd]),
         "aak_test_get_dense": (_i, [C.c_void_p, _pd]),
+        "aak_set_profiling": (_i, [C.c_void_p, _i]),
+        "aak_get_timings": (_i, [C.c_void_p, _pd, _i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -67,6 +68,14 @@ struct aak_handle_s {
   int have_data = 0;
   int state = 0;  // 0 empty, 1 time domain, 2 weighted spectrum
   int info[10] = {0};
+  // stage timing on the handle's stream: rates, folds, mode sum, rFFT, reduction
+  int profiling = 0;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double stage_ms[5] = {0, 0, 0, 0, 0};
+  double host_ms = 0.0;
+  void mark(int k) {
+    if (profiling) (void)hipEventRecord(ev[k], stream);
+  }
   DenseBuffers bufs() const {
     DenseBuffers B;
     const size_t L = (size_t)N + 1;
@@ -159,6 +168,8 @@ static void free_handle(aak_handle h) {
                   h->d_data, h->d_w, h->d_partials, h->d_like, h->d_work};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (hipEvent_t &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -233,7 +244,9 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   SourceParams sp{iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D};
   WaveformPlan plan;
   std::string err;
+  auto t0 = std::chrono::steady_clock::now();
   int rc = plan_waveform(h->cfg, sp, &plan, &err);
+  h->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (rc < 0) return fail(rc, err);
   const DenseParams &P = plan.P;
   const size_t need = plan.packed.size();
@@ -247,7 +260,9 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   HIPCHK(hipMemcpyAsync(h->d_knots, plan.packed.data(), sizeof(double) * need, hipMemcpyHostToDevice, h->stream));
   HIPCHK(hipMemsetAsync(h->d_flags, 0, sizeof(int) * 4, h->stream));
   const DenseBuffers B = h->bufs();
+  h->mark(0);
   launch_rates(P, B, h->stream);
+  h->mark(1);
   int flags[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(flags, h->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -265,9 +280,18 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
     fallbacks += st[0] + st[1] + st[2];
   }
   launch_fold_fill(P, B, h->stream);
+  h->mark(2);
   launch_modesum(P, B, h->stream);
+  h->mark(3);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->profiling) {
+    float ms = 0.f;
+    for (int k = 0; k < 3; k++) {
+      HIPCHK(hipEventElapsedTime(&ms, h->ev[k], h->ev[k + 1]));
+      h->stage_ms[k] = ms;
+    }
+  }
   h->state = 1;
   int *I = h->info;
   I[0] = plan.traj.steps;
@@ -289,10 +313,20 @@ int aak_likelihood(aak_handle h, double *like_out) {
   if (h->state != 1) return fail(-2, "Likelihood needs a waveform from gpu_gen_AAK");
   void *in[1] = {h->d_h};
   void *outb[1] = {h->d_hf};
+  h->mark(3);
   if (rocfft_execute(h->plan, in, outb, h->exec) != rocfft_status_success) return fail(-101, "rocfft execute");
+  h->mark(4);
   launch_weighted_dot(h->d_hf, h->d_data, h->d_w, h->nbins, h->d_hf, h->d_partials, h->d_like, h->stream);
+  h->mark(5);
   HIPCHK(hipMemcpyAsync(like_out, h->d_like, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->profiling) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev[3], h->ev[4]));
+    h->stage_ms[3] = ms;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev[4], h->ev[5]));
+    h->stage_ms[4] = ms;
+  }
   h->state = 2;
   return 0;
 }
@@ -311,6 +345,22 @@ int aak_get_waveform(aak_handle h, double *t, double *hI, double *hII) {
   }
   HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
+}
+
+int aak_set_profiling(aak_handle h, int on) {
+  if (!h) return fail(-1, "null handle");
+  if (on && !h->ev[0])
+    for (hipEvent_t &e : h->ev) HIPCHK(hipEventCreate(&e));
+  h->profiling = on ? 1 : 0;
+  return 0;
+}
+
+int aak_get_timings(aak_handle h, double *ms, int n) {
+  if (!h) return fail(-1, "null handle");
+  const double v[6] = {h->host_ms, h->stage_ms[0], h->stage_ms[1], h->stage_ms[2], h->stage_ms[3], h->stage_ms[4]};
+  const int m = n < 6 ? n : 6;
+  for (int i = 0; i < m; i++) ms[i] = v[i];
+  return m;
 }
 
 int aak_get_info(aak_handle h, int *info, int n) {

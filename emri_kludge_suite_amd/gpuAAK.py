@@ -98,6 +98,17 @@ class GPUAAK:
                 "nonpositive_phidot", "j0", "plunged")
         return dict(zip(keys, (int(x) for x in buf)))
 
+    def set_profiling(self, on=True):
+        """Record HIP events around each stage on the library's stream."""
+        _lib.check(_lib.lib().aak_set_profiling(self._h, int(bool(on))), "set_profiling")
+
+    def timings(self):
+        """Stage times (ms) of the last gpu_gen_AAK / Likelihood."""
+        buf = np.zeros(6)
+        _lib.lib().aak_get_timings(self._h, _lib.dptr(buf), 6)
+        keys = ("host", "rates", "fold", "modesum", "rfft", "reduce")
+        return dict(zip(keys, (float(x) for x in buf)))
+
     def dense_arrays(self):
         """Device dense arrays of the last waveform (test hook)."""
         L = self.length + 1

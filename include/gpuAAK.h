@@ -66,6 +66,13 @@ int aak_set_nmax(aak_handle h, int n_max);
  * non-positive Phidot count, j0, plunged.  Returns the number written. */
 int aak_get_info(aak_handle h, int *info, int n);
 
+/* Stage timing; no counterpart in the reference (its timers are commented out,
+ * AAK_manager.cu:170-194).  aak_set_profiling(h, 1) records HIP events on the
+ * handle's stream; aak_get_timings returns ms[0..5] = host stage (wall clock),
+ * rates+spline, phase folds, mode sum, rFFT, weighted reduction. */
+int aak_set_profiling(aak_handle h, int on);
+int aak_get_timings(aak_handle h, double *ms, int n);
+
 /* Message of the last failed call on this thread. */
 const char *aak_last_error(void);
 

@@ -309,6 +309,27 @@ static int o_pn_dense(double timestep, int vlength, const double *par, int backi
   return 0;
 }
 
+/* Peters-Mathews harmonic coefficients a_n, b_n, c_n of the AK mode sum
+ * (AAK.cc:493-504) at harmonic n, eccentricity e, amplitude Amp and phase nPhi. */
+void o_mode_abc(int n, double e, double Amp, double nPhi, double *abc) {
+  double J[5], Jb[4];
+  const double ne = n * e;
+  if (n == 1) {
+    o_bessel_jn_array(0, 3, ne, Jb);
+    J[0] = -Jb[1];
+    J[1] = Jb[0];
+    J[2] = Jb[1];
+    J[3] = Jb[2];
+    J[4] = Jb[3];
+  } else {
+    o_bessel_jn_array(n - 2, n + 2, ne, J);
+  }
+  const double cnp = cos(nPhi), snp = sin(nPhi);
+  abc[0] = -n * Amp * (J[0] - 2. * e * J[1] + 2. / n * J[2] + 2. * e * J[3] - J[4]) * cnp;
+  abc[1] = -n * Amp * sqrt(1 - e * e) * (J[0] - 2. * J[2] + J[4]) * snp;
+  abc[2] = 2. * Amp * J[2] * cnp;
+}
+
 int o_waveform_from_dense(const double *dense, int vlength, double timestep,
                           int nmodes, double zeta, double M_phys, double lam,
                           double qS, double phiS, double qK, double phiK,
@@ -385,21 +406,9 @@ int o_waveform_from_dense(const double *dense, int vlength, double timestep,
       } else {
         nPhi = n * Phi;
       }
-      const double ne = n * e;
-      if (n == 1) {
-        o_bessel_jn_array(0, 3, ne, Jb);
-        J[0] = -Jb[1];
-        J[1] = Jb[0];
-        J[2] = Jb[1];
-        J[3] = Jb[2];
-        J[4] = Jb[3];
-      } else {
-        o_bessel_jn_array(n - 2, n + 2, ne, J);
-      }
-      const double cnp = cos(nPhi), snp = sin(nPhi);
-      const double a = -n * Amp * (J[0] - 2. * e * J[1] + 2. / n * J[2] + 2. * e * J[3] - J[4]) * cnp;
-      const double b = -n * Amp * sqrt(1 - e * e) * (J[0] - 2. * J[2] + J[4]) * snp;
-      const double c = 2. * Amp * J[2] * cnp;
+      double abc[3];
+      o_mode_abc(n, e, Amp, nPhi, abc);
+      const double a = abc[0], b = abc[1], c = abc[2];
       const double Aplus0 = -(1. + Ldotn2) * (a * cos2gam - b * sin2gam) + c * (1 - Ldotn2);
       const double Acros0 = 2. * Ldotn * (b * cos2gam + a * sin2gam);
       const double Aplus = Aplus0 * rot[0] + Acros0 * rot[1];
