@@ -121,6 +121,10 @@ int o_aak_wave(const double *pars, double T_fit, int length, double dt,
                int LISA, int backint, int n_max,
                double *t, double *hI, double *hII, double *dense, double *meta);
 
+/* Peters-Mathews coefficients of harmonic n (AAK.cc:493-504): abc = a_n, b_n,
+ * c_n at eccentricity e, amplitude Amp and phase nPhi. */
+void o_mode_abc(int n, double e, double Amp, double nPhi, double *abc);
+
 /* Mode sum + response + taper given dense arrays (AAK.cc:409-547). */
 int o_waveform_from_dense(const double *dense, int vlength, double timestep,
                           int nmodes, double zeta, double M_phys, double lam,

@@ -69,6 +69,7 @@ def oracle():
             "o_aak_wave": (_i, [_pd, _d, _i, _d, _i, _i, _i, _pd, _pd, _pd, _pd, _pd]),
             "o_waveform_from_dense": (_i, [_pd, _i, _d, _i, _d, _d, _d, _d, _d, _d, _d,
                                            _i, _i, _i, _pd, _pd, _pd]),
+            "o_mode_abc": (None, [_i, _d, _d, _d, _pd]),
             "o_last_error": (C.c_char_p, []),
         }
         for name, (res, args) in sig.items():

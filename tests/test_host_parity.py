@@ -1,0 +1,163 @@
+"""Host half of the product (libaak_hip.so, no GPU needed) against the oracle.
+
+The sparse inspiral, the Kerr frequency map, the AK evolution with its fit
+corrections and plunge search, and the natural-spline coefficients run on the
+host before any kernel.  They must equal the oracle bit for bit: the dense
+stage on the GPU then reproduces the reference's phase rounding exactly.
+Error paths that stop the reference process (p0 inside the LSO, a start too
+close to plunge) must come back as negative status codes with a message.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+MSUN_SEC = 6.67259e-11 * 1.98892e30 / 299792458.0 ** 3
+
+
+@pytest.fixture(scope="module")
+def L():
+    from emri_kludge_suite_amd import _lib
+    return _lib
+
+
+def _draw(rng, production=False):
+    if production:
+        M, mu, a = 10 ** rng.uniform(5, 7), rng.uniform(5, 50), rng.uniform(0, 0.95)
+        e, p = rng.uniform(1e-3, 0.5), rng.uniform(7, 15)
+    else:
+        M, mu, a, e, p = 1e6, 10.0, 0.5, 0.2, 8.0
+    return [0.5, a, p, e, M, mu, rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi),
+            np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi), np.arccos(rng.uniform(-1, 1)),
+            rng.uniform(0, 2 * np.pi), 1.0]
+
+
+def _product_plan(L, pars, N, dt, backint=False, cap=4_000_000):
+    lib = L.lib()
+    pv = np.array(pars, dtype=np.float64)
+    traj, kn = np.zeros(500), np.zeros(5 * cap)
+    info, dp = np.zeros(16, dtype=np.int32), np.zeros(4)
+    K = lib.aak_test_host_plan(L.dptr(pv), 1.0, N, dt, 1, int(backint), 0, L.dptr(traj), L.dptr(kn), cap,
+                               L.iptr(info), L.dptr(dp))
+    return K, traj, kn, info, dp
+
+
+def _oracle_plan(pars, N, dt, backint=False):
+    O = ol.oracle()
+    e_traj, v, Mm, sm, dtm, phi0 = (np.zeros(100) for _ in range(6))
+    steps = O.o_sparse_stage(1.0, N, dt, pars[0], pars[1], pars[2], pars[3], pars[4], pars[5], pars[7],
+                             ol._ptr(e_traj), ol._ptr(v), ol._ptr(Mm), ol._ptr(sm), ol._ptr(dtm), ol._ptr(phi0), None)
+    if steps <= 0:
+        return steps, None, None, None
+    par = np.array([pars[5] * MSUN_SEC, Mm[0] * MSUN_SEC, sm[0], e_traj[0], pars[0], pars[6], phi0[0],
+                    pars[9], pars[10], pars[11], pars[12], pars[8]])
+    j0 = O.o_pn_j0(dt, N, ol._ptr(dtm), steps)
+    cap = 2 * j0 + 8
+    kn = np.zeros(5 * cap)
+    info = np.zeros(9, dtype=np.int32)
+    K = O.o_pn_knots(dt, N, ol._ptr(par), ol._ptr(e_traj), ol._ptr(v), pars[4], ol._ptr(Mm), pars[1], ol._ptr(sm),
+                     ol._ptr(dtm), steps, int(backint), ol._ptr(kn), ol._ptr(kn[cap:]), ol._ptr(kn[2 * cap:]),
+                     ol._ptr(kn[3 * cap:]), ol._ptr(kn[4 * cap:]), cap, info.ctypes.data_as(ol._pi))
+    traj = np.concatenate([e_traj, v, Mm, sm, dtm])
+    knots = [kn[i * cap:i * cap + max(K, 0)] for i in range(5)]
+    return K, traj, knots, (phi0[0], info)
+
+
+def _assert_same_plan(L, pars, N, dt, backint=False):
+    cap = 4_000_000
+    K, traj, kn, info, dp = _product_plan(L, pars, N, dt, backint, cap)
+    Ko, traj_o, kn_o, (phi0, info_o) = _oracle_plan(pars, N, dt, backint)
+    assert K == Ko and K > 0, (K, Ko, L.last_error())
+    assert np.array_equal(traj, traj_o), "sparse trajectory / frequency map differ"
+    for a in range(5):
+        assert np.array_equal(kn[a * cap:a * cap + K], kn_o[a]), "knot array %d differs" % a
+    assert dp[0] == phi0
+    # i0, j0, j_min, j_max, j_max_temp, j_start, j_end, plunged
+    assert list(info[2:10]) == list(info_o[[0, 1, 2, 3, 4, 5, 6, 7]])
+    return K, info
+
+
+def test_config1_host_stage_bit_identical(L):
+    _assert_same_plan(L, _draw(np.random.default_rng(1)), 2 ** 18, 10.0)
+
+
+def test_production_prior_host_stage_bit_identical(L):
+    rng = np.random.default_rng(2)
+    done = 0
+    while done < 4:
+        pars = _draw(rng, production=True)
+        K, *_ = _product_plan(L, pars, 2 ** 21, 15.0)
+        if K <= 0:
+            continue
+        _assert_same_plan(L, pars, 2 ** 21, 15.0)
+        done += 1
+
+
+def test_plunging_backint_host_stage_bit_identical(L):
+    pars = [0.5, 0.9, 7.2, 0.3, 3e5, 30.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
+    K, info = _assert_same_plan(L, pars, 2 ** 22, 8.0, backint=True)
+    assert info[9] == 1, "this source must plunge inside the window"
+
+
+def test_spline_table_matches_the_natural_spline(L):
+    pars = _draw(np.random.default_rng(1))
+    K, _, kn, _, _ = _product_plan(L, pars, 2 ** 18, 10.0)
+    cap = 4_000_000
+    packed = np.zeros(17 * K)
+    lib = L.lib()
+    assert lib.aak_test_spline_table(L.dptr(np.array(pars)), 1.0, 2 ** 18, 10.0, 1, 0, L.dptr(packed), 17 * K) >= 0
+    t = kn[:K]
+    assert np.array_equal(packed[:K], t)
+    for a in range(4):
+        y = np.ascontiguousarray(kn[(a + 1) * cap:(a + 1) * cap + K])
+        c = np.zeros(K)
+        ol.oracle().o_spline_natural(ol._ptr(np.ascontiguousarray(t)), ol._ptr(y), K, ol._ptr(c))
+        block = packed[K + 4 * a * K:K + 4 * (a + 1) * K].reshape(4, K)
+        assert np.array_equal(block[0], y)
+        assert np.array_equal(block[2], c), "second-derivative coefficients differ for array %d" % a
+
+
+def test_inspiral_rates_bit_identical(L):
+    rng = np.random.default_rng(3)
+    lib = L.lib()
+    n = 0
+    while n < 100:
+        q = (rng.uniform(6.5, 20), rng.uniform(1e-3, 0.7), rng.uniform(-0.95, 0.95), rng.uniform(0.01, 0.95))
+        o = np.zeros(3)
+        if ol.oracle().o_param_derivs(*q, ol._ptr(o)) != 0 or not np.all(np.isfinite(o)):
+            continue
+        got = np.zeros(3)
+        assert lib.aak_test_inspiral_rates(*q, L.dptr(got)) == 0
+        assert np.array_equal(got, o), q
+        n += 1
+
+
+def test_omega_phi_matches_oracle(L):
+    lib = L.lib()
+    rng = np.random.default_rng(4)
+    for v, e, ci, s in zip(rng.uniform(0.1, 0.4, 50), rng.uniform(0, 0.6, 50), rng.uniform(-1, 1, 50),
+                           rng.uniform(0, 0.95, 50)):
+        assert lib.aak_OmegaPhi(v, e, ci, s, 1e6 * MSUN_SEC) == ol.oracle().o_omegaphi(v, e, ci, s, 1e6 * MSUN_SEC)
+
+
+# ------------------------------------------------------------------ error paths
+def test_start_inside_the_lso_is_an_error_not_an_exit(L):
+    pars = [0.5, 0.5, 3.0, 0.2, 1e6, 10.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
+    K, *_ = _product_plan(L, pars, 2 ** 14, 10.0)
+    assert K < 0
+    assert L.last_error()
+
+
+def test_zero_eccentricity_is_an_error(L):
+    # the reference trajectory returns after one point for e0 = 0 (GKTrajFast.cc:177-181)
+    pars = [0.5, 0.5, 8.0, 0.0, 1e6, 10.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
+    K, *_ = _product_plan(L, pars, 2 ** 14, 10.0)
+    Ko, *_ = _oracle_plan(pars, 2 ** 14, 10.0)
+    assert K < 0 and Ko < 0
+    assert L.last_error()
+
+
+def test_knot_buffer_too_small_is_reported(L):
+    pars = _draw(np.random.default_rng(1))
+    K, *_ = _product_plan(L, pars, 2 ** 18, 10.0, cap=10)
+    assert K < 0 and "small" in L.last_error()
