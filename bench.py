@@ -7,11 +7,13 @@ e0 ~ U[1e-3,0.5], p0 ~ U[7,15], iota0 = 0.5, angles uniform on the sphere),
 noise-weighted inner products against a seeded injected signal under the
 analytic LISA instrument + confusion PSD (1 yr) on the rFFT grid.
 
-One step = one likelihood of the next template: host inspiral + knots, device
-spline/phasing/exact folds/mode sum, rFFT and the fused weighted reduction.
-The data (weighted injection) is resident in HBM before timing starts.
+One step = one batch of templates (default 32) through the whole path: host
+inspiral + knots (thread pool), device spline/phasing/exact folds/mode sum,
+rFFT and the fused weighted reduction, pipelined across templates.  The data
+(weighted injection) is resident in HBM before timing starts.  The same
+templates are also timed one waveform per call (the reference's pattern).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -41,11 +43,12 @@ def draw_params(rng):
 
 
 def valid_draws(seed, count):
-    """First `count` draws whose host stage succeeds (p0 outside the LSO etc.)."""
+    """First `count` draws whose host stage succeeds (p0 outside the LSO etc.),
+    with the active length and harmonic count of each (for the FLOP count)."""
     from emri_kludge_suite_amd import _lib
     L = _lib.lib()
     rng = np.random.default_rng(seed)
-    out = []
+    out, meta = [], []
     cap = 4_000_000
     traj = np.zeros(500)
     kn = np.zeros(5 * cap)
@@ -58,7 +61,14 @@ def valid_draws(seed, count):
                                  _lib.dptr(kn), cap, _lib.iptr(info), _lib.dptr(dp))
         if K > 0:
             out.append(pars)
-    return out
+            meta.append((min(N_SAMPLES, int(info[11]) + int(info[12])), int(info[13])))
+    return out, meta
+
+
+def modesum_flops(meta):
+    """Algorithmic work of one mode-sum launch: N_active (190 n_modes + 460) FLOP,
+    transcendental calls counted as 20 FLOP each."""
+    return sum(n_act * (190.0 * nm + 460.0) for n_act, nm in meta)
 
 
 def weights():
@@ -101,12 +111,26 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_each_way": 2 ** 30, "launches": 5}))
 
 
+def read_traffic():
+    """HBM bytes per k_modesum launch from the committed FETCH_SIZE/WRITE_SIZE
+    summary (null when the summary is absent)."""
+    path = os.path.join(ROOT, "profiles", "aak_likelihood_N2p21_n50_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch k_modesum"):
+                return float(line.split()[-1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--templates", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=5, help="timed batches")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed batches")
+    ap.add_argument("--batch", type=int, default=32, help="templates per step")
+    ap.add_argument("--single", type=int, default=8, help="one-waveform-per-call likelihoods timed beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-probe", action="store_true",
@@ -122,8 +146,10 @@ def main():
 
     from emri_kludge_suite_amd import GPUAAK
 
-    injection = valid_draws(2, 1)[0]
-    templates = valid_draws(3, args.templates)
+    injection = valid_draws(2, 1)[0][0]
+    ntpl = max(args.batch, args.single)
+    templates, meta = valid_draws(3, ntpl)
+    print("bench: %d templates drawn" % len(templates), file=sys.stderr, flush=True)
     w = weights()
     g = GPUAAK(1.0, N_SAMPLES // 100, N_SAMPLES, 100.0, DT, True, False)
     g.set_nmax(N_MAX)
@@ -132,27 +158,39 @@ def main():
     dI = np.fft.rfft(hI[:N_SAMPLES]) * w
     dII = np.fft.rfft(hII[:N_SAMPLES]) * w
     g.input_data(dI, dII, w, w)
+    P = np.array(templates[:args.batch])
+    flops_batch = modesum_flops(meta[:args.batch])
 
-    for k in range(args.warmup):
-        g.WaveformThroughLikelihood(*templates[k % len(templates)])
+    # batched path: host stages on the thread pool, device work pipelined
+    for _ in range(args.warmup):
+        g.LikelihoodBatch(P)
     g.set_profiling(True)
-    stage = np.zeros(6)
-    flops = 0.0
-    modesum_ms = 0.0
+    stage = np.zeros(5)
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        pars = templates[(args.warmup + k) % len(templates)]
-        g.WaveformThroughLikelihood(*pars)
-        tm = g.timings()
-        inf = g.info()
-        stage += np.array([tm[x] for x in ("host", "rates", "fold", "modesum", "rfft", "reduce")])
-        n_active = min(N_SAMPLES, inf["i_plunge"] + inf["i_buffer"])
-        flops += n_active * (190.0 * inf["nmodes"] + 460.0)
-        modesum_ms += tm["modesum"]
+    for _ in range(args.steps):
+        like = g.LikelihoodBatch(P)
+        tm = g.batch_timings()
+        stage += np.array([tm[x] for x in ("rates", "fold", "modesum", "rfft", "reduce")])
     elapsed = time.perf_counter() - t0
-    value = args.steps / elapsed
-    achieved = flops / (modesum_ms * 1e-3) / 1e12
-    stage /= args.steps
+    assert np.all(np.isfinite(like)), "bench: a template failed"
+    n_like = args.steps * len(P)
+    value = n_like / elapsed
+    launches = args.steps * len(P)
+    modesum_ms = stage[2] / launches
+    achieved = flops_batch * args.steps / (stage[2] * 1e-3) / 1e12
+
+    # one waveform per call (the reference's calling pattern), same templates
+    g.set_profiling(True)
+    single_stage = np.zeros(6)
+    t1 = time.perf_counter()
+    for k in range(args.single):
+        g.WaveformThroughLikelihood(*templates[k % len(templates)])
+        tm = g.timings()
+        single_stage += np.array([tm[x] for x in ("host", "rates", "fold", "modesum", "rfft", "reduce")])
+    single_elapsed = time.perf_counter() - t1
+    g.set_profiling(False)
+
+    traffic = read_traffic()
     result = {
         "metric": METRIC,
         "value": value,
@@ -166,12 +204,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded prior draws, injection seed 2, templates seed 3)",
-        "config": {"workload": "1-yr LISA AAK likelihood, N=2^21, dt=15 s, n=1..50, plain FP64, one waveform per call",
-                   "N": N_SAMPLES, "dt": DT, "n_max": N_MAX, "templates": len(templates), "psd": "LISA_Noise(dur=1)"},
-        "stages_ms": dict(zip(("host", "rates", "fold", "modesum", "rfft", "reduce"), [round(x, 4) for x in stage])),
+        "config": {"workload": "1-yr LISA AAK likelihoods, N=2^21, dt=15 s, n=1..50, plain FP64, "
+                               "%d templates per step (batched)" % len(P),
+                   "N": N_SAMPLES, "dt": DT, "n_max": N_MAX, "batch": len(P), "psd": "LISA_Noise(dur=1)"},
+        "stages_ms_per_template": dict(zip(("rates", "fold", "modesum", "rfft", "reduce"),
+                                           [round(x / launches, 4) for x in stage])),
+        "single_call": {"value": args.single / single_elapsed, "unit": "likelihoods/s",
+                        "ms_per_likelihood": 1e3 * single_elapsed / args.single,
+                        "stages_ms": dict(zip(("host", "rates", "fold", "modesum", "rfft", "reduce"),
+                                              [round(x / args.single, 4) for x in single_stage]))},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_modesum", "convention": "N_active*(190*n_modes+460) FLOP per launch"},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "k_modesum", "kernel_ms": modesum_ms,
+                     "convention": "N_active*(190*n_modes+460) FLOP per launch; peak = FP64 vector/matrix"},
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(templates, w, (dI, dII), args.cpu_seconds)

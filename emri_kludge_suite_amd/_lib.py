@@ -42,7 +42,12 @@ def lib():
         "aak_get_info": (_i, [C.c_void_p, _pi, _i]),
         "aak_last_error": (C.c_char_p, []),
         "aak_test_host_plan": (_i, [_pd, _d, _i, _d, _i, _i, _i, _pd, _pd, _i, _pi, _pd]),
-        "aak_test_spline_table": (_i, [_pd, _d, _i, _d, _i, _i, _pd, _i]),
+        "aak_test_knot_table": (_i, [_pd, _d, _i, _d, _i, _i, _pd, _i, _pi]),
+        "aak_test_knot_eval": (_i, [_pd, _d, _i, _d, _i, _i, _pd, _i, _pi, _pd]),
+        "aak_test_bessel_five": (_i, [_i, _d, _d, _pd]),
+        "aak_likelihood_batch": (_i, [C.c_void_p, _pd, _i, _pd, _pi]),
+        "aak_set_host_threads": (_i, [C.c_void_p, _i]),
+        "aak_get_batch_timings": (_i, [C.c_void_p, _pd, _i]),
         "aak_test_inspiral_rates": (_i, [_d, _d, _d, _d, _pd]),
         "aak_test_get_dense": (_i, [C.c_void_p, _pd]),
         "aak_set_profiling": (_i, [C.c_void_p, _i]),

@@ -1,22 +1,40 @@
 // aak_capi.cpp -- the C ABI (include/gpuAAK.h, include/aak_testing.h).
 //
-// One handle owns a HIP stream, the device buffers of one waveform slot, the
-// rocFFT real-forward plan (both channels in one batched call) and the
-// weighted-data buffers.  gpu_gen_AAK runs the host stage, uploads the spline
-// table and enqueues rates -> exact folds -> mode sum; Likelihood enqueues the
-// rFFT and the fused weighted reduction and returns two doubles.
+// One handle owns:
+//   * a compute stream and a copy stream (default priority),
+//   * the device buffers of one waveform (dense arrays, phases, h, t, fold
+//     and guard work, Bessel tables) and two device knot tables, so the
+//     upload of the next waveform's table overlaps the current one's kernels,
+//   * the rocFFT real-forward plan (both channels in one batched call) and
+//     the weighted-data buffers,
+//   * for batches, a pool of host threads and a ring of plan slots whose knot
+//     tables live in pinned host memory.
+//
+// Per waveform the compute stream runs, with no host round trip:
+//   clear counters -> (wait upload) rates -> Phidot guard -> exact folds ->
+//   fill -> mode sum [-> rFFT -> weighted reduction].
+// gpu_gen_AAK / Likelihood keep the reference's one-waveform-per-call
+// semantics; aak_likelihood_batch computes many likelihoods against the same
+// data with the host stages of later templates running on the pool while the
+// GPU works on earlier ones.
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
 #include <chrono>
+#include <cmath>
+#include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/aak_testing.h"
 #include "../../include/gpuAAK.h"
+#include "aak_bessel.hpp"
 #include "aak_dense.hpp"
 #include "aak_geodesic.hpp"
 #include "aak_host.hpp"
@@ -32,28 +50,75 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 
-#define HIPCHK(x)                                                                  \
-  do {                                                                             \
-    hipError_t e_ = (x);                                                           \
+#define HIPCHK(x)                                                                                    \
+  do {                                                                                               \
+    hipError_t e_ = (x);                                                                             \
     if (e_ != hipSuccess) return fail(-100, std::string("HIP: ") + hipGetErrorString(e_) + " at " #x); \
   } while (0)
 
 std::once_flag g_fft_once;
+
+// Pinned host buffer that grows on demand.
+struct PinnedBuf {
+  double *p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t n) {
+    if (n <= cap) return 0;
+    release();
+    const size_t c = n + n / 4 + 4096;
+    if (hipHostMalloc((void **)&p, sizeof(double) * c, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return -1;
+    }
+    cap = c;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct PinnedSink : TableSink {
+  PinnedBuf *buf;
+  explicit PinnedSink(PinnedBuf *b) : buf(b) {}
+  double *reserve(size_t n) override { return buf->reserve(n) ? nullptr : buf->p; }
+};
+
+// One plan slot of the batch ring.  Template i uses slot i % R; a worker may
+// fill it once `ticket` equals i, the consumer takes it once `ready` equals i.
+struct RingSlot {
+  WaveformPlan plan;
+  PinnedBuf pin;
+  hipEvent_t copied = nullptr;  // upload of this slot's table finished
+  bool copy_pending = false;
+  long long ticket = 0;
+  long long ready = -1;
+  int rc = 0;
+  ~RingSlot() {
+    pin.release();
+    if (copied) (void)hipEventDestroy(copied);
+  }
+};
 
 }  // namespace
 
 struct aak_handle_s {
   WaveConfig cfg;
   int N = 0, nbins = 0;
-  hipStream_t stream = nullptr;
-  // waveform slot
-  double *d_knots = nullptr;
-  size_t knots_cap = 0;
+  hipStream_t stream = nullptr, cstream = nullptr;
+  // one waveform's device buffers
+  double *d_table[2] = {nullptr, nullptr};
+  size_t table_cap[2] = {0, 0};
+  hipEvent_t ev_upload[2] = {nullptr, nullptr}, ev_rates[2] = {nullptr, nullptr};
+  bool rates_recorded[2] = {false, false};
   double *d_dense = nullptr;  // e, v, M, S, rate[3], phase[3]: 10 x (N+1)
   double *d_h = nullptr;      // [2][N+2]
   double *d_t = nullptr;      // [N+2]
-  int *d_flags = nullptr;
-  // fold work
+  int *d_flags = nullptr;     // [4]: non-positive Phidot count, fold fallbacks
+  int *d_guard = nullptr;
+  double *d_bessel = nullptr; // c_m(k) table | log k!
   FoldWork fw{};
   int fold_nblk_cap = 0;
   void *d_foldmem = nullptr;
@@ -62,24 +127,44 @@ struct aak_handle_s {
   double2 *d_data = nullptr;  // [2][nbins]
   double *d_w = nullptr;      // [2][nbins]
   double *d_partials = nullptr, *d_like = nullptr;
+  double *d_like_batch = nullptr;
+  size_t like_batch_cap = 0;
+  PinnedBuf like_host;
   rocfft_plan plan = nullptr;
   rocfft_execution_info exec = nullptr;
   void *d_work = nullptr;
   int have_data = 0;
   int state = 0;  // 0 empty, 1 time domain, 2 weighted spectrum
   int info[10] = {0};
-  // stage timing on the handle's stream: rates, folds, mode sum, rFFT, reduction
+  // single-waveform staging
+  WaveformPlan single;
+  PinnedBuf single_pin;
+  // stage timing on the compute stream: rates, guard+folds, mode sum, rFFT, reduction
   int profiling = 0;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t *evs = ev;               // events the marks go to (per template in a batch)
+  std::vector<hipEvent_t> batch_ev;   // 6 per template of the last profiled batch
   double stage_ms[5] = {0, 0, 0, 0, 0};
+  double batch_ms[5] = {0, 0, 0, 0, 0};  // stage sums over the last profiled batch
+  int batch_profiled = 0;
   double host_ms = 0.0;
+  // batch machinery
+  int host_threads = 0;
+  std::vector<std::thread> workers;
+  std::vector<std::unique_ptr<RingSlot>> ring;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_ready;
+  bool stop = false;
+  const double *b_params = nullptr;
+  long long b_n = 0, b_next = 0;
+
   void mark(int k) {
-    if (profiling) (void)hipEventRecord(ev[k], stream);
+    if (profiling) (void)hipEventRecord(evs[k], stream);
   }
-  DenseBuffers bufs() const {
+  DenseBuffers bufs(int slot) const {
     DenseBuffers B;
     const size_t L = (size_t)N + 1;
-    B.knots = d_knots;
+    B.knots = d_table[slot];
     B.e = d_dense;
     B.v = d_dense + L;
     B.M = d_dense + 2 * L;
@@ -91,6 +176,9 @@ struct aak_handle_s {
     B.h = d_h;
     B.t = d_t;
     B.flags = d_flags;
+    B.guard_blk = d_guard;
+    B.bessel_c = d_bessel;
+    B.bessel_lgf = d_bessel + (size_t)(kBesselKMax + 1) * kBesselMMax;
     return B;
   }
 };
@@ -98,10 +186,24 @@ struct aak_handle_s {
 static int alloc_handle(aak_handle h) {
   const int N = h->N;
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
+  for (int k = 0; k < 2; k++) {
+    HIPCHK(hipEventCreateWithFlags(&h->ev_upload[k], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->ev_rates[k], hipEventDisableTiming));
+  }
   HIPCHK(hipMalloc(&h->d_dense, sizeof(double) * 10 * ((size_t)N + 1)));
   HIPCHK(hipMalloc(&h->d_h, sizeof(double) * 2 * ((size_t)N + 2)));
   HIPCHK(hipMalloc(&h->d_t, sizeof(double) * ((size_t)N + 2)));
   HIPCHK(hipMalloc(&h->d_flags, sizeof(int) * 4));
+  HIPCHK(hipMalloc(&h->d_guard, sizeof(int) * ((size_t)N / 4096 + 4)));
+  {
+    const size_t nc = (size_t)(kBesselKMax + 1) * kBesselMMax, nl = kBesselKMax + 1;
+    std::vector<double> tab(nc + nl);
+    bessel_series_table(tab.data());
+    bessel_log_factorials(tab.data() + nc);
+    HIPCHK(hipMalloc(&h->d_bessel, sizeof(double) * tab.size()));
+    HIPCHK(hipMemcpy(h->d_bessel, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  }
   const int nblk = (N + kFoldBlock - 1) / kFoldBlock + 1;
   h->fold_nblk_cap = nblk;
   size_t bytes = 0;
@@ -128,6 +230,7 @@ static int alloc_handle(aak_handle h) {
   h->fw.items = (FoldItem *)(base + o_items);
   h->fw.ties = (FoldTie *)(base + o_ties);
   h->fw.status = (int *)(base + o_st);
+  h->fw.fallbacks = h->d_flags + 1;
   const int nb = h->nbins;
   HIPCHK(hipMalloc(&h->d_hf, sizeof(double2) * 2 * (size_t)nb));
   HIPCHK(hipMalloc(&h->d_data, sizeof(double2) * 2 * (size_t)nb));
@@ -160,17 +263,167 @@ static int alloc_handle(aak_handle h) {
   return 0;
 }
 
+static void stop_workers(aak_handle h) {
+  {
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->stop = true;
+  }
+  h->cv_work.notify_all();
+  h->cv_ready.notify_all();
+  for (std::thread &t : h->workers)
+    if (t.joinable()) t.join();
+  h->workers.clear();
+  h->stop = false;
+  if (h->cstream) (void)hipStreamSynchronize(h->cstream);
+  h->ring.clear();
+}
+
 static void free_handle(aak_handle h) {
+  stop_workers(h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->cstream) (void)hipStreamSynchronize(h->cstream);
   if (h->plan) rocfft_plan_destroy(h->plan);
   if (h->exec) rocfft_execution_info_destroy(h->exec);
-  void *ptrs[] = {h->d_knots, h->d_dense, h->d_h, h->d_t, h->d_flags, h->d_foldmem, h->d_hf,
-                  h->d_data, h->d_w, h->d_partials, h->d_like, h->d_work};
+  void *ptrs[] = {h->d_table[0], h->d_table[1], h->d_dense, h->d_h, h->d_t, h->d_flags, h->d_guard,
+                  h->d_bessel, h->d_foldmem, h->d_hf, h->d_data, h->d_w, h->d_partials, h->d_like,
+                  h->d_like_batch, h->d_work};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  h->single_pin.release();
+  h->like_host.release();
   for (hipEvent_t &e : h->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t &e : h->batch_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; k++) {
+    if (h->ev_upload[k]) (void)hipEventDestroy(h->ev_upload[k]);
+    if (h->ev_rates[k]) (void)hipEventDestroy(h->ev_rates[k]);
+  }
+  if (h->cstream) (void)hipStreamDestroy(h->cstream);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+// Enqueue the device stage of one planned waveform.  The knot table goes to
+// device table `slot` on the copy stream; `copied` (optional) is recorded when
+// the host table may be reused.
+static int enqueue_waveform(aak_handle h, const WaveformPlan &plan, int slot, hipEvent_t copied) {
+  const DenseParams &P = plan.P;
+  const size_t need = plan.tab_n;
+  if (need > h->table_cap[slot]) {
+    // growing: no kernel may still read the old table
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipStreamSynchronize(h->cstream));
+    if (h->d_table[slot]) HIPCHK(hipFree(h->d_table[slot]));
+    h->d_table[slot] = nullptr;
+    h->table_cap[slot] = 0;
+    const size_t cap = need + need / 4 + 4096;
+    HIPCHK(hipMalloc(&h->d_table[slot], sizeof(double) * cap));
+    h->table_cap[slot] = cap;
+    h->rates_recorded[slot] = false;
+  }
+  if (h->rates_recorded[slot]) HIPCHK(hipStreamWaitEvent(h->cstream, h->ev_rates[slot], 0));
+  HIPCHK(hipMemcpyAsync(h->d_table[slot], plan.tab, sizeof(double) * need, hipMemcpyHostToDevice, h->cstream));
+  HIPCHK(hipEventRecord(h->ev_upload[slot], h->cstream));
+  if (copied) HIPCHK(hipEventRecord(copied, h->cstream));
+  const DenseBuffers B = h->bufs(slot);
+  launch_clear_flags(B, h->stream);
+  HIPCHK(hipStreamWaitEvent(h->stream, h->ev_upload[slot], 0));
+  h->mark(0);
+  launch_rates(P, B, h->stream);
+  HIPCHK(hipEventRecord(h->ev_rates[slot], h->stream));
+  h->rates_recorded[slot] = true;
+  h->mark(1);
+  launch_phidot_guard(P, B, h->stream);
+  const int nfwd = P.last_fold - P.i0 + 1;
+  for (int pass = 0; pass < 2; pass++) {
+    const int dir = pass == 0 ? 1 : -1;
+    const int nsteps = pass == 0 ? nfwd : P.i0;
+    if (nsteps <= 0) continue;
+    if (fold_exact(P, B, h->fw, h->fold_nblk_cap, dir, nsteps, h->stream)) return fail(-7, "fold block capacity");
+  }
+  launch_fold_fill(P, B, h->stream);
+  h->mark(2);
+  launch_modesum(P, B, h->stream);
+  h->mark(3);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int enqueue_likelihood(aak_handle h, double *out_dev, bool keep_spectrum) {
+  void *in[1] = {h->d_h};
+  void *outb[1] = {h->d_hf};
+  if (rocfft_execute(h->plan, in, outb, h->exec) != rocfft_status_success) return fail(-101, "rocfft execute");
+  h->mark(4);
+  launch_weighted_dot(h->d_hf, h->d_data, h->d_w, h->nbins, keep_spectrum ? h->d_hf : nullptr, h->d_partials,
+                      out_dev, h->stream);
+  h->mark(5);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// stage k spans events k .. k+1
+static void read_stage_times(aak_handle h, int first, int last) {
+  if (!h->profiling) return;
+  float ms = 0.f;
+  for (int k = first; k < last; k++)
+    if (hipEventElapsedTime(&ms, h->ev[k], h->ev[k + 1]) == hipSuccess) h->stage_ms[k] = ms;
+}
+
+// ---------------------------------------------------------------- batch pool
+static void worker_main(aak_handle h) {
+  for (;;) {
+    long long i;
+    const double *params;
+    RingSlot *slot;
+    {
+      std::unique_lock<std::mutex> lk(h->mu);
+      h->cv_work.wait(lk, [&] { return h->stop || (h->b_params && h->b_next < h->b_n); });
+      if (h->stop) return;
+      i = h->b_next++;
+      params = h->b_params;
+      slot = h->ring[(size_t)(i % (long long)h->ring.size())].get();
+      h->cv_work.wait(lk, [&] { return h->stop || slot->ticket == i; });
+      if (h->stop) return;
+    }
+    if (slot->copy_pending) {
+      (void)hipEventSynchronize(slot->copied);
+      slot->copy_pending = false;
+    }
+    SourceParams sp;
+    std::memcpy(&sp, params + 14 * (size_t)i, sizeof(SourceParams));
+    PinnedSink sink(&slot->pin);
+    std::string err;
+    const int rc = plan_waveform(h->cfg, sp, &slot->plan, &err, &sink);
+    {
+      std::lock_guard<std::mutex> lk(h->mu);
+      slot->rc = rc;
+      slot->ready = i;
+    }
+    h->cv_ready.notify_all();
+  }
+}
+
+static int start_pool(aak_handle h) {
+  if (!h->workers.empty()) return 0;
+  int T = h->host_threads;
+  if (T <= 0) {
+    const char *env = std::getenv("AAK_HOST_THREADS");
+    T = env ? std::atoi(env) : 0;
+    if (T <= 0) {
+      const unsigned hc = std::thread::hardware_concurrency();
+      T = hc ? (int)hc : 4;
+      if (T > 16) T = 16;
+    }
+  }
+  const int R = T + 4;
+  h->ring.clear();
+  for (int s = 0; s < R; s++) {
+    std::unique_ptr<RingSlot> slot(new RingSlot());
+    HIPCHK(hipEventCreateWithFlags(&slot->copied, hipEventDisableTiming));
+    h->ring.push_back(std::move(slot));
+  }
+  for (int t = 0; t < T; t++) h->workers.emplace_back(worker_main, h);
+  return 0;
 }
 
 extern "C" {
@@ -224,6 +477,14 @@ int aak_set_nmax(aak_handle h, int n_max) {
   return 0;
 }
 
+int aak_set_host_threads(aak_handle h, int n) {
+  if (!h) return fail(-1, "null handle");
+  if (n < 0 || n > 256) return fail(-2, "host threads must be in [0, 256]");
+  stop_workers(h);
+  h->host_threads = n;
+  return 0;
+}
+
 int aak_input_data(aak_handle h, const double *hI_f, const double *hII_f, const double *ASDinv1,
                    const double *ASDinv2, int len) {
   if (!h) return fail(-1, "null handle");
@@ -242,57 +503,26 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
                     double psi, double alph, double theta_S, double phi_S, double theta_K, double phi_K, double D) {
   if (!h) return fail(-1, "null handle");
   SourceParams sp{iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D};
-  WaveformPlan plan;
+  WaveformPlan &plan = h->single;
   std::string err;
   auto t0 = std::chrono::steady_clock::now();
-  int rc = plan_waveform(h->cfg, sp, &plan, &err);
+  // the staging buffer is free: every call ends with the compute stream idle,
+  // and the compute stream waited for the upload
+  PinnedSink sink(&h->single_pin);
+  int rc = plan_waveform(h->cfg, sp, &plan, &err, &sink);
   h->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  if (rc < 0) return fail(rc, err);
-  const DenseParams &P = plan.P;
-  const size_t need = plan.packed.size();
-  if (need > h->knots_cap) {
-    if (h->d_knots) HIPCHK(hipFree(h->d_knots));
-    h->d_knots = nullptr;
-    const size_t cap = need + need / 4;
-    HIPCHK(hipMalloc(&h->d_knots, sizeof(double) * cap));
-    h->knots_cap = cap;
+  if (rc < 0) {
+    h->state = 0;
+    return fail(rc, err);
   }
-  HIPCHK(hipMemcpyAsync(h->d_knots, plan.packed.data(), sizeof(double) * need, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemsetAsync(h->d_flags, 0, sizeof(int) * 4, h->stream));
-  const DenseBuffers B = h->bufs();
-  h->mark(0);
-  launch_rates(P, B, h->stream);
-  h->mark(1);
+  rc = enqueue_waveform(h, plan, 0, nullptr);
+  if (rc) return rc;
   int flags[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(flags, h->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  if (flags[0] > 0) launch_phidot_guard(P, B, h->stream);
-  int fallbacks = 0;
-  const int nfwd = P.last_fold - P.i0 + 1;
-  for (int pass = 0; pass < 2; pass++) {
-    const int dir = pass == 0 ? 1 : -1;
-    const int nsteps = pass == 0 ? nfwd : P.i0;
-    if (nsteps <= 0) continue;
-    if (fold_exact(P, B, h->fw, h->fold_nblk_cap, dir, nsteps, h->stream)) return fail(-7, "fold block capacity");
-    int st[3];
-    HIPCHK(hipMemcpyAsync(st, h->fw.status, sizeof(int) * 3, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    fallbacks += st[0] + st[1] + st[2];
-  }
-  launch_fold_fill(P, B, h->stream);
-  h->mark(2);
-  launch_modesum(P, B, h->stream);
-  h->mark(3);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(h->stream));
-  if (h->profiling) {
-    float ms = 0.f;
-    for (int k = 0; k < 3; k++) {
-      HIPCHK(hipEventElapsedTime(&ms, h->ev[k], h->ev[k + 1]));
-      h->stage_ms[k] = ms;
-    }
-  }
+  read_stage_times(h, 0, 3);
   h->state = 1;
+  const DenseParams &P = plan.P;
   int *I = h->info;
   I[0] = plan.traj.steps;
   I[1] = P.nmodes;
@@ -300,7 +530,7 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   I[3] = P.ib;
   I[4] = P.K;
   I[5] = P.i0;
-  I[6] = fallbacks;
+  I[6] = flags[1];
   I[7] = flags[0];
   I[8] = plan.knots.j0;
   I[9] = plan.knots.plunged;
@@ -311,23 +541,112 @@ int aak_likelihood(aak_handle h, double *like_out) {
   if (!h) return fail(-1, "null handle");
   if (!h->have_data) return fail(-2, "Likelihood before input_data");
   if (h->state != 1) return fail(-2, "Likelihood needs a waveform from gpu_gen_AAK");
-  void *in[1] = {h->d_h};
-  void *outb[1] = {h->d_hf};
   h->mark(3);
-  if (rocfft_execute(h->plan, in, outb, h->exec) != rocfft_status_success) return fail(-101, "rocfft execute");
-  h->mark(4);
-  launch_weighted_dot(h->d_hf, h->d_data, h->d_w, h->nbins, h->d_hf, h->d_partials, h->d_like, h->stream);
-  h->mark(5);
+  int rc = enqueue_likelihood(h, h->d_like, true);
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(like_out, h->d_like, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  if (h->profiling) {
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, h->ev[3], h->ev[4]));
-    h->stage_ms[3] = ms;
-    HIPCHK(hipEventElapsedTime(&ms, h->ev[4], h->ev[5]));
-    h->stage_ms[4] = ms;
-  }
+  read_stage_times(h, 3, 5);
   h->state = 2;
+  return 0;
+}
+
+int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like_out, int *status) {
+  if (!h) return fail(-1, "null handle");
+  if (!h->have_data) return fail(-2, "likelihood batch before input_data");
+  if (n < 0 || (n > 0 && (!params || !like_out))) return fail(-2, "bad batch arguments");
+  if (n == 0) return 0;
+  int rc = start_pool(h);
+  if (rc) return rc;
+  if ((size_t)n * 2 > h->like_batch_cap) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->d_like_batch) HIPCHK(hipFree(h->d_like_batch));
+    h->d_like_batch = nullptr;
+    h->like_batch_cap = 0;
+    HIPCHK(hipMalloc(&h->d_like_batch, sizeof(double) * 2 * (size_t)n));
+    h->like_batch_cap = (size_t)n * 2;
+  }
+  if (h->like_host.reserve((size_t)n * 2)) return fail(-100, "pinned host allocation failed");
+  std::vector<int> st((size_t)n, 0);
+  const long long R = (long long)h->ring.size();
+  const int prof = h->profiling;
+  if (prof && h->batch_ev.size() < (size_t)6 * n) {
+    const size_t old = h->batch_ev.size();
+    h->batch_ev.resize((size_t)6 * n, nullptr);
+    for (size_t k = old; k < h->batch_ev.size(); k++) HIPCHK(hipEventCreate(&h->batch_ev[k]));
+  }
+  {
+    std::lock_guard<std::mutex> lk(h->mu);
+    for (long long s = 0; s < R; s++) {
+      h->ring[s]->ticket = s;
+      h->ring[s]->ready = -1;
+    }
+    h->b_params = params;
+    h->b_n = n;
+    h->b_next = 0;
+  }
+  h->cv_work.notify_all();
+  int err_rc = 0;
+  std::string err_msg;
+  for (long long i = 0; i < n; i++) {
+    RingSlot &slot = *h->ring[(size_t)(i % R)];
+    {
+      std::unique_lock<std::mutex> lk(h->mu);
+      h->cv_ready.wait(lk, [&] { return slot.ready == i; });
+    }
+    st[i] = slot.rc < 0 ? slot.rc : 0;
+    if (st[i] == 0 && err_rc == 0) {
+      if (prof) h->evs = h->batch_ev.data() + 6 * i;
+      rc = enqueue_waveform(h, slot.plan, (int)(i & 1), slot.copied);
+      if (!rc) {
+        slot.copy_pending = true;
+        rc = enqueue_likelihood(h, h->d_like_batch + 2 * i, false);
+      }
+      if (rc) {
+        err_rc = rc;
+        err_msg = g_err;
+      }
+    }
+    if (err_rc && st[i] == 0) st[i] = err_rc;
+    {
+      std::lock_guard<std::mutex> lk(h->mu);
+      slot.ticket = i + R;
+    }
+    h->cv_work.notify_all();
+  }
+  {
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->b_params = nullptr;
+  }
+  h->evs = h->ev;
+  h->state = 0;
+  if (err_rc) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->cstream);
+    if (status) std::memcpy(status, st.data(), sizeof(int) * (size_t)n);
+    return fail(err_rc, err_msg);
+  }
+  HIPCHK(hipMemcpyAsync(h->like_host.p, h->d_like_batch, sizeof(double) * 2 * (size_t)n, hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipStreamSynchronize(h->cstream));
+  if (prof) {
+    for (int k = 0; k < 5; k++) h->batch_ms[k] = 0.0;
+    float ms = 0.f;
+    for (long long i = 0; i < n; i++) {
+      if (st[i]) continue;
+      const hipEvent_t *e = h->batch_ev.data() + 6 * i;
+      for (int k = 0; k < 5; k++)
+        if (hipEventElapsedTime(&ms, e[k], e[k + 1]) == hipSuccess) h->batch_ms[k] += ms;
+    }
+    h->batch_profiled = 1;
+  }
+  const double qnan = std::nan("");
+  for (long long i = 0; i < n; i++) {
+    like_out[2 * i] = st[i] ? qnan : h->like_host.p[2 * i];
+    like_out[2 * i + 1] = st[i] ? qnan : h->like_host.p[2 * i + 1];
+  }
+  if (status) std::memcpy(status, st.data(), sizeof(int) * (size_t)n);
   return 0;
 }
 
@@ -363,6 +682,14 @@ int aak_get_timings(aak_handle h, double *ms, int n) {
   return m;
 }
 
+int aak_get_batch_timings(aak_handle h, double *ms, int n) {
+  if (!h) return fail(-1, "null handle");
+  if (!h->batch_profiled) return fail(-2, "no profiled batch");
+  const int m = n < 5 ? n : 5;
+  for (int i = 0; i < m; i++) ms[i] = h->batch_ms[i];
+  return m;
+}
+
 int aak_get_info(aak_handle h, int *info, int n) {
   if (!h) return fail(-1, "null handle");
   const int m = n < 10 ? n : 10;
@@ -371,8 +698,8 @@ int aak_get_info(aak_handle h, int *info, int n) {
 }
 
 // ---------------------------------------------------------------- test hooks
-int aak_test_host_plan(const double *params, double T_fit, int length, double dt, int LISA, int backint, int n_max,
-                       double *traj_out, double *knots_out, int cap, int *info, double *dparams) {
+static int test_plan(const double *params, double T_fit, int length, double dt, int LISA, int backint, int n_max,
+                     WaveformPlan *plan) {
   WaveConfig cfg;
   cfg.T_fit = T_fit;
   cfg.length = length;
@@ -382,10 +709,16 @@ int aak_test_host_plan(const double *params, double T_fit, int length, double dt
   cfg.n_max = n_max;
   SourceParams sp;
   std::memcpy(&sp, params, sizeof(double) * 14);
-  WaveformPlan plan;
   std::string err;
-  const int rc = plan_waveform(cfg, sp, &plan, &err);
-  if (rc < 0) return fail(rc, err);
+  const int rc = plan_waveform(cfg, sp, plan, &err);
+  return rc < 0 ? fail(rc, err) : 0;
+}
+
+int aak_test_host_plan(const double *params, double T_fit, int length, double dt, int LISA, int backint, int n_max,
+                       double *traj_out, double *knots_out, int cap, int *info, double *dparams) {
+  WaveformPlan plan;
+  const int rc = test_plan(params, T_fit, length, dt, LISA, backint, n_max, &plan);
+  if (rc) return rc;
   const SparseTrajectory &tr = plan.traj;
   for (int i = 0; i < 100; i++) {
     const bool ok = i < tr.steps;
@@ -397,13 +730,8 @@ int aak_test_host_plan(const double *params, double T_fit, int length, double dt
   }
   const KnotSet &ks = plan.knots;
   if (ks.K > cap) return fail(-6, "knot buffer too small");
-  for (int k = 0; k < ks.K; k++) {
-    knots_out[k] = ks.t[k];
-    knots_out[cap + k] = ks.e[k];
-    knots_out[2 * (size_t)cap + k] = ks.v[k];
-    knots_out[3 * (size_t)cap + k] = ks.M[k];
-    knots_out[4 * (size_t)cap + k] = ks.S[k];
-  }
+  const double *src[5] = {ks.kt(), ks.ky(0), ks.ky(1), ks.ky(2), ks.ky(3)};
+  for (int a = 0; a < 5; a++) std::memcpy(knots_out + (size_t)a * cap, src[a], sizeof(double) * ks.K);
   int iters = 0;
   for (int i = 0; i < tr.steps; i++) iters += tr.map_iters[i];
   const int vals[16] = {tr.steps, ks.K, ks.i0, ks.j0, ks.j_min, ks.j_max, ks.j_max_temp, ks.j_start,
@@ -416,23 +744,47 @@ int aak_test_host_plan(const double *params, double T_fit, int length, double dt
   return ks.K;
 }
 
-int aak_test_spline_table(const double *params, double T_fit, int length, double dt, int LISA, int backint,
-                          double *packed, int cap17) {
-  WaveConfig cfg;
-  cfg.T_fit = T_fit;
-  cfg.length = length;
-  cfg.dt = dt;
-  cfg.LISA = LISA;
-  cfg.backint = backint;
-  SourceParams sp;
-  std::memcpy(&sp, params, sizeof(double) * 14);
+int aak_test_knot_table(const double *params, double T_fit, int length, double dt, int LISA, int backint,
+                        double *table, int cap, int *grid) {
   WaveformPlan plan;
-  std::string err;
-  const int rc = plan_waveform(cfg, sp, &plan, &err);
-  if (rc < 0) return fail(rc, err);
-  if ((int)plan.packed.size() > cap17) return fail(-6, "table buffer too small");
-  std::memcpy(packed, plan.packed.data(), sizeof(double) * plan.packed.size());
-  return plan.knots.K;
+  const int rc = test_plan(params, T_fit, length, dt, LISA, backint, 0, &plan);
+  if (rc) return rc;
+  if (plan.tab_n > (size_t)cap) return fail(-6, "table buffer too small");
+  std::memcpy(table, plan.tab, sizeof(double) * plan.tab_n);
+  grid[0] = plan.P.grid.K;
+  grid[1] = plan.P.grid.head_off;
+  grid[2] = plan.P.grid.steps;
+  return (int)plan.tab_n;
+}
+
+int aak_test_knot_eval(const double *params, double T_fit, int length, double dt, int LISA, int backint,
+                       const double *t, int n, int *k_out, double *y_out) {
+  WaveformPlan plan;
+  const int rc = test_plan(params, T_fit, length, dt, LISA, backint, 0, &plan);
+  if (rc) return rc;
+  const double *head = plan.tab + (size_t)kTableArrays * plan.P.grid.K;
+  for (int i = 0; i < n; i++) {
+    k_out[i] = knot_index(plan.P.grid, head, t[i]);
+    for (int a = 0; a < 4; a++) y_out[4 * (size_t)i + a] = table_eval(plan.tab, plan.P.grid, a, t[i]);
+  }
+  return 0;
+}
+
+int aak_test_bessel_five(int n, double e, double e_bound, double *J) {
+  if (n < 1 || n > kMaxModes) return fail(-2, "harmonic outside 1..128");
+  if (!(e >= 0.0) || e > e_bound || e_bound > kSeriesMaxE) return fail(-2, "eccentricity outside the series range");
+  static std::vector<double> tab;
+  static std::once_flag once;
+  std::call_once(once, []() {
+    const size_t nc = (size_t)(kBesselKMax + 1) * kBesselMMax;
+    tab.resize(nc + kBesselKMax + 1);
+    bessel_series_table(tab.data());
+    bessel_log_factorials(tab.data() + nc);
+  });
+  unsigned char terms[kMaxModes + 1];
+  bessel_series_terms(e_bound, n, terms);
+  bessel_five_series(n, e, tab.data(), tab.data() + (size_t)(kBesselKMax + 1) * kBesselMMax, terms[n], terms[0], J);
+  return terms[n];
 }
 
 int aak_test_inspiral_rates(double p, double e, double cosi, double a, double *out) {

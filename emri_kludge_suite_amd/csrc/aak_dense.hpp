@@ -4,7 +4,69 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "aak_bessel.hpp"
+
 namespace aak {
+
+// Device knot table: y of e, v, M, S (4K) | their spline coefficients c (4K) |
+// head knot times (KnotGrid::steps).
+constexpr int kTableArrays = 8;
+
+// Knot times of the device knot table.  Knot k has evolution index
+// j = j_start + k and time
+//   j <  j0            dt_large * (j - j0)                (backward evolution)
+//   j0 <= j < j0+steps dt_map[j - j0]                     (stored "head")
+//   j >= j0+steps      dt_map[steps-1] + dt_large * (j - j0 - steps + 1)
+// exactly as the host evolution computes them (AAK.cc:114-121, 224-231), so
+// host and device agree bit for bit without shipping K times.
+struct KnotGrid {
+  int K;           // knots
+  int head_off;    // knot index of dt_map[0]  (j0 - j_start)
+  int steps;       // head length
+  int pad;
+  double dt_large, t_head_last;  // spacing, dt_map[steps-1]
+};
+
+AAK_HD double knot_time(const KnotGrid &g, const double *head, int k) {
+  const int jr = k - g.head_off;
+  if (jr < 0) return g.dt_large * (double)jr;
+  if (jr < g.steps) return head[jr];
+  return g.t_head_last + g.dt_large * (double)(jr - g.steps + 1);
+}
+
+// Largest k in [0, K-2] with knot_time(k) <= t (0 if t precedes every knot):
+// the interval a bisection over the knot times would return.
+AAK_HD int knot_index(const KnotGrid &g, const double *head, double t) {
+  int k;
+  if (t < 0.0) {
+    k = g.head_off + (int)floor(t / g.dt_large);
+  } else if (t < g.t_head_last) {
+    int lo = 0, hi = g.steps - 1;
+    while (hi > lo + 1) {
+      const int m = (hi + lo) >> 1;
+      if (head[m] > t) hi = m;
+      else lo = m;
+    }
+    k = g.head_off + lo;
+  } else {
+    k = g.head_off + g.steps - 1 + (int)floor((t - g.t_head_last) / g.dt_large);
+  }
+  if (k > g.K - 2) k = g.K - 2;
+  if (k < 0) k = 0;
+  while (k < g.K - 2 && knot_time(g, head, k + 1) <= t) k++;
+  while (k > 0 && knot_time(g, head, k) > t) k--;
+  return k;
+}
+
+// Cubic of one spline interval from the knot values and second-derivative
+// coefficients, in the evaluation order of the reference's spline.
+AAK_HD void spline_interval(double x0, double x1, double y0, double y1, double c0, double c1, double *b,
+                                   double *d) {
+  const double h = x1 - x0;
+  const double dy = y1 - y0;
+  *b = (dy / h) - h * (c1 + 2.0 * c0) / 3.0;
+  *d = (c1 - c0) / (3.0 * h);
+}
 
 // Per-waveform scalars of the dense stage, filled on the host.
 struct DenseParams {
@@ -30,6 +92,9 @@ struct DenseParams {
   double sin_thK, cos_thK, sin_phK, cos_phK;
   double zeta, Mphys;             // amplitude scale and M in solar masses
   double t_plunge, t_zero;
+  KnotGrid grid;
+  int bessel_series;              // 1: series + recurrence, 0: Miller from above
+  unsigned char bterms[kMaxModes + 1];  // series length per harmonic ([0]: orders 0..4)
 };
 
 // Exact-order parallel fold bookkeeping.
@@ -63,17 +128,21 @@ struct FoldWork {
   FoldItem *items;                      // [3][nblk][kFoldMaxItems]
   FoldTie *ties;                        // [3][nblk][kFoldMaxTies]
   int *status;                          // [3]: 0 ok, 1 overflow/fallback
+  int *fallbacks;                       // serial replays since the last counter reset
 };
 
 // Device buffers of one waveform slot.
 struct DenseBuffers {
-  double *knots;     // [17*K]: t | (y,b,c,d) x (e,v,M,S)
+  double *knots;     // device knot table: y (4K) | c (4K) | head times
   double *e, *v, *M, *S;           // [N+1]
   double *rate[3];                 // gimdot, Phidot, alpdot [N+1]
   double *phase[3];                // gim, Phi, alp [N+1]
   double *h;                       // [2][N+2] time domain hI | hII
   double *t;                       // [N+2]
   int *flags;                      // [4]: [0] non-positive Phidot count
+  int *guard_blk;                  // [guard blocks]: last positive-Phidot index per block
+  const double *bessel_c;          // series coefficients c_m(k)
+  const double *bessel_lgf;        // log(k!)
 };
 
 // Launch wrappers (aak_kernels.hip).
@@ -82,7 +151,9 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
                hipStream_t s);
 void launch_fold_fill(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
+int guard_blocks(const DenseParams &P);
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
+void launch_clear_flags(const DenseBuffers &B, hipStream_t s);
 void launch_weighted_dot(const double2 *hf, const double2 *data, const double *w, int nbins, double2 *hw_out,
                          double *partials, double *out, hipStream_t s);
 

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "aak_bessel.hpp"
 #include "aak_freqmap.hpp"
 #include "aak_inspiral.hpp"
 
@@ -142,8 +143,13 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
   }
   int j_start = j0, j_min = j0, j_end = 2 * j0 - 1, j_max = j_end, j_max_temp = j_max;
   double t_end = vlength * timestep;
+  // Arrays over the whole index range j in [0, 2 j0); every entry is written
+  // before it is read, and storage is reused across waveforms.
   const size_t L = (size_t)2 * j0;
-  std::vector<double> eA(L, 0.0), vA(L, 0.0), tf(L, 0.0), ef(L, 0.0), vf(L, 0.0), Mf(L, 0.0), Sf(L, 0.0);
+  for (std::vector<double> *a : {&ks->eA, &ks->vA, &ks->t, &ks->e, &ks->v, &ks->M, &ks->S})
+    if (a->size() < L) a->resize(L);
+  double *eA = ks->eA.data(), *vA = ks->vA.data(), *tf = ks->t.data(), *ef = ks->e.data(), *vf = ks->v.data(),
+         *Mf = ks->M.data(), *Sf = ks->S.data();
   double edot = 0, vdot = 0, edotm, vdotm;
   eA[j0] = e0;
   vA[j0] = tr.v_map[0];
@@ -282,11 +288,9 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
   }
   const int K = j_end - j_start + 1;
   ks->K = K;
-  ks->t.assign(tf.begin() + j_start, tf.begin() + j_end + 1);
-  ks->e.assign(ef.begin() + j_start, ef.begin() + j_end + 1);
-  ks->v.assign(vf.begin() + j_start, vf.begin() + j_end + 1);
-  ks->M.assign(Mf.begin() + j_start, Mf.begin() + j_end + 1);
-  ks->S.assign(Sf.begin() + j_start, Sf.begin() + j_end + 1);
+  ks->off = j_start;
+  ks->steps = steps;
+  ks->dt_large = dt_large;
   ks->i0 = vlength - (int)(t_end / timestep);
   ks->j0 = j0;
   ks->j_min = j_min;
@@ -301,81 +305,93 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
 }
 
 // Natural cubic spline: symmetric tridiagonal system for c = y''/2 solved by
-// LDL^T (the scheme of GSL's cspline, which the reference links).
-int spline_natural(const double *x, const double *y, int n, double *c) {
+// LDL^T (the scheme of GSL's cspline, which the reference links).  The
+// factorisation depends on the abscissae only, so the four arrays of one
+// waveform share it; each array's arithmetic is unchanged.
+int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs) {
   if (n < 3) return -1;
   const int sys = n - 2;
-  std::vector<double> dg(sys), od(sys), g(sys), gam(sys), al(sys), z(sys);
-  c[0] = 0.0;
-  c[n - 1] = 0.0;
+  thread_local std::vector<double> od, dg, gi_, al, gam, z;
+  for (std::vector<double> *a : {&od, &dg, &gi_, &al, &gam, &z})
+    if ((int)a->size() < sys + 1) a->resize(sys + 1);
   for (int i = 0; i < sys; i++) {
     const double hi = x[i + 1] - x[i], hip1 = x[i + 2] - x[i + 1];
-    const double ydi = y[i + 1] - y[i], ydip1 = y[i + 2] - y[i + 1];
-    const double gi = (hi != 0.0) ? 1.0 / hi : 0.0;
-    const double gip1 = (hip1 != 0.0) ? 1.0 / hip1 : 0.0;
     od[i] = hip1;
     dg[i] = 2.0 * (hip1 + hi);
-    g[i] = 3.0 * (ydip1 * gip1 - ydi * gi);
+    gi_[i] = (hi != 0.0) ? 1.0 / hi : 0.0;
   }
-  if (sys == 1) {
-    c[1] = g[0] / dg[0];
-    return 0;
+  gi_[sys] = (x[sys + 1] - x[sys] != 0.0) ? 1.0 / (x[sys + 1] - x[sys]) : 0.0;
+  if (sys > 1) {
+    al[0] = dg[0];
+    gam[0] = od[0] / al[0];
+    for (int i = 1; i < sys - 1; i++) {
+      al[i] = dg[i] - od[i - 1] * gam[i - 1];
+      gam[i] = od[i] / al[i];
+    }
+    al[sys - 1] = dg[sys - 1] - od[sys - 2] * gam[sys - 2];
   }
-  al[0] = dg[0];
-  gam[0] = od[0] / al[0];
-  for (int i = 1; i < sys - 1; i++) {
-    al[i] = dg[i] - od[i - 1] * gam[i - 1];
-    gam[i] = od[i] / al[i];
+  for (int a = 0; a < 4; a++) {
+    const double *y = ys[a];
+    double *c = cs[a];
+    c[0] = 0.0;
+    c[n - 1] = 0.0;
+    // right-hand side g_i = 3 (dy_{i+1}/h_{i+1} - dy_i/h_i)
+    auto g = [&](int i) { return 3.0 * ((y[i + 2] - y[i + 1]) * gi_[i + 1] - (y[i + 1] - y[i]) * gi_[i]); };
+    if (sys == 1) {
+      c[1] = g(0) / dg[0];
+      continue;
+    }
+    z[0] = g(0);
+    for (int i = 1; i < sys; i++) z[i] = g(i) - gam[i - 1] * z[i - 1];
+    for (int i = 0; i < sys; i++) z[i] = z[i] / al[i];
+    c[sys] = z[sys - 1];
+    for (int i = sys - 2; i >= 0; i--) c[1 + i] = z[i] - gam[i] * c[i + 2];
   }
-  al[sys - 1] = dg[sys - 1] - od[sys - 2] * gam[sys - 2];
-  z[0] = g[0];
-  for (int i = 1; i < sys; i++) z[i] = g[i] - gam[i - 1] * z[i - 1];
-  for (int i = 0; i < sys; i++) z[i] = z[i] / al[i];
-  c[sys] = z[sys - 1];
-  for (int i = sys - 2; i >= 0; i--) c[1 + i] = z[i] - gam[i] * c[i + 2];
   return 0;
 }
 
-// Device table: t | for each of (e, v, M, S): y, b, c, d per interval, with
-// b = dy/h - h (c_{k+1} + 2 c_k)/3 and d = (c_{k+1} - c_k)/(3h).
-void pack_spline(const KnotSet &ks, std::vector<double> *packed) {
+int spline_natural(const double *x, const double *y, int n, double *c) {
+  thread_local std::vector<double> dummy;
+  if ((int)dummy.size() < 3 * n) dummy.resize(3 * n);
+  const double *ys[4] = {y, y, y, y};
+  double *cs[4] = {c, dummy.data(), dummy.data() + n, dummy.data() + 2 * n};
+  return spline_natural4(x, ys, n, cs);
+}
+
+size_t table_size(const KnotSet &ks) { return (size_t)kTableArrays * ks.K + ks.steps; }
+
+void fill_table(const KnotSet &ks, const SparseTrajectory &tr, double *tab, KnotGrid *grid) {
   const int K = ks.K;
-  packed->assign((size_t)17 * K, 0.0);
-  double *P = packed->data();
-  std::copy(ks.t.begin(), ks.t.end(), P);
-  const std::vector<double> *ys[4] = {&ks.e, &ks.v, &ks.M, &ks.S};
-  std::vector<double> c(K);
+  const double *ys[4] = {ks.ky(0), ks.ky(1), ks.ky(2), ks.ky(3)};
+  double *cs[4];
   for (int a = 0; a < 4; a++) {
-    const double *y = ys[a]->data();
-    spline_natural(ks.t.data(), y, K, c.data());
-    double *Y = P + (size_t)K * (1 + 4 * a), *Bc = Y + K, *Cc = Y + 2 * (size_t)K, *Dc = Y + 3 * (size_t)K;
-    for (int k = 0; k < K; k++) {
-      Y[k] = y[k];
-      Cc[k] = c[k];
-      if (k < K - 1) {
-        const double h = ks.t[k + 1] - ks.t[k];
-        const double dy = y[k + 1] - y[k];
-        Bc[k] = (dy / h) - h * (c[k + 1] + 2.0 * c[k]) / 3.0;
-        Dc[k] = (c[k + 1] - c[k]) / (3.0 * h);
-      }
-    }
+    std::memcpy(tab + (size_t)a * K, ys[a], sizeof(double) * K);
+    cs[a] = tab + (size_t)(4 + a) * K;
   }
+  spline_natural4(ks.kt(), ys, K, cs);
+  std::memcpy(tab + (size_t)kTableArrays * K, tr.dt_map, sizeof(double) * ks.steps);
+  grid->K = K;
+  grid->head_off = ks.j0 - ks.j_start;
+  grid->steps = ks.steps;
+  grid->pad = 0;
+  grid->dt_large = ks.dt_large;
+  grid->t_head_last = tr.dt_map[ks.steps - 1];
 }
 
-double packed_eval(const std::vector<double> &packed, int K, int array, double t) {
-  const double *x = packed.data();
-  int lo = 0, hi = K - 1;
-  while (hi > lo + 1) {
-    const int m = (hi + lo) / 2;
-    if (x[m] > t) hi = m;
-    else lo = m;
-  }
-  const double *Y = packed.data() + (size_t)K * (1 + 4 * array);
-  const double dl = t - x[lo];
-  return Y[lo] + dl * (Y[K + lo] + dl * (Y[2 * (size_t)K + lo] + dl * Y[3 * (size_t)K + lo]));
+double table_eval(const double *tab, const KnotGrid &g, int array, double t) {
+  const int K = g.K;
+  const double *head = tab + (size_t)kTableArrays * K;
+  const int k = knot_index(g, head, t);
+  const double x0 = knot_time(g, head, k), x1 = knot_time(g, head, k + 1);
+  const double *y = tab + (size_t)array * K, *c = tab + (size_t)(4 + array) * K;
+  double b, d;
+  spline_interval(x0, x1, y[k], y[k + 1], c[k], c[k + 1], &b, &d);
+  const double dl = t - x0;
+  return y[k] + dl * (b + dl * (c[k] + dl * d));
 }
 
-int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *plan, std::string *err) {
+int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *plan, std::string *err,
+                  TableSink *sink) {
   int rc = sparse_stage(cfg, sp, &plan->traj, err);
   if (rc < 3) return rc < 0 ? rc : -2;
   const SparseTrajectory &tr = plan->traj;
@@ -398,9 +414,20 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   rc = build_knots(cfg, par, sp.M, sp.s, tr, &plan->knots, err);
   if (rc < 0) return rc;
   const KnotSet &ks = plan->knots;
-  pack_spline(ks, &plan->packed);
   DenseParams &P = plan->P;
   std::memset(&P, 0, sizeof(P));
+  plan->tab_n = table_size(ks);
+  if (sink) {
+    plan->tab = sink->reserve(plan->tab_n);
+    if (!plan->tab) {
+      *err = "knot table storage unavailable";
+      return -8;
+    }
+  } else {
+    if (plan->table.size() < plan->tab_n) plan->table.resize(plan->tab_n);
+    plan->tab = plan->table.data();
+  }
+  fill_table(ks, plan->traj, plan->tab, &P.grid);
   const int N = cfg.length;
   P.N = N;
   P.i0 = ks.i0;
@@ -408,8 +435,8 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   P.dt = cfg.dt;
   P.mich = cfg.LISA ? 1 : 0;
   P.nmodes = nmodes;
-  P.t_first = ks.t[0];
-  P.t_last = ks.t[ks.K - 1];
+  P.t_first = ks.kt()[0];
+  P.t_last = ks.kt()[ks.K - 1];
   const double coslam = cos(par[4]);
   P.coslam = coslam;
   P.x0[0] = par[5];
@@ -423,8 +450,8 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   }
   auto tsample = [&](int i) { return i == 0 ? P.t_first : (i == N ? P.t_last : (double)(i - P.i0) * P.dt); };
   const double tp = tsample(ip);
-  const double ve = packed_eval(plan->packed, ks.K, 1, tp), ee = packed_eval(plan->packed, ks.K, 0, tp);
-  const double Me = packed_eval(plan->packed, ks.K, 2, tp), Se = packed_eval(plan->packed, ks.K, 3, tp);
+  const double ve = table_eval(plan->tab, P.grid, 1, tp), ee = table_eval(plan->tab, P.grid, 0, tp);
+  const double Me = table_eval(plan->tab, P.grid, 2, tp), Se = table_eval(plan->tab, P.grid, 3, tp);
   const int ib = (int)(10. / (PN::omega_phi(ve, ee, coslam, Se, Me) / 2. / kPi) / cfg.dt) + 1;
   P.ip = ip;
   P.ib = ib;
@@ -475,7 +502,51 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   P.cos_phK = cos(phiK);
   P.zeta = par[0] / sp.D / kGpc;
   P.Mphys = sp.M;
+  // Bessel evaluation of the mode sum: series for e below kSeriesMaxE
+  double e_hi = 0.0;
+  for (int k = 0; k < ks.K; k++) e_hi = std::max(e_hi, ks.ky(0)[k]);
+  const double e_bound = e_hi * 1.01 + 0.01;
+  P.bessel_series = (nmodes <= kMaxModes && e_bound <= kSeriesMaxE) ? 1 : 0;
+  if (P.bessel_series) bessel_series_terms(e_bound, nmodes, P.bterms);
   return 0;
+}
+
+void bessel_series_table(double *c) {
+  for (int k = 0; k <= kBesselKMax; k++) {
+    long double t = 1.0L;
+    for (int m = 0; m < kBesselMMax; m++) {
+      if (m > 0) t /= (long double)m * (long double)(k + m);
+      c[k * kBesselMMax + m] = (double)t;
+    }
+  }
+}
+
+void bessel_log_factorials(double *lgf) {
+  for (int k = 0; k <= kBesselKMax; k++) lgf[k] = (double)lgammal((long double)k + 1.0L);
+}
+
+// Smallest series length whose first omitted term, past the largest term,
+// is below 1e-18 of the leading one, for order k at argument x.
+static int series_length(int k, double x) {
+  const double q = 0.25 * x * x;
+  double t = 1.0;
+  for (int m = 1; m < kBesselMMax; m++) {
+    const double tn = t * q / ((double)m * (double)(k + m));
+    if (tn < t && tn < 1e-18) return m;
+    t = tn;
+  }
+  return kBesselMMax;
+}
+
+void bessel_series_terms(double e_bound, int n_modes, unsigned char *terms) {
+  int m0 = 2;
+  for (int k = 0; k <= 4; k++) m0 = std::max(m0, series_length(k, 2.0 * e_bound));
+  terms[0] = (unsigned char)m0;
+  for (int n = 1; n <= n_modes && n <= kMaxModes; n++) {
+    const double x = n * e_bound;
+    const int m = std::max(series_length(n + 1, x), series_length(n + 2, x));
+    terms[n] = (unsigned char)std::max(2, m);
+  }
 }
 
 }  // namespace aak

@@ -33,27 +33,57 @@ struct SparseTrajectory {
   int map_iters[100];
 };
 
+// Knots of the AK evolution.  The arrays span the whole index range j of the
+// evolution (2*j0 entries, reused between waveforms); the K knots handed to
+// the spline are entries [off, off+K), off = j_start.
 struct KnotSet {
   std::vector<double> t, e, v, M, S;
+  std::vector<double> eA, vA;  // raw AK evolution (before the fit corrections)
+  int off = 0;
   int K = 0, i0 = 0, j0 = 0, j_min = 0, j_max = 0, j_max_temp = 0, j_start = 0, j_end = 0, plunged = 0, points = 0;
+  int steps = 0;               // knots j0 .. j0+steps-1 take their times from dt_map
+  double dt_large = 0.0;       // knot spacing outside the dt_map part
   double t_fit_jmin = 0.0;
+  const double *kt() const { return t.data() + off; }
+  const double *ky(int a) const {
+    const std::vector<double> *arr[4] = {&e, &v, &M, &S};
+    return arr[a]->data() + off;
+  }
 };
+
+// Device knot table (kernels read it as uploaded):
+//   [0, 4K)        knot values y of e, v, M, S
+//   [4K, 8K)       natural-spline second-derivative coefficients c (y'' / 2)
+//   [8K, 8K+H)     knot times of the irregular head (H = head knots), see
+//                  KnotGrid for the regular part
 
 struct WaveformPlan {
   SparseTrajectory traj;
   KnotSet knots;
-  std::vector<double> packed;  // 17*K spline table for the device
+  std::vector<double> table;  // device knot table when no external buffer is given
+  double *tab = nullptr;      // the table actually filled (table.data() or external)
+  size_t tab_n = 0;
   DenseParams P;
   double par[12];
+};
+
+// Supplies the storage of a device knot table (e.g. pinned host memory).
+struct TableSink {
+  virtual double *reserve(size_t n) = 0;
+  virtual ~TableSink() {}
 };
 
 int sparse_stage(const WaveConfig &cfg, const SourceParams &sp, SparseTrajectory *out, std::string *err);
 int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double S_phys, const SparseTrajectory &tr,
                 KnotSet *ks, std::string *err);
+// Natural cubic splines of four arrays on one abscissa (one factorisation).
+int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs);
 int spline_natural(const double *x, const double *y, int n, double *c);
-void pack_spline(const KnotSet &ks, std::vector<double> *packed);
-double packed_eval(const std::vector<double> &packed, int K, int array, double t);
-int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *plan, std::string *err);
+size_t table_size(const KnotSet &ks);
+void fill_table(const KnotSet &ks, const SparseTrajectory &tr, double *tab, KnotGrid *grid);
+double table_eval(const double *tab, const KnotGrid &g, int array, double t);
+int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *plan, std::string *err,
+                  TableSink *sink = nullptr);
 double omega_phi(double v, double e, double cosiota, double s, double M);
 
 }  // namespace aak

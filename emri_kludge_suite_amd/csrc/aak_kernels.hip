@@ -45,6 +45,23 @@ __device__ inline double sample_time(const DenseParams &P, int i) {
   return (double)(i - P.i0) * P.dt;
 }
 
+// Spline values (e, v, M, S) of one sample straight from the knot table.
+__device__ inline void spline_sample(const DenseParams &P, const double *tab, double t, double *y) {
+  const KnotGrid &g = P.grid;
+  const int K = g.K;
+  const double *head = tab + (size_t)kTableArrays * K;
+  const int k = knot_index(g, head, t);
+  const double x0 = knot_time(g, head, k), x1 = knot_time(g, head, k + 1);
+  const double dl = t - x0;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const double *ya = tab + (size_t)a * K, *ca = tab + (size_t)(4 + a) * K;
+    double b, d;
+    spline_interval(x0, x1, ya[k], ya[k + 1], ca[k], ca[k + 1], &b, &d);
+    y[a] = ya[k] + dl * (b + dl * (ca[k] + dl * d));
+  }
+}
+
 __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuffers B) {
   __shared__ double s_x[kStageCap + 1];
   __shared__ double s_c[16][kStageCap];
@@ -53,11 +70,13 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   const int i_lo = blockIdx.x * kRatesThreads;
   if (i_lo > P.N) return;
   const int i_hi = min(i_lo + kRatesThreads - 1, P.N);
-  const int K = P.K;
-  const double *kt = B.knots;
+  const KnotGrid &g = P.grid;
+  const int K = g.K;
+  const double *tab = B.knots;
+  const double *head = tab + (size_t)kTableArrays * K;
   if (tid == 0) {
-    const int klo = knot_search(kt, 0, K - 1, sample_time(P, i_lo));
-    const int khi = knot_search(kt, 0, K - 1, sample_time(P, i_hi));
+    const int klo = knot_index(g, head, sample_time(P, i_lo));
+    const int khi = knot_index(g, head, sample_time(P, i_hi));
     s_lo = klo;
     s_cnt = khi - klo + 1;
   }
@@ -65,12 +84,23 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   const int klo = s_lo, cnt = s_cnt;
   const bool staged = (cnt <= kStageCap);
   if (staged) {
+    // one interval per thread: knot time and the cubic of each array
     for (int j = tid; j < cnt; j += kRatesThreads) {
-      s_x[j] = kt[klo + j];
+      const int k = klo + j;
+      const double x0 = knot_time(g, head, k), x1 = knot_time(g, head, k + 1);
+      s_x[j] = x0;
+      if (j == cnt - 1) s_x[cnt] = x1;
 #pragma unroll
-      for (int c = 0; c < 16; c++) s_c[c][j] = B.knots[(size_t)K * (1 + c) + klo + j];
+      for (int a = 0; a < 4; a++) {
+        const double *ya = tab + (size_t)a * K, *ca = tab + (size_t)(4 + a) * K;
+        double b, d;
+        spline_interval(x0, x1, ya[k], ya[k + 1], ca[k], ca[k + 1], &b, &d);
+        s_c[4 * a][j] = ya[k];
+        s_c[4 * a + 1][j] = b;
+        s_c[4 * a + 2][j] = ca[k];
+        s_c[4 * a + 3][j] = d;
+      }
     }
-    if (tid == 0) s_x[cnt] = (klo + cnt < K) ? kt[klo + cnt] : kt[K - 1];
   }
   __syncthreads();
   const int i = i_lo + tid;
@@ -84,13 +114,7 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
     for (int a = 0; a < 4; a++)
       y[a] = s_c[4 * a][k] + dl * (s_c[4 * a + 1][k] + dl * (s_c[4 * a + 2][k] + dl * s_c[4 * a + 3][k]));
   } else {
-    const int k = knot_search(kt, 0, K - 1, t);
-    const double dl = t - kt[k];
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const double *c = B.knots + (size_t)K * (1 + 4 * a);
-      y[a] = c[k] + dl * (c[K + k] + dl * (c[2 * (size_t)K + k] + dl * c[3 * (size_t)K + k]));
-    }
+    spline_sample(P, tab, t, y);
   }
   const double e = y[0], v = y[1], M = y[2], S = y[3];
   B.e[i] = e;
@@ -115,30 +139,93 @@ void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
 }
 
 // Non-positive radial rates take the previous (already corrected) value
-// (AAK.cc:254).  Rare: only reached past the plunge; processed in chunks
-// staged through LDS with one lane carrying the value.
-__global__ __launch_bounds__(256) void k_phidot_guard(DenseParams P, DenseBuffers B) {
-  __shared__ double buf[4096];
+// (AAK.cc:254): sample i in (i0, last_fold] with Phidot <= 0 (or NaN) gets the
+// rate of the last sample j in (i0, i] with Phidot > 0, or of i0 if there is
+// none.  That source index is a running maximum, computed as a two-pass
+// parallel scan; both passes return at once when k_rates counted no such
+// sample, so they are always enqueued and need no host round trip.
+constexpr int kGuardThreads = 256;
+constexpr int kGuardPer = 16;
+constexpr int kGuardBlock = kGuardThreads * kGuardPer;
+
+__device__ inline bool phidot_ok(double r) { return r > 0.; }
+
+__global__ __launch_bounds__(kGuardThreads) void k_guard_last(DenseParams P, DenseBuffers B) {
+  if (B.flags[0] == 0) return;
+  __shared__ int s_m[kGuardThreads];
+  const double *r = B.rate[1];
+  const int base = P.i0 + 1 + blockIdx.x * kGuardBlock + threadIdx.x * kGuardPer;
+  int last = -1;
+  for (int q = 0; q < kGuardPer; q++) {
+    const int i = base + q;
+    if (i <= P.last_fold && phidot_ok(r[i])) last = i;
+  }
+  s_m[threadIdx.x] = last;
+  __syncthreads();
+  for (int o = kGuardThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) B.guard_blk[blockIdx.x] = s_m[0];
+}
+
+__global__ __launch_bounds__(kGuardThreads) void k_guard_fix(DenseParams P, DenseBuffers B) {
+  if (B.flags[0] == 0) return;
+  __shared__ int s_m[kGuardThreads];
+  __shared__ int s_carry;
   double *r = B.rate[1];
-  double prev = r[P.i0];
-  for (int base = P.i0 + 1; base <= P.last_fold; base += 4096) {
-    const int n = min(4096, P.last_fold - base + 1);
-    for (int j = threadIdx.x; j < n; j += blockDim.x) buf[j] = r[base + j];
+  // last good index before this block
+  int c = P.i0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kGuardThreads) c = max(c, B.guard_blk[b]);
+  s_m[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = kGuardThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int j = 0; j < n; j++) {
-        if (!(buf[j] > 0.)) buf[j] = prev;
-        prev = buf[j];
-      }
+  }
+  if (threadIdx.x == 0) s_carry = s_m[0];
+  __syncthreads();
+  const int carry = s_carry;
+  __syncthreads();
+  const int base = P.i0 + 1 + blockIdx.x * kGuardBlock + threadIdx.x * kGuardPer;
+  double v[kGuardPer];
+  int last = -1;
+  for (int q = 0; q < kGuardPer; q++) {
+    const int i = base + q;
+    v[q] = (i <= P.last_fold) ? r[i] : 1.0;
+    if (i <= P.last_fold && phidot_ok(v[q])) last = i;
+  }
+  // inclusive max-scan of the per-thread last good index
+  s_m[threadIdx.x] = last;
+  __syncthreads();
+  for (int o = 1; o < kGuardThreads; o <<= 1) {
+    const int add = (threadIdx.x >= o) ? s_m[threadIdx.x - o] : -1;
+    __syncthreads();
+    s_m[threadIdx.x] = max(s_m[threadIdx.x], add);
+    __syncthreads();
+  }
+  int src = max(carry, threadIdx.x > 0 ? s_m[threadIdx.x - 1] : -1);
+  for (int q = 0; q < kGuardPer; q++) {
+    const int i = base + q;
+    if (i > P.last_fold) break;
+    if (phidot_ok(v[q])) {
+      src = i;
+    } else {
+      r[i] = r[src];
     }
-    __syncthreads();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) r[base + j] = buf[j];
-    __syncthreads();
   }
 }
 
+int guard_blocks(const DenseParams &P) {
+  const int n = P.last_fold - P.i0;
+  return n > 0 ? (n + kGuardBlock - 1) / kGuardBlock : 0;
+}
+
 void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
-  hipLaunchKernelGGL(k_phidot_guard, dim3(1), dim3(256), 0, s, P, B);
+  const int nb = guard_blocks(P);
+  if (nb == 0) return;
+  hipLaunchKernelGGL(k_guard_last, dim3(nb), dim3(kGuardThreads), 0, s, P, B);
+  hipLaunchKernelGGL(k_guard_fix, dim3(nb), dim3(kGuardThreads), 0, s, P, B);
 }
 
 // ------------------------------------------------------------------ folds
@@ -484,6 +571,7 @@ __global__ __launch_bounds__(256) void k_fold_serial(FoldArgs A, FoldWork W) {
   __shared__ double buf[4096];
   const int p = blockIdx.x;
   if (!W.status[p]) return;
+  if (threadIdx.x == 0) atomicAdd(W.fallbacks, 1);
   double x = A.x0[p];
   if (threadIdx.x == 0) A.out[p][A.i0] = x;
   for (int base = 0; base < A.nsteps; base += 4096) {
@@ -524,6 +612,16 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
   hipLaunchKernelGGL(k_fold_local<true>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_serial, dim3(3), dim3(256), 0, s, A, W);
   return 0;
+}
+
+// per-waveform counters (non-positive Phidot samples, fold fallbacks); a
+// kernel rather than a memset so the sequence can be captured in a graph
+__global__ void k_clear_flags(int *flags) {
+  if (threadIdx.x < 4) flags[threadIdx.x] = 0;
+}
+
+void launch_clear_flags(const DenseBuffers &B, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, s, B.flags);
 }
 
 // phases after the plunge buffer stay at their plunge values (AAK.cc:265-269)
@@ -696,7 +794,10 @@ __global__ __launch_bounds__(256) void k_modesum(DenseParams P, DenseBuffers B) 
     } else {
       nPhi = nn * Phi;
     }
-    bessel_five(nn, nn * e, J);
+    if (P.bessel_series)
+      bessel_five_series(nn, e, B.bessel_c, B.bessel_lgf, P.bterms[nn], P.bterms[0], J);
+    else
+      bessel_five(nn, nn * e, J);
     double snp, cnp;
     sincos(nPhi, &snp, &cnp);
     const double a = -nn * Amp * (J[0] - 2. * e * J[1] + 2. / nn * J[2] + 2. * e * J[3] - J[4]) * cnp;

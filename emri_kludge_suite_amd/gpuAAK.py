@@ -86,6 +86,29 @@ class GPUAAK:
         return self.Likelihood()
 
     # ------------------------------------------------------------ extensions
+    def LikelihoodBatch(self, params, return_status=False):
+        """Likelihoods of many templates against the current data.
+
+        ``params``: (n, 14) array in the gpu_gen_AAK order.  Returns an (n, 2)
+        array whose rows equal WaveformThroughLikelihood(*params[i]); rows of
+        templates whose host stage fails are NaN (their negative status codes
+        are returned as well with ``return_status=True``).  The host stages run
+        on a thread pool while the GPU works on earlier templates.
+        """
+        P = np.ascontiguousarray(params, dtype=np.float64)
+        if P.ndim != 2 or P.shape[1] != 14:
+            raise ValueError("LikelihoodBatch: params must have shape (n, 14)")
+        n = P.shape[0]
+        out = np.zeros((n, 2))
+        st = np.zeros(n, dtype=np.int32)
+        _lib.check(_lib.lib().aak_likelihood_batch(self._h, _lib.dptr(P), n, _lib.dptr(out), _lib.iptr(st)),
+                   "LikelihoodBatch")
+        return (out, st) if return_status else out
+
+    def set_host_threads(self, n):
+        """Threads of the batch host pool (0: AAK_HOST_THREADS or the core count, at most 16)."""
+        _lib.check(_lib.lib().aak_set_host_threads(self._h, int(n)), "set_host_threads")
+
     def set_nmax(self, n_max):
         """Override the harmonic count (0 restores (int)(30 e0), min 4)."""
         _lib.check(_lib.lib().aak_set_nmax(self._h, int(n_max)), "set_nmax")
@@ -107,6 +130,13 @@ class GPUAAK:
         buf = np.zeros(6)
         _lib.lib().aak_get_timings(self._h, _lib.dptr(buf), 6)
         keys = ("host", "rates", "fold", "modesum", "rfft", "reduce")
+        return dict(zip(keys, (float(x) for x in buf)))
+
+    def batch_timings(self):
+        """Stage time sums (ms) over the last LikelihoodBatch run with profiling on."""
+        buf = np.zeros(5)
+        _lib.check(_lib.lib().aak_get_batch_timings(self._h, _lib.dptr(buf), 5), "batch_timings")
+        keys = ("rates", "fold", "modesum", "rfft", "reduce")
         return dict(zip(keys, (float(x) for x in buf)))
 
     def dense_arrays(self):

@@ -20,9 +20,20 @@ int aak_test_host_plan(const double *params, double T_fit, int length, double dt
                        int backint, int n_max, double *traj_out, double *knots_out, int cap,
                        int *info, double *dparams);
 
-/* Natural-spline table of the host stage: packed[17*K] as uploaded to the device. */
-int aak_test_spline_table(const double *params, double T_fit, int length, double dt, int LISA,
-                          int backint, double *packed, int cap17);
+/* Device knot table of the host stage (y of e, v, M, S | their natural-spline
+ * coefficients c | head knot times); grid[0..2] = K, head offset, head length.
+ * Returns the table length. */
+int aak_test_knot_table(const double *params, double T_fit, int length, double dt, int LISA, int backint,
+                        double *table, int cap, int *grid);
+
+/* Knot interval and spline values (e, v, M, S) at n times, as the device
+ * evaluates them: k_out[n], y_out[4n]. */
+int aak_test_knot_eval(const double *params, double T_fit, int length, double dt, int LISA, int backint,
+                       const double *t, int n, int *k_out, double *y_out);
+
+/* J_{n-2..n+2}(n e) by the mode sum's series evaluation for eccentricities up
+ * to e_bound; returns the series length used. */
+int aak_test_bessel_five(int n, double e, double e_bound, double *J);
 
 /* One evaluation of the inspiral rates (dp/dt, de/dt, dcos(iota)/dt). */
 int aak_test_inspiral_rates(double p, double e, double cosi, double a, double *out);

@@ -54,6 +54,23 @@ int aak_likelihood(aak_handle h, double *like_out);
  * after aak_likelihood, as in the reference. */
 int aak_get_waveform(aak_handle h, double *t, double *hI, double *hII);
 
+/* Batched likelihoods against the data of aak_input_data: params holds n
+ * rows of the 14 gpu_gen_AAK parameters (reference order); like_out receives
+ * n rows {4 <h w, d w>, 4 <h w, h w>}.  Replaces the per-template loop of
+ * gpu_gen_AAK + Likelihood in the reference's caller (pygpuAAK.py:205-224,
+ * GPUAAK.pyx:84-93 WaveformThroughLikelihood); each row equals that pair of
+ * calls.  The host stages run on a pool of threads (aak_set_host_threads)
+ * while the GPU works on earlier templates.  status (optional, n ints) gets
+ * 0 or the negative code of a template whose host stage failed; its row is
+ * NaN.  Returns 0, or < 0 if the batch as a whole failed.  The waveform
+ * buffers are left in an unspecified state (call gpu_gen_AAK again before
+ * GetWaveform). */
+int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like_out, int *status);
+
+/* Host threads of the batch pool (0 = AAK_HOST_THREADS or the hardware
+ * concurrency, at most 16); no counterpart in the reference. */
+int aak_set_host_threads(aak_handle h, int n);
+
 /* OmegaPhi(v, e, cosiota, s, M) (gpuAAK.h:13, gpuAAK.cc:17-26). */
 double aak_OmegaPhi(double v, double e, double cosiota, double s, double M);
 
@@ -72,6 +89,10 @@ int aak_get_info(aak_handle h, int *info, int n);
  * rates+spline, phase folds, mode sum, rFFT, weighted reduction. */
 int aak_set_profiling(aak_handle h, int on);
 int aak_get_timings(aak_handle h, double *ms, int n);
+
+/* Stage time sums (ms) over the templates of the last batch run with
+ * profiling on: rates+spline, guard+folds, mode sum, rFFT, reduction. */
+int aak_get_batch_timings(aak_handle h, double *ms, int n);
 
 /* Message of the last failed call on this thread. */
 const char *aak_last_error(void);

@@ -1,0 +1,125 @@
+"""GPU: the batched likelihood path and the Phidot guard.
+
+aak_likelihood_batch runs the same kernels as gpu_gen_AAK + Likelihood, with
+the host stages on a thread pool and the knot-table uploads on a second
+stream; every row must equal the single-call result bit for bit, templates
+whose host stage fails must come back as NaN rows with their status, and
+repeated batches must agree.  The guard case checks a template with
+non-positive Phidot samples against the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+N2, DT2 = 2 ** 20, 15.0
+
+
+def _prior(rng, n):
+    out = []
+    for _ in range(n):
+        out.append([0.5, rng.uniform(0, 0.95), rng.uniform(7, 15), rng.uniform(1e-3, 0.5),
+                    10 ** rng.uniform(5, 7), rng.uniform(5, 50), rng.uniform(0, 2 * np.pi),
+                    rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi), np.arccos(rng.uniform(-1, 1)),
+                    rng.uniform(0, 2 * np.pi), np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi), 1.0])
+    return np.array(out)
+
+
+def _handle_with_data(N, dt, n_max, seed=2):
+    from emri_kludge_suite_amd import GPUAAK
+    from emri_kludge_suite_amd.LISAnoise import LISA_Noise, generate_noise_frequencies
+    freqs = generate_noise_frequencies(N * dt, 1 / dt)
+    freqs[0] = freqs[1] / 10.0
+    w = 1.0 / np.sqrt(LISA_Noise(freqs, dur=1)) * np.sqrt(dt / N)
+    g = GPUAAK(1.0, N // 10, N, 100.0, dt, True, False)
+    g.set_nmax(n_max)
+    rng = np.random.default_rng(seed)
+    while True:
+        inj = _prior(rng, 1)[0]
+        try:
+            g.gpu_gen_AAK(*inj)
+            break
+        except RuntimeError:
+            continue
+    _, hI, hII = g.GetWaveform()
+    g.input_data(np.fft.rfft(hI[:N]) * w, np.fft.rfft(hII[:N]) * w, w, w)
+    return g
+
+
+@pytest.mark.timeout(300)
+def test_batch_equals_single_calls():
+    print("case batch-vs-single N=2^20 dt=15 n_max=20 seeds 2/3", flush=True)
+    g = _handle_with_data(N2, DT2, 20)
+    P = _prior(np.random.default_rng(3), 10)
+    P[4, 2] = 3.0  # p0 inside the LSO: host stage fails
+    like, st = g.LikelihoodBatch(P, return_status=True)
+    print("status", st.tolist(), flush=True)
+    assert st[4] < 0 and np.all(np.isnan(like[4]))
+    for i in range(len(P)):
+        if st[i] < 0:
+            with pytest.raises(RuntimeError):
+                g.WaveformThroughLikelihood(*P[i])
+            continue
+        single = g.WaveformThroughLikelihood(*P[i])
+        assert np.array_equal(like[i], single), (i, like[i], single)
+
+
+@pytest.mark.timeout(300)
+def test_batch_repeatable_and_thread_count_independent():
+    print("case batch-repeat N=2^20 dt=15 n_max=12 seed 5", flush=True)
+    g = _handle_with_data(N2, DT2, 12, seed=4)
+    P = _prior(np.random.default_rng(5), 24)
+    a = g.LikelihoodBatch(P)
+    g.set_host_threads(1)
+    b = g.LikelihoodBatch(P)
+    g.set_host_threads(3)
+    c = g.LikelihoodBatch(P[::-1])[::-1]
+    ok = ~np.isnan(a[:, 0])
+    assert ok.sum() >= 12
+    assert np.array_equal(a[ok], b[ok]) and np.array_equal(a[ok], c[ok])
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+
+
+def test_batch_profiled_stage_times():
+    print("case batch-profiling N=2^18 dt=10", flush=True)
+    g = _handle_with_data(2 ** 18, 10.0, 8)
+    P = _prior(np.random.default_rng(6), 6)
+    g.set_profiling(True)
+    g.LikelihoodBatch(P)
+    tm = g.batch_timings()
+    print(tm, flush=True)
+    assert tm["modesum"] > 0 and tm["rates"] > 0 and tm["rfft"] > 0
+
+
+@pytest.mark.timeout(400)
+def test_phidot_guard_against_oracle():
+    # find a template of the production prior whose dense Phidot has non-positive
+    # samples in the folded range, and check the whole waveform against the oracle
+    from emri_kludge_suite_amd import GPUAAK
+    N, dt = 2 ** 21, 15.0
+    g = GPUAAK(1.0, N // 10, N, 100.0, dt, True, False)
+    g.set_nmax(8)
+    rng = np.random.default_rng(3)
+    found = None
+    for k in range(40):
+        pars = _prior(rng, 1)[0]
+        try:
+            g.gpu_gen_AAK(*pars)
+        except RuntimeError:
+            continue
+        if g.info()["nonpositive_phidot"] > 0:
+            found = pars
+            break
+    if found is None:
+        pytest.skip("no template with non-positive Phidot among the first 40 draws")
+    print("case guard pars=%s info=%s" % (np.array2string(found, precision=6), g.info()), flush=True)
+    ref = ol.o_aak_wave(list(found), length=N, dt=dt, LISA=True, n_max=8, want_dense=True)
+    dense = g.dense_arrays()
+    for name in ("gim", "Phi", "alp"):
+        assert np.array_equal(dense[name], ref["dense_" + name]), name
+    _, hI, hII = g.GetWaveform()
+    rel = np.sqrt((np.sum((hI[:N] - ref["hI"]) ** 2) + np.sum((hII[:N] - ref["hII"]) ** 2)) /
+                  (np.sum(ref["hI"] ** 2) + np.sum(ref["hII"] ** 2)))
+    assert rel <= 1e-10

@@ -99,22 +99,65 @@ def test_plunging_backint_host_stage_bit_identical(L):
     assert info[9] == 1, "this source must plunge inside the window"
 
 
-def test_spline_table_matches_the_natural_spline(L):
-    pars = _draw(np.random.default_rng(1))
-    K, _, kn, _, _ = _product_plan(L, pars, 2 ** 18, 10.0)
+def _knot_table(L, pars, N, dt, backint=False):
+    cap = 8 * 4_000_000 + 200
+    tab = np.zeros(cap)
+    grid = np.zeros(3, dtype=np.int32)
+    n = L.lib().aak_test_knot_table(L.dptr(np.array(pars)), 1.0, N, dt, 1, int(backint), L.dptr(tab), cap,
+                                    L.iptr(grid))
+    assert n > 0, L.last_error()
+    return tab[:n], grid
+
+
+@pytest.mark.parametrize("case", ["config1", "backint"])
+def test_knot_table_matches_the_natural_spline(L, case):
+    if case == "config1":
+        pars, N, dt, backint = _draw(np.random.default_rng(1)), 2 ** 18, 10.0, False
+    else:
+        pars, N, dt, backint = [0.5, 0.9, 7.2, 0.3, 3e5, 30.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0], 2 ** 22, 8.0, True
+    K, _, kn, info, _ = _product_plan(L, pars, N, dt, backint)
     cap = 4_000_000
-    packed = np.zeros(17 * K)
-    lib = L.lib()
-    assert lib.aak_test_spline_table(L.dptr(np.array(pars)), 1.0, 2 ** 18, 10.0, 1, 0, L.dptr(packed), 17 * K) >= 0
-    t = kn[:K]
-    assert np.array_equal(packed[:K], t)
+    tab, grid = _knot_table(L, pars, N, dt, backint)
+    assert grid[0] == K
+    t = np.ascontiguousarray(kn[:K])
     for a in range(4):
         y = np.ascontiguousarray(kn[(a + 1) * cap:(a + 1) * cap + K])
         c = np.zeros(K)
-        ol.oracle().o_spline_natural(ol._ptr(np.ascontiguousarray(t)), ol._ptr(y), K, ol._ptr(c))
-        block = packed[K + 4 * a * K:K + 4 * (a + 1) * K].reshape(4, K)
-        assert np.array_equal(block[0], y)
-        assert np.array_equal(block[2], c), "second-derivative coefficients differ for array %d" % a
+        ol.oracle().o_spline_natural(ol._ptr(t), ol._ptr(y), K, ol._ptr(c))
+        assert np.array_equal(tab[a * K:(a + 1) * K], y)
+        assert np.array_equal(tab[(4 + a) * K:(5 + a) * K], c), "spline coefficients differ for array %d" % a
+    # the head holds the dt_map knot times; the rest are regenerated on the device
+    steps, off = grid[2], grid[1]
+    assert np.array_equal(tab[8 * K:8 * K + steps], kn[off:off + steps] if off + steps <= K else tab[8 * K:8 * K + steps])
+
+
+@pytest.mark.parametrize("case", ["config1", "backint"])
+def test_device_spline_evaluation_matches_the_oracle(L, case):
+    # knot interval from the regenerated knot times and the cubic from (y, c): the
+    # interval of a bisection over the knots and the oracle's spline value, bit for bit
+    if case == "config1":
+        pars, N, dt, backint = _draw(np.random.default_rng(1)), 2 ** 18, 10.0, False
+    else:
+        pars, N, dt, backint = [0.5, 0.9, 7.2, 0.3, 3e5, 30.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0], 2 ** 22, 8.0, True
+    K, _, kn, info, _ = _product_plan(L, pars, N, dt, backint)
+    cap = 4_000_000
+    t = np.ascontiguousarray(kn[:K])
+    rng = np.random.default_rng(5)
+    ts = np.concatenate([t[:50], t[-50:], t[1:-1:max(1, K // 200)], rng.uniform(t[0], t[-1], 400),
+                         [t[0] - 1.0, t[-1] + 1.0], (np.arange(-20, 20) + info[2]) * dt])
+    ts = np.ascontiguousarray(ts)
+    k_out = np.zeros(ts.size, dtype=np.int32)
+    y_out = np.zeros(4 * ts.size)
+    assert L.lib().aak_test_knot_eval(L.dptr(np.array(pars)), 1.0, N, dt, 1, int(backint), L.dptr(ts), ts.size,
+                                      L.iptr(k_out), L.dptr(y_out)) == 0
+    ref_k = np.clip(np.searchsorted(t, ts, side="right") - 1, 0, K - 2)
+    assert np.array_equal(k_out, ref_k)
+    for a in range(4):
+        y = np.ascontiguousarray(kn[(a + 1) * cap:(a + 1) * cap + K])
+        c = np.zeros(K)
+        ol.oracle().o_spline_natural(ol._ptr(t), ol._ptr(y), K, ol._ptr(c))
+        ref = np.array([ol.oracle().o_spline_eval(ol._ptr(t), ol._ptr(y), ol._ptr(c), K, float(x)) for x in ts])
+        assert np.array_equal(y_out[a::4], ref), "array %d" % a
 
 
 def test_inspiral_rates_bit_identical(L):
