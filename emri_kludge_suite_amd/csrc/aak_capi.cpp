@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -86,15 +87,16 @@ struct PinnedSink : TableSink {
   double *reserve(size_t n) override { return buf->reserve(n) ? nullptr : buf->p; }
 };
 
-// One plan slot of the batch ring.  Template i uses slot i % R; a worker may
-// fill it once `ticket` equals i, the consumer takes it once `ready` equals i.
+// One plan slot of the batch pool.  Workers take a free slot, fill it with
+// the plan of the next template and queue it; the consumer takes queued
+// slots in completion order (a slow host stage does not hold up the GPU) and
+// frees them once their table upload is enqueued.
 struct RingSlot {
   WaveformPlan plan;
   PinnedBuf pin;
   hipEvent_t copied = nullptr;  // upload of this slot's table finished
   bool copy_pending = false;
-  long long ticket = 0;
-  long long ready = -1;
+  long long tpl = -1;           // template index held
   int rc = 0;
   ~RingSlot() {
     pin.release();
@@ -152,6 +154,8 @@ struct aak_handle_s {
   int host_threads = 0;
   std::vector<std::thread> workers;
   std::vector<std::unique_ptr<RingSlot>> ring;
+  std::vector<int> free_slots;
+  std::deque<int> ready_slots;
   std::mutex mu;
   std::condition_variable cv_work, cv_ready;
   bool stop = false;
@@ -375,15 +379,18 @@ static void worker_main(aak_handle h) {
     long long i;
     const double *params;
     RingSlot *slot;
+    int sidx;
     {
       std::unique_lock<std::mutex> lk(h->mu);
-      h->cv_work.wait(lk, [&] { return h->stop || (h->b_params && h->b_next < h->b_n); });
+      h->cv_work.wait(lk, [&] {
+        return h->stop || (h->b_params && h->b_next < h->b_n && !h->free_slots.empty());
+      });
       if (h->stop) return;
       i = h->b_next++;
       params = h->b_params;
-      slot = h->ring[(size_t)(i % (long long)h->ring.size())].get();
-      h->cv_work.wait(lk, [&] { return h->stop || slot->ticket == i; });
-      if (h->stop) return;
+      sidx = h->free_slots.back();
+      h->free_slots.pop_back();
+      slot = h->ring[(size_t)sidx].get();
     }
     if (slot->copy_pending) {
       (void)hipEventSynchronize(slot->copied);
@@ -397,9 +404,10 @@ static void worker_main(aak_handle h) {
     {
       std::lock_guard<std::mutex> lk(h->mu);
       slot->rc = rc;
-      slot->ready = i;
+      slot->tpl = i;
+      h->ready_slots.push_back(sidx);
     }
-    h->cv_ready.notify_all();
+    h->cv_ready.notify_one();
   }
 }
 
@@ -568,7 +576,6 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   }
   if (h->like_host.reserve((size_t)n * 2)) return fail(-100, "pinned host allocation failed");
   std::vector<int> st((size_t)n, 0);
-  const long long R = (long long)h->ring.size();
   const int prof = h->profiling;
   if (prof && h->batch_ev.size() < (size_t)6 * n) {
     const size_t old = h->batch_ev.size();
@@ -577,10 +584,9 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   }
   {
     std::lock_guard<std::mutex> lk(h->mu);
-    for (long long s = 0; s < R; s++) {
-      h->ring[s]->ticket = s;
-      h->ring[s]->ready = -1;
-    }
+    h->free_slots.clear();
+    h->ready_slots.clear();
+    for (int s = (int)h->ring.size() - 1; s >= 0; s--) h->free_slots.push_back(s);
     h->b_params = params;
     h->b_n = n;
     h->b_next = 0;
@@ -588,16 +594,20 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   h->cv_work.notify_all();
   int err_rc = 0;
   std::string err_msg;
-  for (long long i = 0; i < n; i++) {
-    RingSlot &slot = *h->ring[(size_t)(i % R)];
+  for (long long done = 0; done < n; done++) {
+    int sidx;
     {
       std::unique_lock<std::mutex> lk(h->mu);
-      h->cv_ready.wait(lk, [&] { return slot.ready == i; });
+      h->cv_ready.wait(lk, [&] { return !h->ready_slots.empty(); });
+      sidx = h->ready_slots.front();
+      h->ready_slots.pop_front();
     }
+    RingSlot &slot = *h->ring[(size_t)sidx];
+    const long long i = slot.tpl;
     st[i] = slot.rc < 0 ? slot.rc : 0;
     if (st[i] == 0 && err_rc == 0) {
       if (prof) h->evs = h->batch_ev.data() + 6 * i;
-      rc = enqueue_waveform(h, slot.plan, (int)(i & 1), slot.copied);
+      rc = enqueue_waveform(h, slot.plan, (int)(done & 1), slot.copied);
       if (!rc) {
         slot.copy_pending = true;
         rc = enqueue_likelihood(h, h->d_like_batch + 2 * i, false);
@@ -610,9 +620,9 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
     if (err_rc && st[i] == 0) st[i] = err_rc;
     {
       std::lock_guard<std::mutex> lk(h->mu);
-      slot.ticket = i + R;
+      h->free_slots.push_back(sidx);
     }
-    h->cv_work.notify_all();
+    h->cv_work.notify_one();
   }
   {
     std::lock_guard<std::mutex> lk(h->mu);

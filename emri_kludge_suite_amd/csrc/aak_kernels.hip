@@ -515,53 +515,149 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
 }
 
 // Serial replay of the item lists: exact start of every run, resolution of
-// the exact ties, FP64 replay of the steps near binade boundaries.
-__global__ __launch_bounds__(256) void k_fold_chain(FoldArgs A, FoldWork W) {
-  const int p = blockIdx.x;
-  if (W.status[p] || threadIdx.x != 0) return;
-  double x = A.x0[p];
-  A.out[p][A.i0] = x;
-  for (int b = 0; b < A.nblk; b++) {
-    const int ni = W.nitems[p * A.nblk + b];
-    const int nt = W.nties[p * A.nblk + b];
-    FoldItem *items = W.items + ((size_t)p * A.nblk + b) * kFoldMaxItems;
-    FoldTie *ties = W.ties + ((size_t)p * A.nblk + b) * kFoldMaxTies;
-    const int blen = min(kFoldBlock, A.nsteps - b * kFoldBlock);
-    for (int it = 0; it < ni; it++) {
-      const FoldItem I = items[it];
-      const int last = ((it + 1 < ni) ? items[it + 1].first : blen) - 1;
-      if (I.kind == 1) {
-        for (int sl = I.first; sl <= last; sl++) {
-          const int s = b * kFoldBlock + sl;
-          x = x + fold_incr(A.rate[p], A.i0, A.dir, s, A.dt);
-          A.out[p][out_index(A, s)] = x;
-        }
-        continue;
+// the exact ties, FP64 replay of the steps near binade boundaries.  The
+// block's threads first gather the per-block item and tie lists into LDS
+// (offsets by a block scan), so the one serial lane walks LDS instead of
+// issuing dependent global loads; the results go back to global memory for
+// the write pass.  Inputs too large for LDS take the same walk over global
+// memory.
+constexpr int kChainThreads = 256;
+constexpr int kChainMaxBlocks = 1024;
+constexpr int kChainMaxItems = 768;
+constexpr int kChainMaxTies = 512;
+
+// Walks items [0, ni) of block b; returns false on a failed binade check.
+template <class ItemPtr, class TiePtr>
+__device__ inline bool chain_block(const FoldArgs &A, int p, int b, ItemPtr items, int ni, TiePtr ties, int nt,
+                                   double &x) {
+  const int blen = min(kFoldBlock, A.nsteps - b * kFoldBlock);
+  for (int it = 0; it < ni; it++) {
+    const FoldItem I = items[it];
+    const int last = ((it + 1 < ni) ? items[it + 1].first : blen) - 1;
+    if (I.kind == 1) {
+      for (int sl = I.first; sl <= last; sl++) {
+        const int st = b * kFoldBlock + sl;
+        x = x + fold_incr(A.rate[p], A.i0, A.dir, st, A.dt);
+        A.out[p][out_index(A, st)] = x;
       }
-      int kx;
-      frexp(x, &kx);
-      if (kx != I.k || (x > 0.0 ? 1 : -1) != I.sgn) {
-        W.status[p] = 1;
-        return;
-      }
-      const long long m = (long long)ldexp(x, 53 - I.k);
-      items[it].m_start = m;
-      const int tend = (it + 1 < ni) ? items[it + 1].tie_first : nt;
-      long long ups = 0;
-      for (int q = I.tie_first; q < tend; q++) {
-        const long long val = m + ties[q].P + ups;
-        const int up = (int)(val & 1LL);
-        ties[q].up = up;
-        ups += up;
-      }
-      const long long mend = m + I.R + ups;
-      const long long am = mend < 0 ? -mend : mend;
-      if (am < (1LL << 52) || am >= (1LL << 53)) {
-        W.status[p] = 1;
-        return;
-      }
-      x = ldexp((double)mend, I.k - 53);
+      continue;
     }
+    int kx;
+    frexp(x, &kx);
+    if (kx != I.k || (x > 0.0 ? 1 : -1) != I.sgn) return false;
+    const long long m = (long long)ldexp(x, 53 - I.k);
+    items[it].m_start = m;
+    const int tend = (it + 1 < ni) ? items[it + 1].tie_first : nt;
+    long long ups = 0;
+    for (int q = I.tie_first; q < tend; q++) {
+      const long long val = m + ties[q].P + ups;
+      const int up = (int)(val & 1LL);
+      ties[q].up = up;
+      ups += up;
+    }
+    const long long mend = m + I.R + ups;
+    const long long am = mend < 0 ? -mend : mend;
+    if (am < (1LL << 52) || am >= (1LL << 53)) return false;
+    x = ldexp((double)mend, I.k - 53);
+  }
+  return true;
+}
+
+// exclusive scan of v[0..n) in place (n <= kChainMaxBlocks), returns the total
+__device__ inline int block_exclusive_scan(int *v, int n, int *s_tmp) {
+  const int tid = threadIdx.x;
+  const int per = (n + kChainThreads - 1) / kChainThreads;
+  const int lo = min(n, tid * per), hi = min(n, lo + per);
+  int sum = 0;
+  for (int j = lo; j < hi; j++) sum += v[j];
+  s_tmp[tid] = sum;
+  __syncthreads();
+  for (int o = 1; o < kChainThreads; o <<= 1) {
+    const int add = (tid >= o) ? s_tmp[tid - o] : 0;
+    __syncthreads();
+    s_tmp[tid] += add;
+    __syncthreads();
+  }
+  int run = s_tmp[tid] - sum;
+  for (int j = lo; j < hi; j++) {
+    const int c = v[j];
+    v[j] = run;
+    run += c;
+  }
+  const int total = s_tmp[kChainThreads - 1];
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWork W) {
+  __shared__ int s_io[kChainMaxBlocks + 1], s_to[kChainMaxBlocks + 1];
+  __shared__ int s_tmp[kChainThreads];
+  __shared__ FoldItem s_items[kChainMaxItems];
+  __shared__ FoldTie s_ties[kChainMaxTies];
+  __shared__ int s_ok;
+  const int p = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (W.status[p]) return;
+  const int nblk = A.nblk;
+  const int *gni = W.nitems + p * nblk, *gnt = W.nties + p * nblk;
+  FoldItem *gitems = W.items + (size_t)p * nblk * kFoldMaxItems;
+  FoldTie *gties = W.ties + (size_t)p * nblk * kFoldMaxTies;
+  bool fits = nblk <= kChainMaxBlocks;
+  int TI = 0, TT = 0;
+  if (fits) {
+    for (int b = tid; b < nblk; b += kChainThreads) {
+      s_io[b] = gni[b];
+      s_to[b] = gnt[b];
+    }
+    __syncthreads();
+    TI = block_exclusive_scan(s_io, nblk, s_tmp);
+    TT = block_exclusive_scan(s_to, nblk, s_tmp);
+    fits = TI <= kChainMaxItems && TT <= kChainMaxTies;
+  }
+  if (!fits) {
+    // lists too long for LDS: the same walk over global memory
+    if (tid != 0) return;
+    double x = A.x0[p];
+    A.out[p][A.i0] = x;
+    for (int b = 0; b < nblk; b++) {
+      if (!chain_block(A, p, b, gitems + (size_t)b * kFoldMaxItems, gni[b],
+                       gties + (size_t)b * kFoldMaxTies, gnt[b], x)) {
+        W.status[p] = 1;
+        return;
+      }
+    }
+    return;
+  }
+  if (tid == 0) {
+    s_io[nblk] = TI;
+    s_to[nblk] = TT;
+  }
+  __syncthreads();
+  for (int b = tid; b < nblk; b += kChainThreads) {
+    const int io = s_io[b], ni = s_io[b + 1] - io;
+    for (int j = 0; j < ni; j++) s_items[io + j] = gitems[(size_t)b * kFoldMaxItems + j];
+    const int to = s_to[b], nt = s_to[b + 1] - to;
+    for (int j = 0; j < nt; j++) s_ties[to + j] = gties[(size_t)b * kFoldMaxTies + j];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double x = A.x0[p];
+    A.out[p][A.i0] = x;
+    int ok = 1;
+    for (int b = 0; b < nblk && ok; b++) {
+      const int io = s_io[b], to = s_to[b];
+      ok = chain_block(A, p, b, s_items + io, s_io[b + 1] - io, s_ties + to, s_to[b + 1] - to, x) ? 1 : 0;
+    }
+    s_ok = ok;
+    if (!ok) W.status[p] = 1;
+  }
+  __syncthreads();
+  if (!s_ok) return;
+  for (int b = tid; b < nblk; b += kChainThreads) {
+    const int io = s_io[b], ni = s_io[b + 1] - io;
+    for (int j = 0; j < ni; j++) gitems[(size_t)b * kFoldMaxItems + j].m_start = s_items[io + j].m_start;
+    const int to = s_to[b], nt = s_to[b + 1] - to;
+    for (int j = 0; j < nt; j++) gties[(size_t)b * kFoldMaxTies + j].up = s_ties[to + j].up;
   }
 }
 
@@ -608,7 +704,7 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
   hipLaunchKernelGGL(k_fold_sums, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_plan, dim3(3), dim3(1024), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_local<false>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
-  hipLaunchKernelGGL(k_fold_chain, dim3(3), dim3(256), 0, s, A, W);
+  hipLaunchKernelGGL(k_fold_chain, dim3(3), dim3(kChainThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_local<true>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_serial, dim3(3), dim3(256), 0, s, A, W);
   return 0;
