@@ -45,6 +45,7 @@ def lib():
         "aak_test_knot_table": (_i, [_pd, _d, _i, _d, _i, _i, _pd, _i, _pi]),
         "aak_test_knot_eval": (_i, [_pd, _d, _i, _d, _i, _i, _pd, _i, _pi, _pd]),
         "aak_test_bessel_five": (_i, [_i, _d, _d, _pd]),
+        "aak_test_taylor_amplitudes": (_i, [_i, _i, _d, _d, _d, _pd]),
         "aak_likelihood_batch": (_i, [C.c_void_p, _pd, _i, _pd, _pi]),
         "aak_set_host_threads": (_i, [C.c_void_p, _i]),
         "aak_get_batch_timings": (_i, [C.c_void_p, _pd, _i]),

@@ -22,7 +22,7 @@
 
 namespace aak {
 
-constexpr int kBesselKMax = 130;  // orders 0..130: harmonics up to kMaxModes
+constexpr int kBesselKMax = 140;  // orders 0..140: harmonics up to kMaxModes plus Taylor orders
 constexpr int kBesselMMax = 64;   // series terms 0..63
 constexpr int kMaxModes = 128;
 constexpr double kSeriesMaxE = 0.66;
@@ -81,6 +81,149 @@ AAK_HD void bessel_five_series(int n, double e, const double *ctab, const double
   J[2] = j0;
   J[3] = j1;
   J[4] = j2;
+}
+
+// ---------------------------------------------------------- Taylor in e
+// alpha_n(e) = -n (J_{n-2} - 2e J_{n-1} + 2/n J_n + 2e J_{n+1} - J_{n+2}),
+// beta_n(e)  = -n sqrt(1-e^2) (J_{n-2} - 2 J_n + J_{n+2}),  gamma_n(e) = 2 J_n
+// (all J at argument n e; J_{-1} = -J_1) are the e-dependent factors of the
+// harmonic coefficients a_n, b_n, c_n (AAK.cc:493-504).  Their Taylor
+// coefficients in delta = e - e_ref follow from
+//   d^j/de^j J_k(ne) = (n/2)^j sum_i (-1)^i C(j,i) J_{k-j+2i}(n e_ref).
+constexpr int kTaylorMax = 8;
+constexpr int kTaylorOrders = 5 + 2 * kTaylorMax;
+
+// J at orders n-2-p .. n+2+p (out[q] = J_{n-2-p+q}) at x = n e_ref, negative
+// orders by J_{-m} = (-1)^m J_m.  Runs that reach order 0 (low harmonics,
+// where J_0 ~ 1) use Miller's recurrence normalised with J_0 + 2 sum J_2k = 1;
+// the others take the two highest orders from the series and recur down
+// (their values are small enough that the prefactor's relative error does not
+// matter absolutely).  M: series length valid for orders >= n+1.
+AAK_HD void bessel_run(int n, int p, double e_ref, const double *ctab, const double *lgf, int M, double *out) {
+  const int kmin = n - 2 - p, kmax = n + 2 + p, cnt = kmax - kmin + 1;
+  const double x = n * e_ref;
+  if (!(e_ref > 1e-150)) {
+    for (int q = 0; q < cnt; q++) out[q] = (kmin + q == 0) ? 1.0 : 0.0;
+    return;
+  }
+  if (kmin <= 0) {
+    int top = kmax + 30 + (int)x;
+    top += (top & 1);
+    const double inv = 1.0 / x;
+    double jp1 = 0.0, j = 1.0e-280, sum = 0.0;
+    for (int k = top; k >= 1; k--) {
+      if (k <= kmax) out[k - kmin] = j;
+      if ((k & 1) == 0) sum += 2.0 * j;
+      const double jm1 = (2.0 * k) * inv * j - jp1;
+      jp1 = j;
+      j = jm1;
+      if (fabs(j) > 1.0e250) {
+        j *= 1.0e-250;
+        jp1 *= 1.0e-250;
+        sum *= 1.0e-250;
+        for (int q = k - kmin; q < cnt && q >= 0; q++) out[q] *= 1.0e-250;
+      }
+    }
+    out[-kmin] = j;
+    sum += j;
+    const double norm = 1.0 / sum;
+    for (int q = -kmin; q < cnt; q++) out[q] *= norm;
+    for (int k = kmin; k < 0; k++) {
+      const int m = -k;
+      const double v = out[m - kmin];
+      out[k - kmin] = (m & 1) ? -v : v;
+    }
+    return;
+  }
+  const double y = -0.25 * x * x, lhx = log(0.5 * x);
+  const double ptop = exp(kmax * lhx - lgf[kmax]);
+  const double jt = ptop * bessel_series_sum(ctab + kmax * kBesselMMax, M, y);
+  const double jt1 = ptop * ((double)kmax / (0.5 * x)) * bessel_series_sum(ctab + (kmax - 1) * kBesselMMax, M, y);
+  out[cnt - 1] = jt;
+  out[cnt - 2] = jt1;
+  const double inv = 1.0 / x;
+  double jk1 = jt, jk = jt1;
+  const int klo = kmin > 0 ? kmin : 0;
+  for (int k = kmax - 1; k > klo; k--) {
+    const double jm = (2.0 * k) * inv * jk - jk1;
+    out[k - 1 - kmin] = jm;
+    jk1 = jk;
+    jk = jm;
+  }
+}
+
+// Taylor coefficients 0..p of sqrt(1 - (e_ref + d)^2) into sq.
+AAK_HD void taylor_sqrt1me2(double e_ref, int p, double *sq) {
+  const double g0 = 1. - e_ref * e_ref, g1 = -2. * e_ref, g2 = -1.;
+  sq[0] = sqrt(g0);
+  for (int j = 1; j <= p; j++) {
+    double acc = (j == 1 ? g1 : (j == 2 ? g2 : 0.0));
+    for (int q = 1; q < j; q++) acc -= sq[q] * sq[j - q];
+    sq[j] = acc / (2. * sq[0]);
+  }
+}
+
+// coef[f * (kTaylorMax + 1) + j], f = alpha, beta, gamma; j = 0..kTaylorMax
+// (zero above p).  sq: Taylor coefficients of sqrt(1 - e^2) at e_ref.
+AAK_HD void taylor_amplitudes(int n, int p, double e_ref, const double *sq, const double *ctab, const double *lgf,
+                              int M, double *coef) {
+  double Jr[kTaylorOrders];
+  bessel_run(n, p, e_ref, ctab, lgf, M, Jr);
+  double t[5][kTaylorMax + 1];
+  double hn = 1.0;  // (n/2)^j / j!
+  for (int j = 0; j <= p; j++) {
+    for (int k = 0; k < 5; k++) {
+      double acc = 0.0, c = 1.0;  // c = C(j, ii)
+      for (int ii = 0; ii <= j; ii++) {
+        const double v = Jr[k - j + 2 * ii + p];  // J_{n-2+k-j+2ii}
+        acc += (ii & 1) ? -c * v : c * v;
+        c = c * (j - ii) / (ii + 1);
+      }
+      t[k][j] = hn * acc;
+    }
+    hn = hn * (0.5 * n) / (j + 1);
+  }
+  const double dn = (double)n;
+  for (int j = 0; j <= kTaylorMax; j++) {
+    double *ca = coef + j, *cb = coef + (kTaylorMax + 1) + j, *cg = coef + 2 * (kTaylorMax + 1) + j;
+    if (j > p) {
+      *ca = 0.0;
+      *cb = 0.0;
+      *cg = 0.0;
+      continue;
+    }
+    const double e1 = e_ref * t[1][j] + (j ? t[1][j - 1] : 0.0);
+    const double e3 = e_ref * t[3][j] + (j ? t[3][j - 1] : 0.0);
+    *ca = -dn * (t[0][j] - 2. * e1 + 2. / dn * t[2][j] + 2. * e3 - t[4][j]);
+    double bsum = 0.0;
+    for (int q = 0; q <= j; q++) bsum += sq[q] * (t[0][j - q] - 2. * t[2][j - q] + t[4][j - q]);
+    *cb = -dn * bsum;
+    *cg = 2. * t[2][j];
+  }
+}
+
+// Horner evaluation of the three expansions at delta.
+AAK_HD void taylor_eval(const double *coef, int p, double d, double *abg) {
+  double al = coef[p], be = coef[kTaylorMax + 1 + p], ga = coef[2 * (kTaylorMax + 1) + p];
+  for (int j = p - 1; j >= 0; j--) {
+    al = fma(al, d, coef[j]);
+    be = fma(be, d, coef[kTaylorMax + 1 + j]);
+    ga = fma(ga, d, coef[2 * (kTaylorMax + 1) + j]);
+  }
+  abg[0] = al;
+  abg[1] = be;
+  abg[2] = ga;
+}
+
+// Taylor order for harmonics up to n_max and |delta| <= del (-1: none <= kTaylorMax).
+AAK_HD int taylor_order(int n_max, double del) {
+  const double nm = (double)n_max, z = nm * del;
+  double term = 7.0 * nm * z;  // 7 n (n delta)^{p+1} / (p+1)! at p = 0
+  for (int q = 0; q <= kTaylorMax; q++) {
+    if (term <= 1e-17) return q;
+    term = term * z / (q + 2);
+  }
+  return -1;
 }
 
 }  // namespace aak

@@ -797,6 +797,36 @@ int aak_test_bessel_five(int n, double e, double e_bound, double *J) {
   return terms[n];
 }
 
+int aak_test_taylor_amplitudes(int n_max, int n, double e_ref, double del, double d, double *out) {
+  if (n < 1 || n > n_max || n_max > kMaxModes) return fail(-2, "harmonic outside 1..n_max (<= 128)");
+  if (!(e_ref >= 0.0) || !(del >= 0.0) || e_ref + del > kSeriesMaxE) return fail(-2, "eccentricity range");
+  static std::vector<double> tab;
+  static std::once_flag once;
+  std::call_once(once, []() {
+    const size_t nc = (size_t)(kBesselKMax + 1) * kBesselMMax;
+    tab.resize(nc + kBesselKMax + 1);
+    bessel_series_table(tab.data());
+    bessel_log_factorials(tab.data() + nc);
+  });
+  const double *ctab = tab.data(), *lgf = tab.data() + (size_t)(kBesselKMax + 1) * kBesselMMax;
+  unsigned char terms[kMaxModes + 1];
+  bessel_series_terms(e_ref + del, n_max, terms);
+  const int p = taylor_order(n_max, del);
+  if (p < 0) return fail(-2, "no Taylor order for this range");
+  double sq[kTaylorMax + 1], coef[3 * (kTaylorMax + 1)];
+  taylor_sqrt1me2(e_ref, p, sq);
+  taylor_amplitudes(n, p, e_ref, sq, ctab, lgf, terms[n], coef);
+  taylor_eval(coef, p, d, out);
+  // direct evaluation at e = e_ref + d
+  const double e = e_ref + d;
+  double J[5];
+  bessel_five_series(n, e, ctab, lgf, terms[n], terms[0], J);
+  out[3] = -n * (J[0] - 2. * e * J[1] + 2. / n * J[2] + 2. * e * J[3] - J[4]);
+  out[4] = -n * sqrt(1 - e * e) * (J[0] - 2. * J[2] + J[4]);
+  out[5] = 2. * J[2];
+  return p;
+}
+
 int aak_test_inspiral_rates(double p, double e, double cosi, double a, double *out) {
   return inspiral_rates<HostCx>(p, e, cosi, a, out);
 }

@@ -792,31 +792,27 @@ __device__ inline double nrm2_scaled(const double *v) {
   return scale * sqrt(ssq);
 }
 
-__global__ __launch_bounds__(256) void k_modesum(DenseParams P, DenseBuffers B) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.N + 2) return;
-  double *hI = B.h, *hII = B.h + (P.N + 2);
-  if (i >= P.N) {
-    B.t[i] = 0.0;
-    hI[i] = 0.0;
-    hII[i] = 0.0;
-    return;
-  }
+// Per-sample part of the mode sum (AAK.cc:425-481): everything that does not
+// depend on the harmonic, folded into the weights of a_n, b_n, c_n in h_I and
+// h_II, so that h = Amp * sum_n [cos(nPhi)(alpha_n W_a + gamma_n W_c) +
+// sin(nPhi) beta_n W_b] with a_n = Amp alpha_n cos(nPhi) etc. (AAK.cc:493-504).
+struct SampleGeom {
+  double e, Phi, nu, gimdot, Amp, ti, cosorbphs;
+  double W[6];  // a, b, c weights of h_I, then of h_II
+};
+
+__device__ inline void sample_geometry(const DenseParams &P, const DenseBuffers &B, int i, SampleGeom &G) {
   const double ti = sample_time(P, i);
-  B.t[i] = ti;
-  if (i >= P.ip + P.ib) {
-    hI[i] = 0.0;
-    hII[i] = 0.0;
-    return;
-  }
   const double e = B.e[i], v = B.v[i], M = B.M[i], S = B.S[i];
-  const double gim = B.phase[0][i], Phi = B.phase[1][i], alp = B.phase[2][i];
-  const double gimdot = B.rate[0][i];
-  double nu;
+  const double gim = B.phase[0][i], alp = B.phase[2][i];
+  G.ti = ti;
+  G.e = e;
+  G.Phi = B.phase[1][i];
+  G.gimdot = B.rate[0][i];
   if (i == 0) {
-    nu = PN::drdm(v, e, P.coslam, S) / PN::dtdm(v, e, P.coslam, S) / (2. * kPi * M);
+    G.nu = PN::drdm(v, e, P.coslam, S) / PN::dtdm(v, e, P.coslam, S) / (2. * kPi * M);
   } else {
-    nu = B.rate[1][i] / 2. / kPi;
+    G.nu = B.rate[1][i] / 2. / kPi;
   }
   const double coslam = P.coslam, sinlam = P.sinlam;
   const double cosqS = P.cosqS, sinqS = P.sinqS, cosqK = P.cosqK, sinqK = P.sinqK;
@@ -835,7 +831,7 @@ __global__ __launch_bounds__(256) void k_modesum(DenseParams P, DenseBuffers B) 
   const double beta = atan2(betaup, betadown);
   const double gam = 2. * (gim + beta);
   const double cos2gam = cos(gam), sin2gam = sin(gam);
-  double cosorbphs = 0.0, FplusI, FcrosI, FplusII, FcrosII;
+  double cosorbphs = 0.0, FplusI, FcrosI, FplusII, FcrosII, h3;
   if (P.mich) {
     const double orbphs = 2. * kPi * ti / kYear;
     cosorbphs = cos(orbphs - P.phiS);
@@ -852,16 +848,19 @@ __global__ __launch_bounds__(256) void k_modesum(DenseParams P, DenseBuffers B) 
     FcrosI = cosq1 * cos2phi * sin2psi + cosq * sin2phi * cos2psi;
     FplusII = cosq1 * sin2phi * cos2psi + cosq * cos2phi * sin2psi;
     FcrosII = cosq1 * sin2phi * sin2psi - cosq * cos2phi * cos2psi;
+    h3 = halfsqrt3;
   } else {
     FplusI = P.ldc_c2psi;
     FcrosI = -P.ldc_s2psi;
     FplusII = P.ldc_s2psi;
     FcrosII = P.ldc_c2psi;
+    h3 = 1.0;
   }
-  const double Amp = pow(PN::omega_phi(v, e, coslam, S, M) * P.Mphys * kMsunSec, 2. / 3.) * P.zeta;
+  G.cosorbphs = cosorbphs;
+  G.Amp = pow(PN::omega_phi(v, e, coslam, S, M) * P.Mphys * kMsunSec, 2. / 3.) * P.zeta;
   // rotation to the wave frame (KSParMap.cc:532-574); n-independent
   double L[3], nxL[3], nxS[3];
-  const double ca = cos(alp), sa = sin(alp);
+  const double ca = cosalp, sa = sinalp;
   L[0] = P.cosiota_L * P.sin_thK * P.cos_phK + P.siniota_L * (sa * P.sin_phK - ca * P.cos_thK * P.cos_phK);
   L[1] = P.cosiota_L * P.sin_thK * P.sin_phK - P.siniota_L * (sa * P.cos_phK + ca * P.cos_thK * P.sin_phK);
   L[2] = P.cosiota_L * P.cos_thK + P.siniota_L * ca * P.sin_thK;
@@ -878,55 +877,190 @@ __global__ __launch_bounds__(256) void k_modesum(DenseParams P, DenseBuffers B) 
   sinrot -= 0.0 + Sv[0] * nxL[0] + Sv[1] * nxL[1] + Sv[2] * nxL[2];
   sinrot /= norm;
   const double rot0 = 2. * cosrot * cosrot - 1., rot1 = cosrot * sinrot;
-  const double sq1me2 = sqrt(1 - e * e);
-  double accI = 0.0, accII = 0.0;
-  double J[5];
+  // h = Ap0 X + Ac0 Y per channel, with Ap0, Ac0 linear in (a, b, c)
+  const double XI = h3 * (FplusI * rot0 - FcrosI * rot1), YI = h3 * (FplusI * rot1 + FcrosI * rot0);
+  const double XII = h3 * (FplusII * rot0 - FcrosII * rot1), YII = h3 * (FplusII * rot1 + FcrosII * rot0);
+  const double pa = -(1. + Ldotn2) * cos2gam, pb = (1. + Ldotn2) * sin2gam, pc = 1. - Ldotn2;
+  const double qa = 2. * Ldotn * sin2gam, qb = 2. * Ldotn * cos2gam;
+  G.W[0] = pa * XI + qa * YI;
+  G.W[1] = pb * XI + qb * YI;
+  G.W[2] = pc * XI;
+  G.W[3] = pa * XII + qa * YII;
+  G.W[4] = pb * XII + qb * YII;
+  G.W[5] = pc * XII;
+}
+
+// Planck taper between plunge and zero time (AAK.cc:535-547)
+__device__ inline double taper(const DenseParams &P, int i, double ti) {
+  if (i <= P.ip) return 1.0;
+  const double tp = P.t_plunge, tz = P.t_zero;
+  return exp((tp - tz) / (ti - tp) + (tp - tz) / (ti - tz)) + 1.;
+}
+
+// Doppler-shifted harmonic phase exactly as the reference rounds it.
+__device__ inline double harmonic_phase(const DenseParams &P, const SampleGeom &G, double nn, double gdpi) {
+  if (P.mich) {
+    const double fn = nn * G.nu + gdpi;
+    const double Doppler = 2. * kPi * fn * kAUsec * P.sinqS * G.cosorbphs;
+    return nn * G.Phi + Doppler;
+  }
+  return nn * G.Phi;
+}
+
+// One sample, Bessel functions per harmonic (series or Miller) and one
+// sincos per harmonic: used where the Taylor form is not applicable.
+__device__ inline void modesum_direct(const DenseParams &P, const DenseBuffers &B, const SampleGeom &G, double *hI,
+                                      double *hII) {
+  const double e = G.e, sq1me2 = sqrt(1 - e * e), gdpi = G.gimdot / kPi;
+  double accI = 0.0, accII = 0.0, J[5];
   for (int nn = 1; nn <= P.nmodes; nn++) {
-    double nPhi;
-    if (P.mich) {
-      const double fn = nn * nu + gimdot / kPi;
-      const double Doppler = 2. * kPi * fn * kAUsec * sinqS * cosorbphs;
-      nPhi = nn * Phi + Doppler;
-    } else {
-      nPhi = nn * Phi;
-    }
+    const double nPhi = harmonic_phase(P, G, (double)nn, gdpi);
     if (P.bessel_series)
       bessel_five_series(nn, e, B.bessel_c, B.bessel_lgf, P.bterms[nn], P.bterms[0], J);
     else
       bessel_five(nn, nn * e, J);
     double snp, cnp;
     sincos(nPhi, &snp, &cnp);
-    const double a = -nn * Amp * (J[0] - 2. * e * J[1] + 2. / nn * J[2] + 2. * e * J[3] - J[4]) * cnp;
-    const double bb = -nn * Amp * sq1me2 * (J[0] - 2. * J[2] + J[4]) * snp;
-    const double c = 2. * Amp * J[2] * cnp;
-    const double Ap0 = -(1. + Ldotn2) * (a * cos2gam - bb * sin2gam) + c * (1 - Ldotn2);
-    const double Ac0 = 2. * Ldotn * (bb * cos2gam + a * sin2gam);
-    const double Ap = Ap0 * rot0 + Ac0 * rot1;
-    const double Ac = Ap0 * (-rot1) + Ac0 * rot0;
-    double hnI, hnII;
-    if (P.mich) {
-      hnI = halfsqrt3 * (FplusI * Ap + FcrosI * Ac);
-      hnII = halfsqrt3 * (FplusII * Ap + FcrosII * Ac);
-    } else {
-      hnI = FplusI * Ap + FcrosI * Ac;
-      hnII = FplusII * Ap + FcrosII * Ac;
+    const double al = -nn * (J[0] - 2. * e * J[1] + 2. / nn * J[2] + 2. * e * J[3] - J[4]);
+    const double be = -nn * sq1me2 * (J[0] - 2. * J[2] + J[4]);
+    const double ga = 2. * J[2];
+    accI += cnp * (al * G.W[0] + ga * G.W[2]) + snp * (be * G.W[1]);
+    accII += cnp * (al * G.W[3] + ga * G.W[5]) + snp * (be * G.W[4]);
+  }
+  *hI = accI;
+  *hII = accII;
+}
+
+// ------------------------------------------------------------------ Taylor form
+// Inside one block of samples e varies by a few 1e-5: the amplitude factors
+// alpha_n, beta_n, gamma_n are expanded in delta = e - e_ref once per block
+// (taylor_amplitudes) and evaluated per sample by Horner.  The order p is the
+// smallest with remainder bound 7 n (n delta)^{p+1}/(p+1)! <= 1e-17.
+constexpr int kMsThreads = 256;
+constexpr int kTaylorModes = 128;
+
+__global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuffers B) {
+  __shared__ double s_coef[kTaylorModes][3][kTaylorMax + 1];
+  __shared__ double s_sq[kTaylorMax + 1];
+  __shared__ double s_min[kMsThreads / 64], s_max[kMsThreads / 64];
+  __shared__ double s_eref;
+  __shared__ int s_p;
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * kMsThreads + tid;
+  double *hI = B.h, *hII = B.h + (P.N + 2);
+  const int n_end = min(P.N, P.ip + P.ib);  // samples [0, n_end) carry the signal
+  const bool active = i < n_end;
+  // block range of e over the active samples
+  double emin = active ? B.e[i] : 1e300, emax = active ? B.e[i] : -1e300;
+  for (int o = 32; o > 0; o >>= 1) {
+    emin = fmin(emin, __shfl_xor(emin, o));
+    emax = fmax(emax, __shfl_xor(emax, o));
+  }
+  if ((tid & 63) == 0) {
+    s_min[tid >> 6] = emin;
+    s_max[tid >> 6] = emax;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double lo = s_min[0], hi = s_max[0];
+    for (int w = 1; w < kMsThreads / 64; w++) {
+      lo = fmin(lo, s_min[w]);
+      hi = fmax(hi, s_max[w]);
     }
-    accI += hnI;
-    accII += hnII;
+    int p = -1;
+    if (lo <= hi && P.bessel_series && P.nmodes <= kTaylorModes && hi <= kSeriesMaxE) {
+      s_eref = 0.5 * (lo + hi);
+      p = taylor_order(P.nmodes, 0.5 * (hi - lo));
+      if (p >= 0) {
+        double sq[kTaylorMax + 1];
+        taylor_sqrt1me2(s_eref, p, sq);
+        for (int j = 0; j <= kTaylorMax; j++) s_sq[j] = j <= p ? sq[j] : 0.0;
+      }
+    }
+    s_p = p;
   }
-  if (i > P.ip) {
-    const double tp = P.t_plunge, tz = P.t_zero;
-    const double den = exp((tp - tz) / (ti - tp) + (tp - tz) / (ti - tz)) + 1.;
-    accI = accI / den;
-    accII = accII / den;
+  __syncthreads();
+  const int p = s_p;
+  if (p >= 0 && tid < P.nmodes) {
+    // series length: the host bound for orders n+1, n+2 covers the higher orders
+    taylor_amplitudes(tid + 1, p, s_eref, s_sq, B.bessel_c, B.bessel_lgf, P.bterms[tid + 1], &s_coef[tid][0][0]);
   }
-  hI[i] = accI;
-  hII[i] = accII;
+  __syncthreads();
+  if (i >= P.N + 2) return;
+  if (i >= P.N) {
+    B.t[i] = 0.0;
+    hI[i] = 0.0;
+    hII[i] = 0.0;
+    return;
+  }
+  if (!active) {
+    B.t[i] = sample_time(P, i);
+    hI[i] = 0.0;
+    hII[i] = 0.0;
+    return;
+  }
+  SampleGeom G;
+  sample_geometry(P, B, i, G);
+  B.t[i] = G.ti;
+  double accI, accII;
+  if (p < 0) {
+    modesum_direct(P, B, G, &accI, &accII);
+  } else {
+    const double d = G.e - s_eref;
+    // exp(i n Phi) by rotation from Phi reduced mod 2 pi (Cody-Waite, three
+    // parts of 2 pi); the Doppler term is linear in n: Doppler_n = n dA + dB
+    const double kq = rint(G.Phi * (1.0 / (2.0 * kPi)));
+    double phr = fma(-kq, 6.283185307179586, G.Phi);
+    phr = fma(-kq, 2.4492935982947064e-16, phr);
+    phr = fma(-kq, -5.989539619436679e-33, phr);
+    const double gdpi = G.gimdot / kPi;
+    double dA = 0.0, dB = 0.0;
+    if (P.mich) {
+      const double proj = kAUsec * P.sinqS * G.cosorbphs;
+      dA = 2. * kPi * G.nu * proj;
+      dB = 2. * G.gimdot * proj;
+    }
+    double ur, ui, qr, qi;
+    sincos(phr + dA, &ui, &ur);
+    sincos(dB, &qi, &qr);
+    accI = 0.0;
+    accII = 0.0;
+    for (int nn = 1; nn <= P.nmodes; nn++) {
+      // q = exp(i (n (Phi + dA) + dB))
+      const double tr = qr * ur - qi * ui;
+      qi = qr * ui + qi * ur;
+      qr = tr;
+      // the reference's rounded argument s = rn(rn(n Phi) + Doppler) differs
+      // from n Phi + Doppler by -(e1 + e2), both exact: e1 = n Phi - rn(n Phi),
+      // e2 = (rn(n Phi) + Doppler) - s
+      const double dn = (double)nn;
+      const double pn = dn * G.Phi;
+      const double e1 = fma(dn, G.Phi, -pn);
+      double s_ = pn, e2 = 0.0;
+      if (P.mich) {
+        const double fn = dn * G.nu + gdpi;
+        const double Doppler = 2. * kPi * fn * kAUsec * P.sinqS * G.cosorbphs;
+        s_ = pn + Doppler;
+        const double bb = s_ - pn;
+        e2 = (pn - (s_ - bb)) + (Doppler - bb);
+      }
+      const double del = -(e1 + e2);
+      const double cnp = qr - del * qi, snp = qi + del * qr;
+      double abg[3];
+      taylor_eval(&s_coef[nn - 1][0][0], p, d, abg);
+      const double al = abg[0], be = abg[1], ga = abg[2];
+      accI += cnp * (al * G.W[0] + ga * G.W[2]) + snp * (be * G.W[1]);
+      accII += cnp * (al * G.W[3] + ga * G.W[5]) + snp * (be * G.W[4]);
+    }
+  }
+  const double den = taper(P, i, G.ti);
+  hI[i] = G.Amp * accI / den;
+  hII[i] = G.Amp * accII / den;
 }
 
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
   const int n = P.N + 2;
-  hipLaunchKernelGGL(k_modesum, dim3((n + 255) / 256), dim3(256), 0, s, P, B);
+  hipLaunchKernelGGL(k_modesum, dim3((n + kMsThreads - 1) / kMsThreads), dim3(kMsThreads), 0, s, P, B);
 }
 
 // ------------------------------------------------------------------ likelihood

@@ -35,6 +35,12 @@ int aak_test_knot_eval(const double *params, double T_fit, int length, double dt
  * to e_bound; returns the series length used. */
 int aak_test_bessel_five(int n, double e, double e_bound, double *J);
 
+/* Mode-sum amplitude factors alpha_n, beta_n, gamma_n at e = e_ref + d from
+ * their Taylor expansion about e_ref (order chosen for |d| <= del and
+ * harmonics up to n_max), out[0..2], and from the direct Bessel evaluation,
+ * out[3..5].  Returns the Taylor order. */
+int aak_test_taylor_amplitudes(int n_max, int n, double e_ref, double del, double d, double *out);
+
 /* One evaluation of the inspiral rates (dp/dt, de/dt, dcos(iota)/dt). */
 int aak_test_inspiral_rates(double p, double e, double cosi, double a, double *out);
 

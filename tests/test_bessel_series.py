@@ -45,3 +45,27 @@ def test_out_of_range_is_refused(L):
     J = np.zeros(5)
     assert L.lib().aak_test_bessel_five(3, 0.7, 0.7, L.dptr(J)) < 0
     assert L.lib().aak_test_bessel_five(129, 0.1, 0.2, L.dptr(J)) < 0
+
+
+@pytest.mark.parametrize("n_max", [6, 30, 50, 128])
+def test_block_taylor_amplitudes_match_direct_evaluation(L, n_max):
+    # the mode sum expands alpha_n, beta_n, gamma_n in e about each block's
+    # mid eccentricity; within the block's range the expansion must agree with
+    # the direct Bessel evaluation to rounding (the amplitudes scale with n)
+    out = np.zeros(6)
+    checked = 0
+    for e_ref in (0.0, 1e-3, 0.1, 0.3, 0.5, 0.6):
+        for dl in (0.0, 1e-6, 2e-5, 1e-4, 5e-4):
+            if e_ref + dl > 0.66:
+                continue
+            for n in sorted({1, 2, 3, 5, max(1, n_max // 2), n_max}):
+                for d in (-dl, -dl / 3, 0.0, dl / 2, dl):
+                    if e_ref + d < 0:
+                        continue
+                    p = L.lib().aak_test_taylor_amplitudes(n_max, n, e_ref, dl, d, L.dptr(out))
+                    if p < 0:
+                        continue
+                    scale = max(1.0, np.max(np.abs(out[3:])))
+                    assert np.max(np.abs(out[:3] - out[3:])) <= 4e-15 * scale, (n_max, n, e_ref, dl, d, p)
+                    checked += 1
+    assert checked > 100

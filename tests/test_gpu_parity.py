@@ -135,3 +135,18 @@ def test_likelihood_parity():
     t, sI, sII = g.GetWaveform()
     spec = sI[0::2] + 1j * sI[1::2]
     assert np.max(np.abs(spec - hI)) <= 1e-9 * np.max(np.abs(hI))
+
+
+def test_high_eccentricity_parity():
+    # e0 = 0.7 lies above the series range: the mode sum takes the Miller path
+    pars = [0.5, 0.6, 12.0, 0.7, 1e6, 10.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
+    out, g, ref = _compare(pars, 2 ** 16, 10.0, label="e0.7")
+    assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
+
+
+def test_mode_sum_forms_agree_across_a_plunge():
+    # fast inspiral close to plunge: blocks with a wide e range fall back to the
+    # per-sample Bessel evaluation, the others use the block Taylor form
+    pars = [0.5, 0.3, 8.0, 0.45, 3e5, 5.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
+    out, g, ref = _compare(pars, 2 ** 20, 15.0, n_max=40, label="fast-plunge")
+    assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
