@@ -163,8 +163,7 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
   for (int j = j0; j < j0 + steps - 1; j++) {
     edotm = edot;
     vdotm = vdot;
-    edot = PN::dedt(vA[j], eA[j], coslam, mu, M0, S0);
-    vdot = PN::dvdt(vA[j], eA[j], coslam, mu, M0, S0);
+    PN::ak_rates(vA[j], eA[j], coslam, mu, M0, S0, &edot, &vdot);
     if (j == j0) {
       edotm = edot;
       vdotm = vdot;
@@ -195,8 +194,7 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
   for (int j = j0 + steps - 1; j < j_end; j++) {
     edotm = edot;
     vdotm = vdot;
-    edot = PN::dedt(vA[j], eA[j], coslam, mu, M0, S0);
-    vdot = PN::dvdt(vA[j], eA[j], coslam, mu, M0, S0);
+    PN::ak_rates(vA[j], eA[j], coslam, mu, M0, S0, &edot, &vdot);
     eA[j + 1] = eA[j] + (1.5 * edot - .5 * edotm) * dt_large;
     vA[j + 1] = vA[j] + (1.5 * vdot - .5 * vdotm) * dt_large;
     clamp_step(j);
@@ -237,11 +235,12 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
   int bad = 0;
   for (int j = j0; j <= j_max_temp; j++) {
     if (j > j_max) break;
-    if (pn_p(vf[j], ef[j], coslam, Sf[j], Mf[j], M_phys) < LSO_min && j_max == j_max_temp) {
+    const double pj = pn_p(vf[j], ef[j], coslam, Sf[j], Mf[j], M_phys);
+    if (pj < LSO_min && j_max == j_max_temp) {
       j_min = std::max(j - j_RR, j0);
       j_max = std::min(j + 1, j_max_temp);
     }
-    if (j > j0 && (j - j0) % j_RR == 0 && pn_p(vf[j], ef[j], coslam, Sf[j], Mf[j], M_phys) < LSO_max && j_max == j_max_temp) {
+    if (j > j0 && (j - j0) % j_RR == 0 && pj < LSO_max && j_max == j_max_temp) {
       j_min = std::max(j - j_RR, j0);
       if (unstable(j, &bad)) j_max = std::min(j + 1, j_max_temp);
       if (bad) break;
@@ -264,8 +263,7 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
     for (int j = j0; j > j_start; j--) {
       edotp = edot;
       vdotp = vdot;
-      edot = PN::dedt(vA[j], eA[j], coslam, mu, M0, S0);
-      vdot = PN::dvdt(vA[j], eA[j], coslam, mu, M0, S0);
+      PN::ak_rates(vA[j], eA[j], coslam, mu, M0, S0, &edot, &vdot);
       if (j == j0) {
         edotp = edot;
         vdotp = vdot;
@@ -311,8 +309,8 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
 int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs) {
   if (n < 3) return -1;
   const int sys = n - 2;
-  thread_local std::vector<double> od, dg, gi_, al, gam, z;
-  for (std::vector<double> *a : {&od, &dg, &gi_, &al, &gam, &z})
+  thread_local std::vector<double> od, dg, gi_, al, gam;
+  for (std::vector<double> *a : {&od, &dg, &gi_, &al, &gam})
     if ((int)a->size() < sys + 1) a->resize(sys + 1);
   for (int i = 0; i < sys; i++) {
     const double hi = x[i + 1] - x[i], hip1 = x[i + 2] - x[i + 1];
@@ -330,22 +328,52 @@ int spline_natural4(const double *x, const double *const *ys, int n, double *con
     }
     al[sys - 1] = dg[sys - 1] - od[sys - 2] * gam[sys - 2];
   }
-  for (int a = 0; a < 4; a++) {
+  // right-hand side g_i = 3 (dy_{i+1}/h_{i+1} - dy_i/h_i) of array a
+  auto g = [&](int a, int i) {
     const double *y = ys[a];
-    double *c = cs[a];
-    c[0] = 0.0;
-    c[n - 1] = 0.0;
-    // right-hand side g_i = 3 (dy_{i+1}/h_{i+1} - dy_i/h_i)
-    auto g = [&](int i) { return 3.0 * ((y[i + 2] - y[i + 1]) * gi_[i + 1] - (y[i + 1] - y[i]) * gi_[i]); };
-    if (sys == 1) {
-      c[1] = g(0) / dg[0];
-      continue;
-    }
-    z[0] = g(0);
-    for (int i = 1; i < sys; i++) z[i] = g(i) - gam[i - 1] * z[i - 1];
-    for (int i = 0; i < sys; i++) z[i] = z[i] / al[i];
-    c[sys] = z[sys - 1];
-    for (int i = sys - 2; i >= 0; i--) c[1 + i] = z[i] - gam[i] * c[i + 2];
+    return 3.0 * ((y[i + 2] - y[i + 1]) * gi_[i + 1] - (y[i + 1] - y[i]) * gi_[i]);
+  };
+  for (int a = 0; a < 4; a++) {
+    cs[a][0] = 0.0;
+    cs[a][n - 1] = 0.0;
+  }
+  if (sys == 1) {
+    for (int a = 0; a < 4; a++) cs[a][1] = g(a, 0) / dg[0];
+    return 0;
+  }
+  // the four arrays' recurrences run side by side (independent chains); each
+  // array's operations and their order are those of the single-array solve.
+  // z of array a is kept in c[a][1..sys] until the back substitution.
+  double z0 = g(0, 0), z1 = g(1, 0), z2 = g(2, 0), z3 = g(3, 0);
+  cs[0][1] = z0;
+  cs[1][1] = z1;
+  cs[2][1] = z2;
+  cs[3][1] = z3;
+  for (int i = 1; i < sys; i++) {
+    const double gm = gam[i - 1];
+    z0 = g(0, i) - gm * z0;
+    z1 = g(1, i) - gm * z1;
+    z2 = g(2, i) - gm * z2;
+    z3 = g(3, i) - gm * z3;
+    cs[0][1 + i] = z0;
+    cs[1][1 + i] = z1;
+    cs[2][1 + i] = z2;
+    cs[3][1 + i] = z3;
+  }
+  for (int i = 0; i < sys; i++) {
+    const double a_ = al[i];
+    cs[0][1 + i] = cs[0][1 + i] / a_;
+    cs[1][1 + i] = cs[1][1 + i] / a_;
+    cs[2][1 + i] = cs[2][1 + i] / a_;
+    cs[3][1 + i] = cs[3][1 + i] / a_;
+  }
+  // c[sys] = z[sys-1] already in place; c[1+i] = z[i] - gam[i] c[i+2]
+  for (int i = sys - 2; i >= 0; i--) {
+    const double gm = gam[i];
+    cs[0][1 + i] = cs[0][1 + i] - gm * cs[0][i + 2];
+    cs[1][1 + i] = cs[1][1 + i] - gm * cs[1][i + 2];
+    cs[2][1 + i] = cs[2][1 + i] - gm * cs[2][i + 2];
+    cs[3][1 + i] = cs[3][1 + i] - gm * cs[3][i + 2];
   }
   return 0;
 }

@@ -235,10 +235,10 @@ struct PN {
     return dphidm(v, e, -cosi, -s) / dtdm(v, e, -cosi, -s) / M;
   }
   // Radiation-reaction rates of v and e (KSParMap.cc:86-144).
-  AAK_HD static double dvdt(double v, double e, double Y, double m, double M, double q) {
+  AAK_HD static double dvdt_p(double v, double e, double Y, double m, double M, double q, double pw, double lv) {
     const double v2 = v * v, v3 = v2 * v, v6 = v3 * v3, e2 = e * e, e4 = e2 * e2, e6 = e4 * e2;
     const double Y2 = Y * Y, q2 = q * q, EG = kEulerGamma, P2 = kPi2Trunc, PI = kPi;
-    return (32. * pow(1. - e2, 1.5) * m * v6 * v3 *
+    return (32. * pw * m * v6 * v3 *
             (1. + (7. * e2) / 8. + ((-5944. - 7040. * e2 + 8539. * e4) * v2) / 2688. -
              (v3 * ((-18432. - 55872. * e2 - 7056. * e4 + 49. * e6) * PI + 48. * (1064. + 1516. * e2 + 475. * e4) * q * Y)) / 4608. +
              (v3 * v2 * ((-4791168. - 28113984. * e2 + 12239226. * e4 + 4005097. * e6) * PI -
@@ -256,13 +256,13 @@ struct PN {
                    e6 * (202.0979603432282 - (2461. * EG) / 560. + (23. * P2) / 16. - (3571. * q2) / 3584. -
                          (108577. * PI * q * Y) / 13824. + (41071. * q2 * Y2) / 1536. + (94138279. * log(2.)) / 2160. -
                          (42667641. * log(3.)) / 3584. - (1044921875. * log(5.)) / 96768.) -
-                   (107. * (256. + 1832. * e2 + 1308. * e4 + 69. * e6) * log(v)) / 1680.))) /
+                   (107. * (256. + 1832. * e2 + 1308. * e4 + 69. * e6) * lv) / 1680.))) /
            (5. * M * M);
   }
-  AAK_HD static double dedt(double v, double e, double Y, double m, double M, double q) {
+  AAK_HD static double dedt_p(double v, double e, double Y, double m, double M, double q, double pw, double lv) {
     const double v2 = v * v, v3 = v2 * v, v6 = v3 * v3, e2 = e * e, e4 = e2 * e2;
     const double Y2 = Y * Y, q2 = q * q, EG = kEulerGamma, P2 = kPi2Trunc, PI = kPi;
-    return -(e * pow(1. - e2, 1.5) * m * v6 * v2 *
+    return -(e * pw * m * v6 * v2 *
              (2451456. + 975744. * e2 + 144. * (-54792. - 18600. * e2 + 22579. * e4) * v2 +
               84. * v3 * ((189120. + 286512. * e2 + 24217. * e4) * PI - 48. * (7032. + 5592. * e2 + 1313. * e4) * q * Y) +
               3. * v3 * v2 * ((-16885824. - 42474444. * e2 + 5678971. * e4) * PI - 48. * (106568. + 187862. * e2 + 442811. * e4) * q * Y) +
@@ -278,8 +278,21 @@ struct PN {
                    e4 * (725.9069951397873 - (1147147. * EG) / 15960. + (10721. * P2) / 456. + (56509. * q2) / 9728. -
                          (1739605. * PI * q * Y) / 29184. + (3248951. * q2 * Y2) / 68096. + (760314287. * log(2.)) / 47880. -
                          (1022385321. * log(3.)) / 340480. - (1044921875. * log(5.)) / 204228.) -
-                   (107. * (6152. + 22256. * e2 + 10721. * e4) * log(v)) / 15960.))) /
+                   (107. * (6152. + 22256. * e2 + 10721. * e4) * lv) / 15960.))) /
            (120960. * M * M);
+  }
+  // dv/dt and de/dt (KSParMap.cc:86-144); the _p forms take pow(1-e^2, 1.5)
+  // and log(v) precomputed, so both rates of one step share them
+  AAK_HD static double dvdt(double v, double e, double Y, double m, double M, double q) {
+    return dvdt_p(v, e, Y, m, M, q, pow(1. - e * e, 1.5), log(v));
+  }
+  AAK_HD static double dedt(double v, double e, double Y, double m, double M, double q) {
+    return dedt_p(v, e, Y, m, M, q, pow(1. - e * e, 1.5), log(v));
+  }
+  AAK_HD static void ak_rates(double v, double e, double Y, double m, double M, double q, double *edot, double *vdot) {
+    const double pw = pow(1. - e * e, 1.5), lv = log(v);
+    *edot = dedt_p(v, e, Y, m, M, q, pw, lv);
+    *vdot = dvdt_p(v, e, Y, m, M, q, pw, lv);
   }
 };
 
