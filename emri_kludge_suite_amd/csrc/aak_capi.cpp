@@ -485,6 +485,13 @@ int aak_set_nmax(aak_handle h, int n_max) {
   return 0;
 }
 
+int aak_set_precision(aak_handle h, int mode) {
+  if (!h) return fail(-1, "null handle");
+  if (mode != 0 && mode != 1) return fail(-2, "precision mode must be 0 (FP64) or 1 (mixed)");
+  h->cfg.mixed = mode;
+  return 0;
+}
+
 int aak_set_host_threads(aak_handle h, int n) {
   if (!h) return fail(-1, "null handle");
   if (n < 0 || n > 256) return fail(-2, "host threads must be in [0, 256]");

@@ -93,6 +93,7 @@ struct DenseParams {
   double zeta, Mphys;             // amplitude scale and M in solar masses
   double t_plunge, t_zero;
   KnotGrid grid;
+  int mixed;                      // 1: FP32 harmonic loop (mixed precision)
   int bessel_series;              // 1: series + recurrence, 0: Miller from above
   unsigned char bterms[kMaxModes + 1];  // series length per harmonic ([0]: orders 0..4)
 };

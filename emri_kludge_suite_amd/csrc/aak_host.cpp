@@ -535,6 +535,7 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   for (int k = 0; k < ks.K; k++) e_hi = std::max(e_hi, ks.ky(0)[k]);
   const double e_bound = e_hi * 1.01 + 0.01;
   P.bessel_series = (nmodes <= kMaxModes && e_bound <= kSeriesMaxE) ? 1 : 0;
+  P.mixed = cfg.mixed ? 1 : 0;
   if (P.bessel_series) bessel_series_terms(e_bound, nmodes, P.bterms);
   return 0;
 }

@@ -19,6 +19,7 @@ struct WaveConfig {
   int LISA = 1;
   int backint = 0;
   int n_max = 0;  // > 0 overrides the reference harmonic rule
+  int mixed = 0;  // 1: mixed-precision mode sum
 };
 
 // 14 source parameters in the reference order (AAK_manager.hh:134-148).

@@ -939,8 +939,14 @@ __device__ inline void modesum_direct(const DenseParams &P, const DenseBuffers &
 constexpr int kMsThreads = 256;
 constexpr int kTaylorModes = 128;
 
+// MIXED: the harmonic loop (rotation recurrence, amplitude polynomials,
+// response accumulation) runs in FP32 from FP64-reduced phase factors and
+// FP64 block coefficients; phases, their reduction mod 2 pi, the block
+// expansions and the final scaling stay FP64.
+template <bool MIXED>
 __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuffers B) {
   __shared__ double s_coef[kTaylorModes][3][kTaylorMax + 1];
+  __shared__ float s_coef_f[MIXED ? kTaylorModes : 1][3][kTaylorMax + 1];
   __shared__ double s_sq[kTaylorMax + 1];
   __shared__ double s_min[kMsThreads / 64], s_max[kMsThreads / 64];
   __shared__ double s_eref;
@@ -984,6 +990,10 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
   if (p >= 0 && tid < P.nmodes) {
     // series length: the host bound for orders n+1, n+2 covers the higher orders
     taylor_amplitudes(tid + 1, p, s_eref, s_sq, B.bessel_c, B.bessel_lgf, P.bterms[tid + 1], &s_coef[tid][0][0]);
+    if (MIXED) {
+      for (int f = 0; f < 3; f++)
+        for (int j = 0; j <= kTaylorMax; j++) s_coef_f[tid][f][j] = (float)s_coef[tid][f][j];
+    }
   }
   __syncthreads();
   if (i >= P.N + 2) return;
@@ -1025,6 +1035,29 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
     sincos(dB, &qi, &qr);
     accI = 0.0;
     accII = 0.0;
+    if (MIXED) {
+      float fur = (float)ur, fui = (float)ui, fqr = (float)qr, fqi = (float)qi;
+      const float fd = (float)d;
+      const float W0 = (float)G.W[0], W1 = (float)G.W[1], W2 = (float)G.W[2];
+      const float W3 = (float)G.W[3], W4 = (float)G.W[4], W5 = (float)G.W[5];
+      float fI = 0.f, fII = 0.f;
+      for (int nn = 1; nn <= P.nmodes; nn++) {
+        const float tr = fqr * fur - fqi * fui;
+        fqi = fqr * fui + fqi * fur;
+        fqr = tr;
+        const float *cf = &s_coef_f[nn - 1][0][0];
+        float al = cf[p], be = cf[kTaylorMax + 1 + p], ga = cf[2 * (kTaylorMax + 1) + p];
+        for (int j = p - 1; j >= 0; j--) {
+          al = fmaf(al, fd, cf[j]);
+          be = fmaf(be, fd, cf[kTaylorMax + 1 + j]);
+          ga = fmaf(ga, fd, cf[2 * (kTaylorMax + 1) + j]);
+        }
+        fI += fqr * (al * W0 + ga * W2) + fqi * (be * W1);
+        fII += fqr * (al * W3 + ga * W5) + fqi * (be * W4);
+      }
+      accI = (double)fI;
+      accII = (double)fII;
+    } else {
     for (int nn = 1; nn <= P.nmodes; nn++) {
       // q = exp(i (n (Phi + dA) + dB))
       const double tr = qr * ur - qi * ui;
@@ -1052,6 +1085,7 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
       accI += cnp * (al * G.W[0] + ga * G.W[2]) + snp * (be * G.W[1]);
       accII += cnp * (al * G.W[3] + ga * G.W[5]) + snp * (be * G.W[4]);
     }
+    }
   }
   const double den = taper(P, i, G.ti);
   hI[i] = G.Amp * accI / den;
@@ -1060,7 +1094,10 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
 
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
   const int n = P.N + 2;
-  hipLaunchKernelGGL(k_modesum, dim3((n + kMsThreads - 1) / kMsThreads), dim3(kMsThreads), 0, s, P, B);
+  if (P.mixed)
+    hipLaunchKernelGGL(k_modesum<true>, dim3((n + kMsThreads - 1) / kMsThreads), dim3(kMsThreads), 0, s, P, B);
+  else
+    hipLaunchKernelGGL(k_modesum<false>, dim3((n + kMsThreads - 1) / kMsThreads), dim3(kMsThreads), 0, s, P, B);
 }
 
 // ------------------------------------------------------------------ likelihood

@@ -105,6 +105,12 @@ class GPUAAK:
                    "LikelihoodBatch")
         return (out, st) if return_status else out
 
+    def set_precision(self, mode):
+        """Mode-sum arithmetic: "fp64" (default, parity with the reference) or
+        "mixed" (FP32 harmonic loop, mismatch below 1e-6)."""
+        m = {"fp64": 0, "mixed": 1}.get(mode, mode)
+        _lib.check(_lib.lib().aak_set_precision(self._h, int(m)), "set_precision")
+
     def set_host_threads(self, n):
         """Threads of the batch host pool (0: AAK_HOST_THREADS or the core count, at most 16)."""
         _lib.check(_lib.lib().aak_set_host_threads(self._h, int(n)), "set_host_threads")

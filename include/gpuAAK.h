@@ -67,6 +67,15 @@ int aak_get_waveform(aak_handle h, double *t, double *hI, double *hII);
  * GetWaveform). */
 int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like_out, int *status);
 
+/* Arithmetic of the mode sum; no counterpart in the reference.  mode 0
+ * (default): plain FP64, within relative L2 1e-10 of the reference CPU AAK.
+ * mode 1: mixed precision -- phases, their reduction mod 2 pi, the Bessel
+ * expansions and the likelihood sums stay FP64, the per-harmonic rotation,
+ * amplitude and response arithmetic runs in FP32 (noise-weighted mismatch
+ * against mode 0 below 1e-6).  Blocks outside the Taylor form (fast
+ * evolution, e > 0.66) keep the FP64 per-harmonic evaluation. */
+int aak_set_precision(aak_handle h, int mode);
+
 /* Host threads of the batch pool (0 = AAK_HOST_THREADS or the hardware
  * concurrency, at most 16); no counterpart in the reference. */
 int aak_set_host_threads(aak_handle h, int n);
