@@ -1,6 +1,6 @@
 """Benchmark: AAK likelihoods per second on one MI355X.
 
-Workload (production-size plain-FP64 case): 1-yr-class LISA waveforms,
+Workload (production size): 1-yr-class LISA waveforms,
 N = 2^21 samples at dt = 15 s (0.997 yr), harmonics n = 1..50, parameters drawn
 from the production priors (log10 M ~ U[5,7], mu ~ U[5,50], a ~ U[0,0.95],
 e0 ~ U[1e-3,0.5], p0 ~ U[7,15], iota0 = 0.5, angles uniform on the sphere),
@@ -10,8 +10,13 @@ analytic LISA instrument + confusion PSD (1 yr) on the rFFT grid.
 One step = one batch of templates (default 32) through the whole path: host
 inspiral + knots (thread pool), device spline/phasing/exact folds/mode sum,
 rFFT and the fused weighted reduction, pipelined across templates.  The data
-(weighted injection) is resident in HBM before timing starts.  The same
-templates are also timed one waveform per call (the reference's pattern).
+(weighted injection) is resident in HBM before timing starts.
+
+The headline value is the mixed-precision form (FP32 harmonic loop on FP64
+phases, expansions and sums), accepted only if its noise-weighted mismatch
+against the FP64 waveforms of the same templates is <= 1e-6.  The plain FP64
+batched path and the one-waveform-per-call FP64 path (the reference's calling
+pattern) are timed on the same templates and reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu-baseline]
 """
@@ -32,6 +37,7 @@ DT = 15.0
 N_MAX = 50
 MSUN_SEC = 6.67259e-11 * 1.98892e30 / 299792458.0 ** 3
 FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
 
 
 def draw_params(rng):
@@ -131,6 +137,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed batches")
     ap.add_argument("--batch", type=int, default=32, help="templates per step")
     ap.add_argument("--single", type=int, default=8, help="one-waveform-per-call likelihoods timed beside")
+    ap.add_argument("--mismatch", type=int, default=4, help="templates whose mixed/FP64 mismatch is measured")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-probe", action="store_true",
@@ -161,25 +168,31 @@ def main():
     P = np.array(templates[:args.batch])
     flops_batch = modesum_flops(meta[:args.batch])
 
-    # batched path: host stages on the thread pool, device work pipelined
-    for _ in range(args.warmup):
-        g.LikelihoodBatch(P)
-    g.set_profiling(True)
-    stage = np.zeros(5)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        like = g.LikelihoodBatch(P)
-        tm = g.batch_timings()
-        stage += np.array([tm[x] for x in ("rates", "fold", "modesum", "rfft", "reduce")])
-    elapsed = time.perf_counter() - t0
-    assert np.all(np.isfinite(like)), "bench: a template failed"
-    n_like = args.steps * len(P)
-    value = n_like / elapsed
-    launches = args.steps * len(P)
-    modesum_ms = stage[2] / launches
-    achieved = flops_batch * args.steps / (stage[2] * 1e-3) / 1e12
+    def run_batched(mode):
+        g.set_precision(mode)
+        for _ in range(args.warmup):
+            g.LikelihoodBatch(P)
+        g.set_profiling(True)
+        stage = np.zeros(5)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            like = g.LikelihoodBatch(P)
+            tm = g.batch_timings()
+            stage += np.array([tm[x] for x in ("rates", "fold", "modesum", "rfft", "reduce")])
+        elapsed = time.perf_counter() - t0
+        g.set_profiling(False)
+        assert np.all(np.isfinite(like)), "bench: a template failed"
+        return elapsed, stage, like
 
-    # one waveform per call (the reference's calling pattern), same templates
+    # batched paths: host stages on the thread pool, device work pipelined
+    el_mx, st_mx, like_mx = run_batched("mixed")
+    el_64, st_64, like_64 = run_batched("fp64")
+    launches = args.steps * len(P)
+    value = launches / el_mx
+    value_64 = launches / el_64
+
+    # one waveform per call in FP64 (the reference's calling pattern)
+    g.set_precision("fp64")
     g.set_profiling(True)
     single_stage = np.zeros(6)
     t1 = time.perf_counter()
@@ -189,8 +202,32 @@ def main():
         single_stage += np.array([tm[x] for x in ("host", "rates", "fold", "modesum", "rfft", "reduce")])
     single_elapsed = time.perf_counter() - t1
     g.set_profiling(False)
+    value_single = args.single / single_elapsed
 
+    # accuracy of the mixed form: noise-weighted mismatch against FP64 waveforms
+    mism = []
+    for k in range(min(args.mismatch, len(templates))):
+        wf = {}
+        for mode in ("fp64", "mixed"):
+            g.set_precision(mode)
+            g.gpu_gen_AAK(*templates[k])
+            _, a, b = g.GetWaveform()
+            wf[mode] = (np.fft.rfft(a[:N_SAMPLES]) * w, np.fft.rfft(b[:N_SAMPLES]) * w)
+        ab = sum(np.real(np.vdot(x, y)) for x, y in zip(wf["fp64"], wf["mixed"]))
+        aa = sum(np.real(np.vdot(x, x)) for x in wf["fp64"])
+        bb = sum(np.real(np.vdot(y, y)) for y in wf["mixed"])
+        mism.append(1.0 - ab / np.sqrt(aa * bb))
+    like_rel = float(np.max(np.abs(like_mx[:, 1] - like_64[:, 1]) / np.abs(like_64[:, 1])))
+    assert max(mism) <= 1e-6, "bench: mixed-precision mismatch above 1e-6"
+
+    def roof(stage):
+        """Mean k_modesum time per launch and the convention's TFLOP/s."""
+        return stage[2] / launches, flops_batch * args.steps / (stage[2] * 1e-3) / 1e12
+
+    ms_mx, ach_mx = roof(st_mx)
+    ms_64, ach_64 = roof(st_64)
     traffic = read_traffic()
+    stages = ("rates", "fold", "modesum", "rfft", "reduce")
     result = {
         "metric": METRIC,
         "value": value,
@@ -198,25 +235,31 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": 1e3 * el_mx / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "mixed: f64 phases/expansions/sums, f32 harmonic loop",
         "data": "synthetic (seeded prior draws, injection seed 2, templates seed 3)",
-        "config": {"workload": "1-yr LISA AAK likelihoods, N=2^21, dt=15 s, n=1..50, plain FP64, "
-                               "%d templates per step (batched)" % len(P),
+        "config": {"workload": "1-yr LISA AAK likelihoods, N=2^21, dt=15 s, n=1..50, %d templates per step "
+                               "(batched), mixed precision" % len(P),
                    "N": N_SAMPLES, "dt": DT, "n_max": N_MAX, "batch": len(P), "psd": "LISA_Noise(dur=1)"},
-        "stages_ms_per_template": dict(zip(("rates", "fold", "modesum", "rfft", "reduce"),
-                                           [round(x / launches, 4) for x in stage])),
-        "single_call": {"value": args.single / single_elapsed, "unit": "likelihoods/s",
-                        "ms_per_likelihood": 1e3 * single_elapsed / args.single,
-                        "stages_ms": dict(zip(("host", "rates", "fold", "modesum", "rfft", "reduce"),
-                                              [round(x / args.single, 4) for x in single_stage]))},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "k_modesum", "kernel_ms": modesum_ms,
-                     "convention": "N_active*(190*n_modes+460) FLOP per launch; peak = FP64 vector/matrix"},
+        "accuracy": {"max_mismatch_vs_fp64": max(mism), "templates_checked": len(mism),
+                     "max_rel_diff_hh_vs_fp64": like_rel},
+        "stages_ms_per_template": dict(zip(stages, [round(x / launches, 4) for x in st_mx])),
+        "plain_fp64_batched": {"value": value_64, "unit": "likelihoods/s", "ms_per_step": 1e3 * el_64 / args.steps,
+                               "stages_ms_per_template": dict(zip(stages, [round(x / launches, 4) for x in st_64])),
+                               "k_modesum_tflops": ach_64, "k_modesum_ms": ms_64},
+        "single_call_fp64": {"value": value_single, "unit": "likelihoods/s",
+                             "ms_per_likelihood": 1e3 * single_elapsed / args.single,
+                             "stages_ms": dict(zip(("host",) + stages,
+                                                   [round(x / args.single, 4) for x in single_stage]))},
+        "speedup": {"mixed_batched_over_single_fp64": value / value_single,
+                    "fp64_batched_over_single_fp64": value_64 / value_single},
+        "roofline": {"bound": "mfma", "achieved": ach_mx, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach_mx / FP32_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "k_modesum (mixed)", "kernel_ms": ms_mx,
+                     "convention": "N_active*(190*n_modes+460) FLOP per launch; peak = published FP32 (VALU-bound kernel)"},
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(templates, w, (dI, dII), args.cpu_seconds)
