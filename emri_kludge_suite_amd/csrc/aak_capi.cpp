@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -161,6 +162,7 @@ struct aak_handle_s {
   bool stop = false;
   const double *b_params = nullptr;
   long long b_n = 0, b_next = 0;
+  std::vector<long long> b_order;  // templates in decreasing estimated host cost
 
   void mark(int k) {
     if (profiling) (void)hipEventRecord(evs[k], stream);
@@ -386,7 +388,7 @@ static void worker_main(aak_handle h) {
         return h->stop || (h->b_params && h->b_next < h->b_n && !h->free_slots.empty());
       });
       if (h->stop) return;
-      i = h->b_next++;
+      i = h->b_order[(size_t)h->b_next++];
       params = h->b_params;
       sidx = h->free_slots.back();
       h->free_slots.pop_back();
@@ -589,8 +591,24 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
     h->batch_ev.resize((size_t)6 * n, nullptr);
     for (size_t k = old; k < h->batch_ev.size(); k++) HIPCHK(hipEventCreate(&h->batch_ev[k]));
   }
+  // Longest first: the host stage costs a few ms plus O(K) with K, the knot
+  // count, ~ 99 max(1, t_rr / T_fit) where t_rr = length dt mu / (M^2 M_sun)
+  // is the observation time in radiation-reaction units (AAKpy.cc:24-26).
+  std::vector<std::pair<double, long long>> cost((size_t)n);
+  for (long long i = 0; i < n; i++) {
+    const double M = params[14 * i + 4], mu = params[14 * i + 5];
+    const double tr = h->cfg.length * h->cfg.dt / kMsunSec / M / M * mu;
+    const double c = (tr > h->cfg.T_fit) ? tr / h->cfg.T_fit : 1.0;
+    cost[(size_t)i] = {std::isfinite(c) ? c : 0.0, i};
+  }
+  std::stable_sort(cost.begin(), cost.end(),
+                   [](const std::pair<double, long long> &a, const std::pair<double, long long> &b) {
+                     return a.first > b.first;
+                   });
   {
     std::lock_guard<std::mutex> lk(h->mu);
+    h->b_order.resize((size_t)n);
+    for (long long i = 0; i < n; i++) h->b_order[(size_t)i] = cost[(size_t)i].second;
     h->free_slots.clear();
     h->ready_slots.clear();
     for (int s = (int)h->ring.size() - 1; s >= 0; s--) h->free_slots.push_back(s);

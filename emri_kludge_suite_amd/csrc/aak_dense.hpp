@@ -111,9 +111,10 @@ struct FoldItem {
   int sgn;   // +1 / -1
   int first; // first step (block-local)
   int tie_first;
-  int pad0;
+  int blk;            // block of the item (set when the chain gathers the lists)
   long long R;        // run total of rounded increments (ties rounded down)
   long long m_start;  // integer representation of the exact run start (set by the chain)
+  double d;           // increment of a single exact step (kind 1)
 };
 
 struct FoldTie {

@@ -475,6 +475,8 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
   FoldTie *ties = W.ties + ((size_t)p * A.nblk + b) * kFoldMaxTies;
   long long P = carry;
   int item = item_base - 1, tcount = tie_base;
+  int c_item = -1, c_k = 0, c_tf = 0;  // WRITE: the item of the current run
+  long long c_m = 0;
 #pragma unroll
   for (int j = 0; j < kFoldPerThread; j++) {
     if (key[j] == -2) break;
@@ -488,6 +490,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
         items[item].sgn = (key[j] >= 0) ? ((key[j] & 1) ? 1 : -1) : 0;
         items[item].first = sl;
         items[item].tie_first = tcount;
+        items[item].d = d[j];
       }
     }
     P += r[j];
@@ -505,61 +508,59 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
       // the run total is the inclusive prefix at the run's last step
       if (last_of_run) items[item].R = P;
     } else if (key[j] >= 0) {
-      const FoldItem it = items[item];
+      if (item != c_item) {
+        c_item = item;
+        c_m = items[item].m_start;
+        c_k = items[item].k;
+        c_tf = items[item].tie_first;
+      }
       long long ups = 0;
-      for (int q = it.tie_first; q < tcount; q++) ups += ties[q].up;
-      const long long m = it.m_start + P + ups;
-      A.out[p][out_index(A, base + j)] = ldexp((double)m, it.k - 53);
+      for (int q = c_tf; q < tcount; q++) ups += ties[q].up;
+      const long long m = c_m + P + ups;
+      A.out[p][out_index(A, base + j)] = ldexp((double)m, c_k - 53);
     }
   }
 }
 
-// Serial replay of the item lists: exact start of every run, resolution of
-// the exact ties, FP64 replay of the steps near binade boundaries.  The
-// block's threads first gather the per-block item and tie lists into LDS
-// (offsets by a block scan), so the one serial lane walks LDS instead of
-// issuing dependent global loads; the results go back to global memory for
-// the write pass.  Inputs too large for LDS take the same walk over global
-// memory.
+// Exact start of every run, resolution of the exact ties, FP64 replay of the
+// single steps near binade boundaries.  The item and tie lists of all blocks
+// are gathered into LDS; consecutive runs on one binade grid without ties
+// form a segment whose total is a segmented int64 scan, so the one serial
+// lane only visits segment heads (binade changes, ties, single steps); the
+// run starts inside a segment are filled in parallel and re-checked against
+// their binade.  Lists too long for LDS take a serial walk over global
+// memory with the same arithmetic.
 constexpr int kChainThreads = 256;
-constexpr int kChainMaxBlocks = 1024;
-constexpr int kChainMaxItems = 768;
+constexpr int kChainMaxBlocks = 2048;
+constexpr int kChainMaxItems = 1152;
 constexpr int kChainMaxTies = 512;
 
-// Walks items [0, ni) of block b; returns false on a failed binade check.
-template <class ItemPtr, class TiePtr>
-__device__ inline bool chain_block(const FoldArgs &A, int p, int b, ItemPtr items, int ni, TiePtr ties, int nt,
-                                   double &x) {
-  const int blen = min(kFoldBlock, A.nsteps - b * kFoldBlock);
-  for (int it = 0; it < ni; it++) {
-    const FoldItem I = items[it];
-    const int last = ((it + 1 < ni) ? items[it + 1].first : blen) - 1;
-    if (I.kind == 1) {
-      for (int sl = I.first; sl <= last; sl++) {
-        const int st = b * kFoldBlock + sl;
-        x = x + fold_incr(A.rate[p], A.i0, A.dir, st, A.dt);
-        A.out[p][out_index(A, st)] = x;
-      }
-      continue;
-    }
-    int kx;
-    frexp(x, &kx);
-    if (kx != I.k || (x > 0.0 ? 1 : -1) != I.sgn) return false;
-    const long long m = (long long)ldexp(x, 53 - I.k);
-    items[it].m_start = m;
-    const int tend = (it + 1 < ni) ? items[it + 1].tie_first : nt;
-    long long ups = 0;
-    for (int q = I.tie_first; q < tend; q++) {
-      const long long val = m + ties[q].P + ups;
-      const int up = (int)(val & 1LL);
-      ties[q].up = up;
-      ups += up;
-    }
-    const long long mend = m + I.R + ups;
-    const long long am = mend < 0 ? -mend : mend;
-    if (am < (1LL << 52) || am >= (1LL << 53)) return false;
-    x = ldexp((double)mend, I.k - 53);
+// One item (run or single step) in walk order; returns false on a failed
+// binade check.  ties: the item's ties, nt of them.
+template <class ItemRef, class TiePtr>
+__device__ inline bool chain_item(const FoldArgs &A, int p, int b, ItemRef I, TiePtr ties, int nt, double &x,
+                                  long long extra_R) {
+  if (I.kind == 1) {
+    x = x + I.d;
+    A.out[p][out_index(A, b * kFoldBlock + I.first)] = x;
+    return true;
   }
+  int kx;
+  frexp(x, &kx);
+  if (kx != I.k || (x > 0.0 ? 1 : -1) != I.sgn) return false;
+  const long long m = (long long)ldexp(x, 53 - I.k);
+  I.m_start = m;
+  long long ups = 0;
+  for (int q = 0; q < nt; q++) {
+    const long long val = m + ties[q].P + ups;
+    const int up = (int)(val & 1LL);
+    ties[q].up = up;
+    ups += up;
+  }
+  const long long mend = m + I.R + extra_R + ups;
+  const long long am = mend < 0 ? -mend : mend;
+  if (am < (1LL << 52) || am >= (1LL << 53)) return false;
+  x = ldexp((double)mend, I.k - 53);
   return true;
 }
 
@@ -592,9 +593,13 @@ __device__ inline int block_exclusive_scan(int *v, int n, int *s_tmp) {
 __global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWork W) {
   __shared__ int s_io[kChainMaxBlocks + 1], s_to[kChainMaxBlocks + 1];
   __shared__ int s_tmp[kChainThreads];
+  __shared__ long long s_ltmp[kChainThreads];
+  __shared__ int s_ftmp[kChainThreads];
   __shared__ FoldItem s_items[kChainMaxItems];
   __shared__ FoldTie s_ties[kChainMaxTies];
-  __shared__ int s_ok;
+  __shared__ long long s_pref[kChainMaxItems];  // exclusive R prefix inside the segment
+  __shared__ int s_head[kChainMaxItems + 1];     // head flags, then the compacted head list
+  __shared__ int s_nheads, s_ok;
   const int p = blockIdx.x;
   const int tid = threadIdx.x;
   if (W.status[p]) return;
@@ -615,15 +620,19 @@ __global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWo
     fits = TI <= kChainMaxItems && TT <= kChainMaxTies;
   }
   if (!fits) {
-    // lists too long for LDS: the same walk over global memory
     if (tid != 0) return;
     double x = A.x0[p];
     A.out[p][A.i0] = x;
     for (int b = 0; b < nblk; b++) {
-      if (!chain_block(A, p, b, gitems + (size_t)b * kFoldMaxItems, gni[b],
-                       gties + (size_t)b * kFoldMaxTies, gnt[b], x)) {
-        W.status[p] = 1;
-        return;
+      FoldItem *it = gitems + (size_t)b * kFoldMaxItems;
+      FoldTie *tb = gties + (size_t)b * kFoldMaxTies;
+      const int ni = gni[b], nt = gnt[b];
+      for (int q = 0; q < ni; q++) {
+        const int t0 = it[q].tie_first, t1 = (q + 1 < ni) ? it[q + 1].tie_first : nt;
+        if (!chain_item<FoldItem &>(A, p, b, it[q], tb + t0, t1 - t0, x, 0)) {
+          W.status[p] = 1;
+          return;
+        }
       }
     }
     return;
@@ -633,26 +642,137 @@ __global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWo
     s_to[nblk] = TT;
   }
   __syncthreads();
+  // gather: items get their block and a block-global tie offset
   for (int b = tid; b < nblk; b += kChainThreads) {
-    const int io = s_io[b], ni = s_io[b + 1] - io;
-    for (int j = 0; j < ni; j++) s_items[io + j] = gitems[(size_t)b * kFoldMaxItems + j];
-    const int to = s_to[b], nt = s_to[b + 1] - to;
+    const int io = s_io[b], ni = s_io[b + 1] - io, to = s_to[b];
+    for (int j = 0; j < ni; j++) {
+      FoldItem it = gitems[(size_t)b * kFoldMaxItems + j];
+      it.blk = b;
+      it.tie_first += to;
+      s_items[io + j] = it;
+    }
+    const int nt = s_to[b + 1] - to;
     for (int j = 0; j < nt; j++) s_ties[to + j] = gties[(size_t)b * kFoldMaxTies + j];
   }
   __syncthreads();
+  // tie count of item i: up to the next item's first tie, or the block's end
+  auto nties_of = [&](int i) {
+    const FoldItem &I = s_items[i];
+    const int end = (i + 1 < TI && s_items[i + 1].blk == I.blk) ? s_items[i + 1].tie_first : s_to[I.blk + 1];
+    return end - I.tie_first;
+  };
+  // segment heads: single steps, items with ties and their successors,
+  // changes of binade or sign
+  for (int i = tid; i < TI; i += kChainThreads) {
+    const FoldItem &I = s_items[i];
+    int h = 1;
+    if (i > 0) {
+      const FoldItem &Q = s_items[i - 1];
+      h = (I.kind != 0 || Q.kind != 0 || nties_of(i) > 0 || nties_of(i - 1) > 0 || I.k != Q.k || I.sgn != Q.sgn) ? 1 : 0;
+    }
+    s_head[i] = h;
+  }
+  __syncthreads();
+  // segmented exclusive scan of R over items (per-thread chunks, then lanes)
+  {
+    const int per = (TI + kChainThreads - 1) / kChainThreads;
+    const int lo = min(TI, tid * per), hi = min(TI, lo + per);
+    long long tail = 0;
+    int has = 0;
+    for (int j = lo; j < hi; j++) {
+      if (s_head[j]) {
+        has = 1;
+        tail = 0;
+      }
+      tail += s_items[j].kind == 0 ? s_items[j].R : 0;
+    }
+    s_ltmp[tid] = tail;
+    s_ftmp[tid] = has;
+    __syncthreads();
+    for (int off = 1; off < kChainThreads; off <<= 1) {
+      long long v = 0;
+      int f = 0;
+      if (tid >= off) {
+        v = s_ltmp[tid - off];
+        f = s_ftmp[tid - off];
+      }
+      __syncthreads();
+      if (tid >= off && !s_ftmp[tid]) {
+        s_ltmp[tid] += v;
+        s_ftmp[tid] |= f;
+      }
+      __syncthreads();
+    }
+    long long run = (tid > 0) ? s_ltmp[tid - 1] : 0;
+    for (int j = lo; j < hi; j++) {
+      if (s_head[j]) run = 0;
+      s_pref[j] = run;
+      run += s_items[j].kind == 0 ? s_items[j].R : 0;
+    }
+  }
+  __syncthreads();
+  // compact the head list (flags -> indices)
+  {
+    const int per = (TI + kChainThreads - 1) / kChainThreads;
+    const int lo = min(TI, tid * per), hi = min(TI, lo + per);
+    int cnt = 0;
+    for (int j = lo; j < hi; j++) cnt += s_head[j];
+    s_tmp[tid] = cnt;
+    __syncthreads();
+    for (int o = 1; o < kChainThreads; o <<= 1) {
+      const int add = (tid >= o) ? s_tmp[tid - o] : 0;
+      __syncthreads();
+      s_tmp[tid] += add;
+      __syncthreads();
+    }
+    int pos = s_tmp[tid] - cnt;
+    int flags[8];
+    const int nloc = hi - lo;
+    for (int j = 0; j < nloc && j < 8; j++) flags[j] = s_head[lo + j];
+    __syncthreads();
+    for (int j = 0; j < nloc; j++) {
+      const int f = (j < 8) ? flags[j] : 0;
+      if (f) s_head[pos++] = lo + j;
+    }
+    if (tid == kChainThreads - 1) s_nheads = s_tmp[kChainThreads - 1];
+  }
+  __syncthreads();
+  // serial walk over the segment heads
   if (tid == 0) {
+    const int nh = s_nheads;
     double x = A.x0[p];
     A.out[p][A.i0] = x;
     int ok = 1;
-    for (int b = 0; b < nblk && ok; b++) {
-      const int io = s_io[b], to = s_to[b];
-      ok = chain_block(A, p, b, s_items + io, s_io[b + 1] - io, s_ties + to, s_to[b + 1] - to, x) ? 1 : 0;
+    for (int q = 0; q < nh && ok; q++) {
+      const int h = s_head[q];
+      const int last = ((q + 1 < nh) ? s_head[q + 1] : TI) - 1;
+      FoldItem &I = s_items[h];
+      // the segment's later runs add their totals to the head's end value
+      // (s_pref is the exclusive prefix from the head, so it includes R_h)
+      const long long extra = (last > h) ? s_pref[last] + s_items[last].R - I.R : 0;
+      ok = chain_item<FoldItem &>(A, p, I.blk, I, s_ties + I.tie_first, nties_of(h), x, extra) ? 1 : 0;
     }
     s_ok = ok;
     if (!ok) W.status[p] = 1;
   }
   __syncthreads();
   if (!s_ok) return;
+  // run starts inside segments, re-checked against their binade
+  const int nh = s_nheads;
+  int bad = 0;
+  for (int q = tid; q < nh; q += kChainThreads) {
+    const int h = s_head[q];
+    const int last = ((q + 1 < nh) ? s_head[q + 1] : TI) - 1;
+    const long long mh = s_items[h].m_start;
+    for (int i = h + 1; i <= last; i++) {
+      const long long m = mh + s_pref[i];
+      const long long am = m < 0 ? -m : m;
+      if (am < (1LL << 52) || am >= (1LL << 53)) bad = 1;
+      s_items[i].m_start = m;
+    }
+  }
+  if (bad) W.status[p] = 1;
+  __syncthreads();
   for (int b = tid; b < nblk; b += kChainThreads) {
     const int io = s_io[b], ni = s_io[b + 1] - io;
     for (int j = 0; j < ni; j++) gitems[(size_t)b * kFoldMaxItems + j].m_start = s_items[io + j].m_start;
