@@ -49,6 +49,7 @@ def lib():
         "aak_likelihood_batch": (_i, [C.c_void_p, _pd, _i, _pd, _pi]),
         "aak_set_host_threads": (_i, [C.c_void_p, _i]),
         "aak_set_precision": (_i, [C.c_void_p, _i]),
+        "aak_gen_phases": (_i, [C.c_void_p, _pd] + [_pd] * 8),
         "aak_get_batch_timings": (_i, [C.c_void_p, _pd, _i]),
         "aak_test_inspiral_rates": (_i, [_d, _d, _d, _d, _pd]),
         "aak_test_get_dense": (_i, [C.c_void_p, _pd]),

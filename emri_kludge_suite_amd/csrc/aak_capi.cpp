@@ -131,6 +131,7 @@ struct aak_handle_s {
   double *d_w = nullptr;      // [2][nbins]
   double *d_partials = nullptr, *d_like = nullptr;
   double *d_like_batch = nullptr;
+  double *d_phase = nullptr;  // [8][N] phase output, allocated on first use
   size_t like_batch_cap = 0;
   PinnedBuf like_host;
   rocfft_plan plan = nullptr;
@@ -292,7 +293,7 @@ static void free_handle(aak_handle h) {
   if (h->exec) rocfft_execution_info_destroy(h->exec);
   void *ptrs[] = {h->d_table[0], h->d_table[1], h->d_dense, h->d_h, h->d_t, h->d_flags, h->d_guard,
                   h->d_bessel, h->d_foldmem, h->d_hf, h->d_data, h->d_w, h->d_partials, h->d_like,
-                  h->d_like_batch, h->d_work};
+                  h->d_like_batch, h->d_work, h->d_phase};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   h->single_pin.release();
@@ -312,8 +313,9 @@ static void free_handle(aak_handle h) {
 // Enqueue the device stage of one planned waveform.  The knot table goes to
 // device table `slot` on the copy stream; `copied` (optional) is recorded when
 // the host table may be reused.
-static int enqueue_waveform(aak_handle h, const WaveformPlan &plan, int slot, hipEvent_t copied) {
-  const DenseParams &P = plan.P;
+static int enqueue_waveform(aak_handle h, const WaveformPlan &plan, int slot, hipEvent_t copied,
+                            const DenseParams *Pover = nullptr) {
+  const DenseParams &P = Pover ? *Pover : plan.P;
   const size_t need = plan.tab_n;
   if (need > h->table_cap[slot]) {
     // growing: no kernel may still read the old table
@@ -349,6 +351,10 @@ static int enqueue_waveform(aak_handle h, const WaveformPlan &plan, int slot, hi
   }
   launch_fold_fill(P, B, h->stream);
   h->mark(2);
+  if (Pover) {  // dense stage only
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   launch_modesum(P, B, h->stream);
   h->mark(3);
   HIPCHK(hipGetLastError());
@@ -551,6 +557,39 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   I[7] = flags[0];
   I[8] = plan.knots.j0;
   I[9] = plan.knots.plunged;
+  return 0;
+}
+
+int aak_gen_phases(aak_handle h, const double *params, double *t, double *phase_r, double *phase_theta,
+                   double *phase_phi, double *omega_r, double *omega_theta, double *omega_phi, double *ecc) {
+  if (!h) return fail(-1, "null handle");
+  if (!params) return fail(-2, "null parameters");
+  SourceParams sp;
+  std::memcpy(&sp, params, sizeof(SourceParams));
+  WaveformPlan &plan = h->single;
+  std::string err;
+  PinnedSink sink(&h->single_pin);
+  int rc = plan_waveform(h->cfg, sp, &plan, &err, &sink);
+  if (rc < 0) {
+    h->state = 0;
+    return fail(rc, err);
+  }
+  if (!h->d_phase) HIPCHK(hipMalloc(&h->d_phase, sizeof(double) * 8 * (size_t)h->N));
+  // the phase output applies the Phidot guard to every forward sample (AAKPhase.cc:233-241)
+  DenseParams P = plan.P;
+  P.guard_last = P.N - 1;
+  rc = enqueue_waveform(h, plan, 0, nullptr, &P);
+  if (rc) return rc;
+  const DenseBuffers B = h->bufs(0);
+  launch_phase_out(P, B, h->d_phase, h->stream);
+  HIPCHK(hipGetLastError());
+  double *dst[8] = {t, phase_r, phase_theta, phase_phi, omega_r, omega_theta, omega_phi, ecc};
+  for (int a = 0; a < 8; a++)
+    if (dst[a])
+      HIPCHK(hipMemcpyAsync(dst[a], h->d_phase + (size_t)a * h->N, sizeof(double) * h->N, hipMemcpyDeviceToHost,
+                            h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->state = 0;
   return 0;
 }
 

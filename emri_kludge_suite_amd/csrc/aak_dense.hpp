@@ -75,6 +75,7 @@ struct DenseParams {
   int K;           // spline knots
   int ip, ib;      // plunge index and taper length in samples
   int last_fold;   // last forward step index folded (min(N-1, ip+ib))
+  int guard_last;  // last sample of the Phidot guard (last_fold, or N-1 for phase output)
   int nmodes;
   int mich;        // LISA response (1) or polarisation rotation (0)
   double dt;
@@ -156,6 +157,7 @@ void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_
 int guard_blocks(const DenseParams &P);
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 void launch_clear_flags(const DenseBuffers &B, hipStream_t s);
+void launch_phase_out(const DenseParams &P, const DenseBuffers &B, double *out, hipStream_t s);
 void launch_weighted_dot(const double2 *hf, const double2 *data, const double *w, int nbins, double2 *hw_out,
                          double *partials, double *out, hipStream_t s);
 

@@ -485,6 +485,7 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   P.ib = ib;
   P.last_fold = std::min(N - 1, ip + ib);
   if (P.last_fold < P.i0) P.last_fold = P.i0 - 1;
+  P.guard_last = P.last_fold;
   P.t_plunge = tp;
   P.t_zero = tp + cfg.dt * ib;
   // mode-sum constants (AAK.cc:323-338, 434-479; KSParMap.cc:541-549)

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   B.rate[0][i] = (Th - R) / T / M;
   B.rate[1][i] = phidot;
   B.rate[2][i] = (Ph - Th) / T / M;
-  if (i > P.i0 && i <= P.last_fold && !(phidot > 0.)) atomicAdd(&B.flags[0], 1);
+  if (i > P.i0 && i <= P.guard_last && !(phidot > 0.)) atomicAdd(&B.flags[0], 1);
 }
 
 void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
@@ -139,7 +139,7 @@ void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
 }
 
 // Non-positive radial rates take the previous (already corrected) value
-// (AAK.cc:254): sample i in (i0, last_fold] with Phidot <= 0 (or NaN) gets the
+// (AAK.cc:254): sample i in (i0, guard_last] with Phidot <= 0 (or NaN) gets the
 // rate of the last sample j in (i0, i] with Phidot > 0, or of i0 if there is
 // none.  That source index is a running maximum, computed as a two-pass
 // parallel scan; both passes return at once when k_rates counted no such
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kGuardThreads) void k_guard_last(DenseParams P, Den
   int last = -1;
   for (int q = 0; q < kGuardPer; q++) {
     const int i = base + q;
-    if (i <= P.last_fold && phidot_ok(r[i])) last = i;
+    if (i <= P.guard_last && phidot_ok(r[i])) last = i;
   }
   s_m[threadIdx.x] = last;
   __syncthreads();
@@ -192,8 +192,8 @@ __global__ __launch_bounds__(kGuardThreads) void k_guard_fix(DenseParams P, Dens
   int last = -1;
   for (int q = 0; q < kGuardPer; q++) {
     const int i = base + q;
-    v[q] = (i <= P.last_fold) ? r[i] : 1.0;
-    if (i <= P.last_fold && phidot_ok(v[q])) last = i;
+    v[q] = (i <= P.guard_last) ? r[i] : 1.0;
+    if (i <= P.guard_last && phidot_ok(v[q])) last = i;
   }
   // inclusive max-scan of the per-thread last good index
   s_m[threadIdx.x] = last;
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kGuardThreads) void k_guard_fix(DenseParams P, Dens
   int src = max(carry, threadIdx.x > 0 ? s_m[threadIdx.x - 1] : -1);
   for (int q = 0; q < kGuardPer; q++) {
     const int i = base + q;
-    if (i > P.last_fold) break;
+    if (i > P.guard_last) break;
     if (phidot_ok(v[q])) {
       src = i;
     } else {
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kGuardThreads) void k_guard_fix(DenseParams P, Dens
 }
 
 int guard_blocks(const DenseParams &P) {
-  const int n = P.last_fold - P.i0;
+  const int n = P.guard_last - P.i0;
   return n > 0 ? (n + kGuardBlock - 1) / kGuardBlock : 0;
 }
 
@@ -838,6 +838,28 @@ __global__ void k_clear_flags(int *flags) {
 
 void launch_clear_flags(const DenseBuffers &B, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, s, B.flags);
+}
+
+// Phases and frequencies per sample (AAKPhase.cc:338-346): t, Phi, Phi+gamma,
+// Phi+gamma+alpha, Phidot, Phidot+gammadot, Phidot+gammadot+alphadot, e.
+__global__ void k_phase_out(DenseParams P, DenseBuffers B, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = P.N;
+  if (i >= N) return;
+  const double gim = B.phase[0][i], Phi = B.phase[1][i], alp = B.phase[2][i];
+  const double gd = B.rate[0][i], Pd = B.rate[1][i], ad = B.rate[2][i];
+  out[i] = sample_time(P, i);
+  out[(size_t)N + i] = Phi;
+  out[2 * (size_t)N + i] = Phi + gim;
+  out[3 * (size_t)N + i] = Phi + gim + alp;
+  out[4 * (size_t)N + i] = Pd;
+  out[5 * (size_t)N + i] = Pd + gd;
+  out[6 * (size_t)N + i] = Pd + gd + ad;
+  out[7 * (size_t)N + i] = B.e[i];
+}
+
+void launch_phase_out(const DenseParams &P, const DenseBuffers &B, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_phase_out, dim3((P.N + 255) / 256), dim3(256), 0, s, P, B, out);
 }
 
 // phases after the plunge buffer stay at their plunge values (AAK.cc:265-269)

@@ -80,6 +80,16 @@ int aak_set_precision(aak_handle h, int mode);
  * concurrency, at most 16); no counterpart in the reference. */
 int aak_set_host_threads(aak_handle h, int n);
 
+/* AAK phases and frequencies of one source, on the handle's grid: the
+ * reference's CPU AAKphase (AAKpy.cc:53-88, AAKPhase.cc:13-374; Python
+ * AAKwrapper.phase, AAKwrapper.pyx:90-133).  params: the 14 gpu_gen_AAK
+ * parameters.  Each output (length doubles, any may be NULL): t, phase_r =
+ * Phi, phase_theta = Phi+gamma, phase_phi = Phi+gamma+alpha, omega_r = Phidot,
+ * omega_theta = Phidot+gammadot, omega_phi = Phidot+gammadot+alphadot,
+ * eccentricity.  The waveform buffers are left unspecified. */
+int aak_gen_phases(aak_handle h, const double *params, double *t, double *phase_r, double *phase_theta,
+                   double *phase_phi, double *omega_r, double *omega_theta, double *omega_phi, double *ecc);
+
 /* OmegaPhi(v, e, cosiota, s, M) (gpuAAK.h:13, gpuAAK.cc:17-26). */
 double aak_OmegaPhi(double v, double e, double cosiota, double s, double M);
 

@@ -125,6 +125,11 @@ int o_aak_wave(const double *pars, double T_fit, int length, double dt,
  * c_n at eccentricity e, amplitude Amp and phase nPhi. */
 void o_mode_abc(int n, double e, double Amp, double nPhi, double *abc);
 
+/* AAK phases (AAKpy.cc:53-88, AAKPhase.cc): out[8*length] = t | Phi |
+ * Phi+gamma | Phi+gamma+alpha | Phidot | Phidot+gammadot |
+ * Phidot+gammadot+alphadot | e. */
+int o_aak_phase(const double *pars, double T_fit, int length, double dt, int backint, double *out);
+
 /* Mode sum + response + taper given dense arrays (AAK.cc:409-547). */
 int o_waveform_from_dense(const double *dense, int vlength, double timestep,
                           int nmodes, double zeta, double M_phys, double lam,

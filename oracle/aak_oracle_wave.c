@@ -231,11 +231,28 @@ done:
 
 /* Dense arrays and sequential phase folds (AAK.cc:228-295).
  * dense layout: 10 blocks of (vlength+1): t,e,v,M,S,gim,Phi,alp,nu,gimdot. */
+static int o_pn_dense_rates(double timestep, int vlength, const double *par, int backint,
+                            const double *t_in, const double *e_in, const double *v_in,
+                            const double *M_in, const double *S_in, int K, int i0,
+                            int plunged, double t_fit_jmin, double *dense,
+                            int *i_plunge, int *i_buffer, double *Phidotvec, double *alpdotvec);
+
 static int o_pn_dense(double timestep, int vlength, const double *par, int backint,
                       const double *t_in, const double *e_in, const double *v_in,
                       const double *M_in, const double *S_in, int K, int i0,
                       int plunged, double t_fit_jmin, double *dense,
                       int *i_plunge, int *i_buffer) {
+  return o_pn_dense_rates(timestep, vlength, par, backint, t_in, e_in, v_in, M_in, S_in, K, i0, plunged,
+                          t_fit_jmin, dense, i_plunge, i_buffer, NULL, NULL);
+}
+
+/* Same fold; Phidotvec/alpdotvec (optional, vlength+1) receive the rates as
+ * PNevolutionPhase stores them (AAKPhase.cc:233-290). */
+static int o_pn_dense_rates(double timestep, int vlength, const double *par, int backint,
+                            const double *t_in, const double *e_in, const double *v_in,
+                            const double *M_in, const double *S_in, int K, int i0,
+                            int plunged, double t_fit_jmin, double *dense,
+                            int *i_plunge, int *i_buffer, double *Phidotvec, double *alpdotvec) {
   const size_t L = (size_t)vlength + 1;
   double *t = dense, *e = dense + L, *v = dense + 2 * L, *M = dense + 3 * L, *S = dense + 4 * L;
   double *gim = dense + 5 * L, *Phi = dense + 6 * L, *alp = dense + 7 * L, *nu = dense + 8 * L, *gdv = dense + 9 * L;
@@ -276,6 +293,7 @@ static int o_pn_dense(double timestep, int vlength, const double *par, int backi
     if (i == i0) { gimdotm = gimdot; Phidotm = Phidot; alpdotm = alpdot; }
     nu[i] = Phidot / 2. / M_PI;
     gdv[i] = gimdot;
+    if (Phidotvec) { Phidotvec[i] = Phidot; alpdotvec[i] = alpdot; }
     gim[i + 1] = gim[i] + (1.5 * gimdot - .5 * gimdotm) * timestep;
     Phi[i + 1] = Phi[i] + (1.5 * Phidot - .5 * Phidotm) * timestep;
     alp[i + 1] = alp[i] + (1.5 * alpdot - .5 * alpdotm) * timestep;
@@ -289,6 +307,10 @@ static int o_pn_dense(double timestep, int vlength, const double *par, int backi
     const int n = vlength;
     nu[n] = o_drdm(v[n], e[n], coslam, S[n]) / o_dtdm(v[n], e[n], coslam, S[n]) / (2. * M_PI * M[n]);
     gdv[n] = (o_dthetadm(v[n], e[n], coslam, S[n]) - o_drdm(v[n], e[n], coslam, S[n])) / o_dtdm(v[n], e[n], coslam, S[n]) / M[n];
+    if (Phidotvec) {
+      Phidotvec[n] = o_drdm(v[n], e[n], coslam, S[n]) / o_dtdm(v[n], e[n], coslam, S[n]) / M[n];
+      alpdotvec[n] = (o_dphidm(v[n], e[n], coslam, S[n]) - o_dthetadm(v[n], e[n], coslam, S[n])) / o_dtdm(v[n], e[n], coslam, S[n]) / M[n];
+    }
   }
   for (int i = i0; i > 0; i--) {
     gimdotp = gimdot;
@@ -300,12 +322,17 @@ static int o_pn_dense(double timestep, int vlength, const double *par, int backi
     if (i == i0) { gimdotp = gimdot; Phidotp = Phidot; alpdotp = alpdot; }
     nu[i] = Phidot / 2. / M_PI;
     gdv[i] = gimdot;
+    if (Phidotvec) { Phidotvec[i] = Phidot; alpdotvec[i] = alpdot; }
     gim[i - 1] = gim[i] - (1.5 * gimdot - .5 * gimdotp) * timestep;
     Phi[i - 1] = Phi[i] - (1.5 * Phidot - .5 * Phidotp) * timestep;
     alp[i - 1] = alp[i] - (1.5 * alpdot - .5 * alpdotp) * timestep;
   }
   nu[0] = o_drdm(v[0], e[0], coslam, S[0]) / o_dtdm(v[0], e[0], coslam, S[0]) / (2. * M_PI * M[0]);
   gdv[0] = (o_dthetadm(v[0], e[0], coslam, S[0]) - o_drdm(v[0], e[0], coslam, S[0])) / o_dtdm(v[0], e[0], coslam, S[0]) / M[0];
+  if (Phidotvec) {
+    Phidotvec[0] = o_drdm(v[0], e[0], coslam, S[0]) / o_dtdm(v[0], e[0], coslam, S[0]) / M[0];
+    alpdotvec[0] = (o_dphidm(v[0], e[0], coslam, S[0]) - o_dthetadm(v[0], e[0], coslam, S[0])) / o_dtdm(v[0], e[0], coslam, S[0]) / M[0];
+  }
   return 0;
 }
 
@@ -492,6 +519,53 @@ int o_aak_wave(const double *pars, double T_fit, int length, double dt,
     meta[4] = K; meta[5] = info[0]; meta[6] = Phi0; meta[7] = zeta;
   }
   if (!dense) free(dn);
+  free(kn);
+  return rc;
+}
+
+/* AAK phases and frequencies (AAKpy.cc:53-88, AAKPhase.cc:13-374): the sparse
+ * stage and knots of the waveform, the same dense folds, then per sample
+ * t, Phi, Phi+gamma, Phi+gamma+alpha, their rates and e (GenPhase,
+ * AAKPhase.cc:338-346).  out: 8 arrays of length each. */
+int o_aak_phase(const double *pars, double T_fit, int length, double dt, int backint, double *out) {
+  const double iota = pars[0], s = pars[1], p = pars[2], e = pars[3], M = pars[4], mu = pars[5];
+  const double gamma = pars[6], psi = pars[7], alph = pars[8], qS = pars[9], phiS = pars[10];
+  const double qK = pars[11], phiK = pars[12];
+  double e_traj[100], v_map[100], M_map[100], s_map[100], dt_map[100], Phi0 = 0;
+  const int steps = o_sparse_stage(T_fit, length, dt, iota, s, p, e, M, mu, psi,
+                                   e_traj, v_map, M_map, s_map, dt_map, &Phi0, NULL);
+  if (steps < 3) return steps < 0 ? steps : -2;
+  double par[12] = {mu * O_MSUN_SEC, M_map[0] * O_MSUN_SEC, s_map[0], e_traj[0], iota, gamma, Phi0,
+                    qS, phiS, qK, phiK, alph};
+  const int j0 = o_pn_j0(dt, length, dt_map, steps);
+  if (j0 < steps || j0 > 400000000) { o_set_error("knot count out of range"); return -3; }
+  const int cap = 2 * j0 + 8;
+  double *kn = malloc(sizeof(double) * 5 * (size_t)cap);
+  int info[9];
+  const int K = o_pn_knots(dt, length, par, e_traj, v_map, M, M_map, s, s_map, dt_map, steps, backint,
+                           kn, kn + cap, kn + 2 * cap, kn + 3 * cap, kn + 4 * cap, cap, info);
+  if (K < 0) { free(kn); return K; }
+  const size_t L = (size_t)length + 1;
+  double *dn = malloc(sizeof(double) * 12 * L);
+  double *Pd = dn + 10 * L, *Ad = dn + 11 * L;
+  const double t_fit_jmin = kn[info[2] - info[5]];
+  int i_plunge = 0, i_buffer = 0;
+  const int rc = o_pn_dense_rates(dt, length, par, backint, kn, kn + cap, kn + 2 * cap, kn + 3 * cap,
+                                  kn + 4 * cap, K, info[0], info[7], t_fit_jmin, dn, &i_plunge, &i_buffer, Pd, Ad);
+  if (rc == 0) {
+    const double *t = dn, *ev = dn + L, *gim = dn + 5 * L, *Phi = dn + 6 * L, *alp = dn + 7 * L, *gd = dn + 9 * L;
+    for (int i = 0; i < length; i++) {
+      out[i] = t[i];
+      out[length + i] = Phi[i];
+      out[2 * (size_t)length + i] = Phi[i] + gim[i];
+      out[3 * (size_t)length + i] = Phi[i] + gim[i] + alp[i];
+      out[4 * (size_t)length + i] = Pd[i];
+      out[5 * (size_t)length + i] = Pd[i] + gd[i];
+      out[6 * (size_t)length + i] = Pd[i] + gd[i] + Ad[i];
+      out[7 * (size_t)length + i] = ev[i];
+    }
+  }
+  free(dn);
   free(kn);
   return rc;
 }

@@ -70,6 +70,7 @@ def oracle():
             "o_waveform_from_dense": (_i, [_pd, _i, _d, _i, _d, _d, _d, _d, _d, _d, _d,
                                            _i, _i, _i, _pd, _pd, _pd]),
             "o_mode_abc": (None, [_i, _d, _d, _d, _pd]),
+            "o_aak_phase": (_i, [_pd, _d, _i, _d, _i, _pd]),
             "o_last_error": (C.c_char_p, []),
         }
         for name, (res, args) in sig.items():
@@ -171,3 +172,18 @@ def o_aak_wave(pars, T_fit=1.0, length=2 ** 14, dt=10.0, LISA=True, backint=Fals
             res["dense_" + name] = d[k]
         res["dense"] = dense
     return res
+
+
+PHASE_NAMES = ("t", "phase_r", "phase_theta", "phase_phi", "omega_r", "omega_theta", "omega_phi", "eccentricity")
+
+
+def o_aak_phase(pars, T_fit=1.0, length=2 ** 14, dt=10.0, backint=False):
+    """AAK phases and frequencies (AAKwrapper.phase)."""
+    if isinstance(pars, dict):
+        pars = [pars[k] for k in PARAM_ORDER]
+    pv = np.ascontiguousarray(pars, dtype=np.float64)
+    out = np.zeros(8 * length)
+    rc = oracle().o_aak_phase(_ptr(pv), T_fit, length, dt, int(backint), _ptr(out))
+    if rc != 0:
+        raise RuntimeError("oracle AAK phase failed (%d): %s" % (rc, oracle().o_last_error().decode()))
+    return dict(zip(PHASE_NAMES, out.reshape(8, length)))
