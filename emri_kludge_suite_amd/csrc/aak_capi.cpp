@@ -132,6 +132,10 @@ struct aak_handle_s {
   double *d_partials = nullptr, *d_like = nullptr;
   double *d_like_batch = nullptr;
   double *d_phase = nullptr;  // [8][N] phase output, allocated on first use
+  double *d_ms_coef = nullptr; // mode-sum block Taylor coefficients (grown with nmodes)
+  size_t ms_coef_cap = 0;
+  int *d_ms_p = nullptr;
+  double *d_ms_eref = nullptr;
   size_t like_batch_cap = 0;
   PinnedBuf like_host;
   rocfft_plan plan = nullptr;
@@ -184,6 +188,9 @@ struct aak_handle_s {
     B.t = d_t;
     B.flags = d_flags;
     B.guard_blk = d_guard;
+    B.ms_coef = d_ms_coef;
+    B.ms_p = d_ms_p;
+    B.ms_eref = d_ms_eref;
     B.bessel_c = d_bessel;
     B.bessel_lgf = d_bessel + (size_t)(kBesselKMax + 1) * kBesselMMax;
     return B;
@@ -203,6 +210,8 @@ static int alloc_handle(aak_handle h) {
   HIPCHK(hipMalloc(&h->d_t, sizeof(double) * ((size_t)N + 2)));
   HIPCHK(hipMalloc(&h->d_flags, sizeof(int) * 4));
   HIPCHK(hipMalloc(&h->d_guard, sizeof(int) * ((size_t)N / 4096 + 4)));
+  HIPCHK(hipMalloc(&h->d_ms_p, sizeof(int) * ((size_t)N / 256 + 4)));
+  HIPCHK(hipMalloc(&h->d_ms_eref, sizeof(double) * ((size_t)N / 256 + 4)));
   {
     const size_t nc = (size_t)(kBesselKMax + 1) * kBesselMMax, nl = kBesselKMax + 1;
     std::vector<double> tab(nc + nl);
@@ -293,7 +302,7 @@ static void free_handle(aak_handle h) {
   if (h->exec) rocfft_execution_info_destroy(h->exec);
   void *ptrs[] = {h->d_table[0], h->d_table[1], h->d_dense, h->d_h, h->d_t, h->d_flags, h->d_guard,
                   h->d_bessel, h->d_foldmem, h->d_hf, h->d_data, h->d_w, h->d_partials, h->d_like,
-                  h->d_like_batch, h->d_work, h->d_phase};
+                  h->d_like_batch, h->d_work, h->d_phase, h->d_ms_coef, h->d_ms_p, h->d_ms_eref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   h->single_pin.release();
@@ -309,6 +318,9 @@ static void free_handle(aak_handle h) {
   if (h->cstream) (void)hipStreamDestroy(h->cstream);
   if (h->stream) (void)hipStreamDestroy(h->stream);
 }
+
+// Buffers after a possible growth of the mode-sum coefficient buffer.
+static DenseBuffers B2(aak_handle h, int slot) { return h->bufs(slot); }
 
 // Enqueue the device stage of one planned waveform.  The knot table goes to
 // device table `slot` on the copy stream; `copied` (optional) is recorded when
@@ -351,11 +363,23 @@ static int enqueue_waveform(aak_handle h, const WaveformPlan &plan, int slot, hi
   }
   launch_fold_fill(P, B, h->stream);
   h->mark(2);
+  if (!Pover) {
+    // block Taylor coefficients: blocks x nmodes x 3 x (kTaylorMax+1) doubles
+    const size_t need = (size_t)modesum_blocks(P) * (size_t)(P.nmodes > 0 ? P.nmodes : 1) * 3 * (kTaylorMax + 1);
+    if (need > h->ms_coef_cap) {
+      HIPCHK(hipStreamSynchronize(h->stream));
+      if (h->d_ms_coef) HIPCHK(hipFree(h->d_ms_coef));
+      h->d_ms_coef = nullptr;
+      h->ms_coef_cap = 0;
+      HIPCHK(hipMalloc(&h->d_ms_coef, sizeof(double) * need));
+      h->ms_coef_cap = need;
+    }
+  }
   if (Pover) {  // dense stage only
     HIPCHK(hipGetLastError());
     return 0;
   }
-  launch_modesum(P, B, h->stream);
+  launch_modesum(P, B2(h, slot), h->stream);
   h->mark(3);
   HIPCHK(hipGetLastError());
   return 0;

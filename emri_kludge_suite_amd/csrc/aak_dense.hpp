@@ -144,6 +144,9 @@ struct DenseBuffers {
   double *t;                       // [N+2]
   int *flags;                      // [4]: [0] non-positive Phidot count
   int *guard_blk;                  // [guard blocks]: last positive-Phidot index per block
+  double *ms_coef;                 // [mode-sum blocks][nmodes][3][kTaylorMax+1] Taylor coefficients
+  int *ms_p;                       // [mode-sum blocks] Taylor order (-1: per-sample path)
+  double *ms_eref;                 // [mode-sum blocks] expansion point
   const double *bessel_c;          // series coefficients c_m(k)
   const double *bessel_lgf;        // log(k!)
 };
@@ -155,6 +158,7 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
 void launch_fold_fill(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 int guard_blocks(const DenseParams &P);
+int modesum_blocks(const DenseParams &P);  // mode-sum blocks (256 samples each)
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 void launch_clear_flags(const DenseBuffers &B, hipStream_t s);
 void launch_phase_out(const DenseParams &P, const DenseBuffers &B, double *out, hipStream_t s);

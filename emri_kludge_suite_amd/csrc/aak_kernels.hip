@@ -1085,21 +1085,28 @@ constexpr int kTaylorModes = 128;
 // response accumulation) runs in FP32 from FP64-reduced phase factors and
 // FP64 block coefficients; phases, their reduction mod 2 pi, the block
 // expansions and the final scaling stay FP64.
-template <bool MIXED>
-__global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuffers B) {
-  __shared__ double s_coef[kTaylorModes][3][kTaylorMax + 1];
-  __shared__ float s_coef_f[MIXED ? kTaylorModes : 1][3][kTaylorMax + 1];
+// Block setup of the Taylor form, one workgroup per mode-sum block: the range
+// of e over the block's signal samples, the order p and the expansion of
+// alpha_n, beta_n, gamma_n for every harmonic (one thread each), written to
+// B.ms_coef[block][n][3][kTaylorMax+1]; B.ms_p[block] = p (-1: per-sample path).
+constexpr int kSetupThreads = 128;
+
+__global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, DenseBuffers B) {
   __shared__ double s_sq[kTaylorMax + 1];
-  __shared__ double s_min[kMsThreads / 64], s_max[kMsThreads / 64];
+  __shared__ double s_min[kSetupThreads / 64], s_max[kSetupThreads / 64];
   __shared__ double s_eref;
   __shared__ int s_p;
-  const int tid = threadIdx.x;
-  const int i = blockIdx.x * kMsThreads + tid;
-  double *hI = B.h, *hII = B.h + (P.N + 2);
-  const int n_end = min(P.N, P.ip + P.ib);  // samples [0, n_end) carry the signal
-  const bool active = i < n_end;
-  // block range of e over the active samples
-  double emin = active ? B.e[i] : 1e300, emax = active ? B.e[i] : -1e300;
+  const int tid = threadIdx.x, blk = blockIdx.x;
+  const int n_end = min(P.N, P.ip + P.ib);
+  double emin = 1e300, emax = -1e300;
+  for (int q = tid; q < kMsThreads; q += kSetupThreads) {
+    const int i = blk * kMsThreads + q;
+    if (i < n_end) {
+      const double e = B.e[i];
+      emin = fmin(emin, e);
+      emax = fmax(emax, e);
+    }
+  }
   for (int o = 32; o > 0; o >>= 1) {
     emin = fmin(emin, __shfl_xor(emin, o));
     emax = fmax(emax, __shfl_xor(emax, o));
@@ -1111,30 +1118,60 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
   __syncthreads();
   if (tid == 0) {
     double lo = s_min[0], hi = s_max[0];
-    for (int w = 1; w < kMsThreads / 64; w++) {
+    for (int w = 1; w < kSetupThreads / 64; w++) {
       lo = fmin(lo, s_min[w]);
       hi = fmax(hi, s_max[w]);
     }
     int p = -1;
+    double eref = 0.0;
     if (lo <= hi && P.bessel_series && P.nmodes <= kTaylorModes && hi <= kSeriesMaxE) {
-      s_eref = 0.5 * (lo + hi);
+      eref = 0.5 * (lo + hi);
       p = taylor_order(P.nmodes, 0.5 * (hi - lo));
       if (p >= 0) {
         double sq[kTaylorMax + 1];
-        taylor_sqrt1me2(s_eref, p, sq);
+        taylor_sqrt1me2(eref, p, sq);
         for (int j = 0; j <= kTaylorMax; j++) s_sq[j] = j <= p ? sq[j] : 0.0;
       }
     }
+    s_eref = eref;
     s_p = p;
+    B.ms_p[blk] = p;
+    B.ms_eref[blk] = eref;
   }
   __syncthreads();
   const int p = s_p;
   if (p >= 0 && tid < P.nmodes) {
     // series length: the host bound for orders n+1, n+2 covers the higher orders
-    taylor_amplitudes(tid + 1, p, s_eref, s_sq, B.bessel_c, B.bessel_lgf, P.bterms[tid + 1], &s_coef[tid][0][0]);
-    if (MIXED) {
-      for (int f = 0; f < 3; f++)
-        for (int j = 0; j <= kTaylorMax; j++) s_coef_f[tid][f][j] = (float)s_coef[tid][f][j];
+    double coef[3 * (kTaylorMax + 1)];
+    taylor_amplitudes(tid + 1, p, s_eref, s_sq, B.bessel_c, B.bessel_lgf, P.bterms[tid + 1], coef);
+    double *dst = B.ms_coef + ((size_t)blk * P.nmodes + tid) * (3 * (kTaylorMax + 1));
+    for (int q = 0; q < 3 * (kTaylorMax + 1); q++) dst[q] = coef[q];
+  }
+}
+
+int modesum_blocks(const DenseParams &P) { return (P.N + 2 + kMsThreads - 1) / kMsThreads; }
+
+// MIXED: the harmonic loop (rotation recurrence, amplitude polynomials,
+// response accumulation) runs in FP32 from FP64-reduced phase factors and
+// FP64 block coefficients; phases, their reduction mod 2 pi, the block
+// expansions and the final scaling stay FP64.
+template <bool MIXED>
+__global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuffers B) {
+  __shared__ double s_coef[MIXED ? 1 : kTaylorModes][3][kTaylorMax + 1];
+  __shared__ float s_coef_f[MIXED ? kTaylorModes : 1][3][kTaylorMax + 1];
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * kMsThreads + tid;
+  double *hI = B.h, *hII = B.h + (P.N + 2);
+  const int n_end = min(P.N, P.ip + P.ib);  // samples [0, n_end) carry the signal
+  const bool active = i < n_end;
+  const int p = B.ms_p[blockIdx.x];
+  const double eref = B.ms_eref[blockIdx.x];
+  if (p >= 0) {
+    const int nc = P.nmodes * 3 * (kTaylorMax + 1);
+    const double *src = B.ms_coef + (size_t)blockIdx.x * nc;
+    for (int q = tid; q < nc; q += kMsThreads) {
+      if (MIXED) (&s_coef_f[0][0][0])[q] = (float)src[q];
+      else (&s_coef[0][0][0])[q] = src[q];
     }
   }
   __syncthreads();
@@ -1158,7 +1195,7 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
   if (p < 0) {
     modesum_direct(P, B, G, &accI, &accII);
   } else {
-    const double d = G.e - s_eref;
+    const double d = G.e - eref;
     // exp(i n Phi) by rotation from Phi reduced mod 2 pi (Cody-Waite, three
     // parts of 2 pi); the Doppler term is linear in n: Doppler_n = n dA + dB
     const double kq = rint(G.Phi * (1.0 / (2.0 * kPi)));
@@ -1236,6 +1273,7 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
 
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
   const int n = P.N + 2;
+  hipLaunchKernelGGL(k_modesum_setup, dim3(modesum_blocks(P)), dim3(kSetupThreads), 0, s, P, B);
   if (P.mixed)
     hipLaunchKernelGGL(k_modesum<true>, dim3((n + kMsThreads - 1) / kMsThreads), dim3(kMsThreads), 0, s, P, B);
   else
