@@ -93,17 +93,18 @@ AAK_HD void bessel_five_series(int n, double e, const double *ctab, const double
 constexpr int kTaylorMax = 8;
 constexpr int kTaylorOrders = 5 + 2 * kTaylorMax;
 
-// J at orders n-2-p .. n+2+p (out[q] = J_{n-2-p+q}) at x = n e_ref, negative
+// J at orders n-2-p .. n+2+p (out[q * S] = J_{n-2-p+q}) at x = n e_ref, negative
 // orders by J_{-m} = (-1)^m J_m.  Runs that reach order 0 (low harmonics,
 // where J_0 ~ 1) use Miller's recurrence normalised with J_0 + 2 sum J_2k = 1;
 // the others take the two highest orders from the series and recur down
 // (their values are small enough that the prefactor's relative error does not
 // matter absolutely).  M: series length valid for orders >= n+1.
+template <int S = 1>
 AAK_HD void bessel_run(int n, int p, double e_ref, const double *ctab, const double *lgf, int M, double *out) {
   const int kmin = n - 2 - p, kmax = n + 2 + p, cnt = kmax - kmin + 1;
   const double x = n * e_ref;
   if (!(e_ref > 1e-150)) {
-    for (int q = 0; q < cnt; q++) out[q] = (kmin + q == 0) ? 1.0 : 0.0;
+    for (int q = 0; q < cnt; q++) out[q * S] = (kmin + q == 0) ? 1.0 : 0.0;
     return;
   }
   if (kmin <= 0) {
@@ -112,7 +113,7 @@ AAK_HD void bessel_run(int n, int p, double e_ref, const double *ctab, const dou
     const double inv = 1.0 / x;
     double jp1 = 0.0, j = 1.0e-280, sum = 0.0;
     for (int k = top; k >= 1; k--) {
-      if (k <= kmax) out[k - kmin] = j;
+      if (k <= kmax) out[(k - kmin) * S] = j;
       if ((k & 1) == 0) sum += 2.0 * j;
       const double jm1 = (2.0 * k) * inv * j - jp1;
       jp1 = j;
@@ -121,17 +122,17 @@ AAK_HD void bessel_run(int n, int p, double e_ref, const double *ctab, const dou
         j *= 1.0e-250;
         jp1 *= 1.0e-250;
         sum *= 1.0e-250;
-        for (int q = k - kmin; q < cnt && q >= 0; q++) out[q] *= 1.0e-250;
+        for (int q = k - kmin; q < cnt && q >= 0; q++) out[q * S] *= 1.0e-250;
       }
     }
-    out[-kmin] = j;
+    out[(-kmin) * S] = j;
     sum += j;
     const double norm = 1.0 / sum;
-    for (int q = -kmin; q < cnt; q++) out[q] *= norm;
+    for (int q = -kmin; q < cnt; q++) out[q * S] *= norm;
     for (int k = kmin; k < 0; k++) {
       const int m = -k;
-      const double v = out[m - kmin];
-      out[k - kmin] = (m & 1) ? -v : v;
+      const double v = out[(m - kmin) * S];
+      out[(k - kmin) * S] = (m & 1) ? -v : v;
     }
     return;
   }
@@ -139,14 +140,14 @@ AAK_HD void bessel_run(int n, int p, double e_ref, const double *ctab, const dou
   const double ptop = exp(kmax * lhx - lgf[kmax]);
   const double jt = ptop * bessel_series_sum(ctab + kmax * kBesselMMax, M, y);
   const double jt1 = ptop * ((double)kmax / (0.5 * x)) * bessel_series_sum(ctab + (kmax - 1) * kBesselMMax, M, y);
-  out[cnt - 1] = jt;
-  out[cnt - 2] = jt1;
+  out[(cnt - 1) * S] = jt;
+  out[(cnt - 2) * S] = jt1;
   const double inv = 1.0 / x;
   double jk1 = jt, jk = jt1;
   const int klo = kmin > 0 ? kmin : 0;
   for (int k = kmax - 1; k > klo; k--) {
     const double jm = (2.0 * k) * inv * jk - jk1;
-    out[k - 1 - kmin] = jm;
+    out[(k - 1 - kmin) * S] = jm;
     jk1 = jk;
     jk = jm;
   }
@@ -163,52 +164,62 @@ AAK_HD void taylor_sqrt1me2(double e_ref, int p, double *sq) {
   }
 }
 
-// coef[f * (kTaylorMax + 1) + j], f = alpha, beta, gamma; j = 0..kTaylorMax
-// (zero above p).  sq: Taylor coefficients of sqrt(1 - e^2) at e_ref.
+// Taylor coefficients of alpha_n, beta_n, gamma_n at e_ref to order PO into
+// coef[f * (PO + 1) + j] (f = alpha, beta, gamma).  Jr[q * S] = J_{n-2-PO+q}(n e_ref)
+// for q < 5 + 2 PO; sq: Taylor coefficients of sqrt(1 - e^2) at e_ref.  The
+// signed binomial sums of the derivative formula are formed as iterated
+// differences, D_j(k) = D_{j-1}(k-1) - D_{j-1}(k+1) with D_0 = J, on one row that
+// shrinks by two orders per step.
+template <int PO, int S>
+AAK_HD void taylor_amplitudes_p(int n, double e_ref, const double *sq, const double *Jr, double *coef) {
+  constexpr int L = 5 + 2 * PO;
+  double row[L];
+  for (int q = 0; q < L; q++) row[q] = Jr[q * S];
+  double t[5][PO + 1];  // t[k][j] = (n/2)^j / j! D_j(n - 2 + k)
+  double hn = 1.0;
+  for (int j = 0; j <= PO; j++) {
+    for (int k = 0; k < 5; k++) t[k][j] = hn * row[k + PO - j];
+    for (int q = 0; q + 2 < L - 2 * j; q++) row[q] = row[q] - row[q + 2];
+    hn = hn * (0.5 * n) * (1.0 / (j + 1));
+  }
+  const double dn = (double)n, two_n = 2. / dn;
+  for (int j = 0; j <= PO; j++) {
+    const double e1 = e_ref * t[1][j] + (j ? t[1][j - 1] : 0.0);
+    const double e3 = e_ref * t[3][j] + (j ? t[3][j - 1] : 0.0);
+    coef[j] = -dn * (t[0][j] - 2. * e1 + two_n * t[2][j] + 2. * e3 - t[4][j]);
+    double bsum = 0.0;
+    for (int q = 0; q <= j; q++) bsum += sq[q] * (t[0][j - q] - 2. * t[2][j - q] + t[4][j - q]);
+    coef[PO + 1 + j] = -dn * bsum;
+    coef[2 * (PO + 1) + j] = 2. * t[2][j];
+  }
+}
+
+// Runtime-order form of taylor_amplitudes_p (coef holds 3 (p + 1) values).
 AAK_HD void taylor_amplitudes(int n, int p, double e_ref, const double *sq, const double *ctab, const double *lgf,
                               int M, double *coef) {
   double Jr[kTaylorOrders];
   bessel_run(n, p, e_ref, ctab, lgf, M, Jr);
-  double t[5][kTaylorMax + 1];
-  double hn = 1.0;  // (n/2)^j / j!
-  for (int j = 0; j <= p; j++) {
-    for (int k = 0; k < 5; k++) {
-      double acc = 0.0, c = 1.0;  // c = C(j, ii)
-      for (int ii = 0; ii <= j; ii++) {
-        const double v = Jr[k - j + 2 * ii + p];  // J_{n-2+k-j+2ii}
-        acc += (ii & 1) ? -c * v : c * v;
-        c = c * (j - ii) / (ii + 1);
-      }
-      t[k][j] = hn * acc;
-    }
-    hn = hn * (0.5 * n) / (j + 1);
-  }
-  const double dn = (double)n;
-  for (int j = 0; j <= kTaylorMax; j++) {
-    double *ca = coef + j, *cb = coef + (kTaylorMax + 1) + j, *cg = coef + 2 * (kTaylorMax + 1) + j;
-    if (j > p) {
-      *ca = 0.0;
-      *cb = 0.0;
-      *cg = 0.0;
-      continue;
-    }
-    const double e1 = e_ref * t[1][j] + (j ? t[1][j - 1] : 0.0);
-    const double e3 = e_ref * t[3][j] + (j ? t[3][j - 1] : 0.0);
-    *ca = -dn * (t[0][j] - 2. * e1 + 2. / dn * t[2][j] + 2. * e3 - t[4][j]);
-    double bsum = 0.0;
-    for (int q = 0; q <= j; q++) bsum += sq[q] * (t[0][j - q] - 2. * t[2][j - q] + t[4][j - q]);
-    *cb = -dn * bsum;
-    *cg = 2. * t[2][j];
+  switch (p) {
+    case 0: taylor_amplitudes_p<0, 1>(n, e_ref, sq, Jr, coef); break;
+    case 1: taylor_amplitudes_p<1, 1>(n, e_ref, sq, Jr, coef); break;
+    case 2: taylor_amplitudes_p<2, 1>(n, e_ref, sq, Jr, coef); break;
+    case 3: taylor_amplitudes_p<3, 1>(n, e_ref, sq, Jr, coef); break;
+    case 4: taylor_amplitudes_p<4, 1>(n, e_ref, sq, Jr, coef); break;
+    case 5: taylor_amplitudes_p<5, 1>(n, e_ref, sq, Jr, coef); break;
+    case 6: taylor_amplitudes_p<6, 1>(n, e_ref, sq, Jr, coef); break;
+    case 7: taylor_amplitudes_p<7, 1>(n, e_ref, sq, Jr, coef); break;
+    default: taylor_amplitudes_p<8, 1>(n, e_ref, sq, Jr, coef); break;
   }
 }
 
-// Horner evaluation of the three expansions at delta.
+// Horner evaluation of the three expansions (layout of taylor_amplitudes) at delta.
 AAK_HD void taylor_eval(const double *coef, int p, double d, double *abg) {
-  double al = coef[p], be = coef[kTaylorMax + 1 + p], ga = coef[2 * (kTaylorMax + 1) + p];
+  const int s = p + 1;
+  double al = coef[p], be = coef[s + p], ga = coef[2 * s + p];
   for (int j = p - 1; j >= 0; j--) {
     al = fma(al, d, coef[j]);
-    be = fma(be, d, coef[kTaylorMax + 1 + j]);
-    ga = fma(ga, d, coef[2 * (kTaylorMax + 1) + j]);
+    be = fma(be, d, coef[s + j]);
+    ga = fma(ga, d, coef[2 * s + j]);
   }
   abg[0] = al;
   abg[1] = be;

@@ -116,7 +116,7 @@ struct aak_handle_s {
   size_t table_cap[2] = {0, 0};
   hipEvent_t ev_upload[2] = {nullptr, nullptr}, ev_rates[2] = {nullptr, nullptr};
   bool rates_recorded[2] = {false, false};
-  double *d_dense = nullptr;  // e, v, M, S, rate[3], phase[3]: 10 x (N+1)
+  double *d_dense = nullptr;  // e, v, M, S, rate[3], phase[3], amp: 11 x (N+1), then nu0
   double *d_h = nullptr;      // [2][N+2]
   double *d_t = nullptr;      // [N+2]
   int *d_flags = nullptr;     // [4]: non-positive Phidot count, fold fallbacks
@@ -184,6 +184,8 @@ struct aak_handle_s {
       B.rate[p] = d_dense + (4 + p) * L;
       B.phase[p] = d_dense + (7 + p) * L;
     }
+    B.amp = d_dense + 10 * L;
+    B.nu0 = d_dense + 11 * L;
     B.h = d_h;
     B.t = d_t;
     B.flags = d_flags;
@@ -205,7 +207,7 @@ static int alloc_handle(aak_handle h) {
     HIPCHK(hipEventCreateWithFlags(&h->ev_upload[k], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&h->ev_rates[k], hipEventDisableTiming));
   }
-  HIPCHK(hipMalloc(&h->d_dense, sizeof(double) * 10 * ((size_t)N + 1)));
+  HIPCHK(hipMalloc(&h->d_dense, sizeof(double) * (11 * ((size_t)N + 1) + 1)));
   HIPCHK(hipMalloc(&h->d_h, sizeof(double) * 2 * ((size_t)N + 2)));
   HIPCHK(hipMalloc(&h->d_t, sizeof(double) * ((size_t)N + 2)));
   HIPCHK(hipMalloc(&h->d_flags, sizeof(int) * 4));

@@ -91,6 +91,13 @@ struct DenseParams {
   double n_vec[3], S_vec[3];      // sky and spin unit vectors
   double cosiota_L, siniota_L;    // cos/sin of the inclination in the L vector
   double sin_thK, cos_thK, sin_phK, cos_phK;
+  double cos_phS, sin_phS;        // cos/sin(phi_S)
+  // orientation terms of one sample as affine functions of (sin alpha, cos alpha):
+  // X = c[0] + c[1] sin(alpha) + c[2] cos(alpha), alpha the Lense-Thirring phase
+  double g_cosqL[3], g_phiLup[3], g_phiLdown[3], g_betadown[3];
+  double g_rc[3], g_rs[3];        // (n x L).(n x S) and L.(n x S) - S.(n x L)
+  double g_nxL2[6];               // |n x L|^2 on 1, sa^2, ca^2, sa, ca, sa ca
+  double nxS2;                    // |n x S|^2
   double zeta, Mphys;             // amplitude scale and M in solar masses
   double t_plunge, t_zero;
   KnotGrid grid;
@@ -140,11 +147,13 @@ struct DenseBuffers {
   double *e, *v, *M, *S;           // [N+1]
   double *rate[3];                 // gimdot, Phidot, alpdot [N+1]
   double *phase[3];                // gim, Phi, alp [N+1]
+  double *amp;                     // [N+1] mode-sum amplitude (Omega_phi M Msun)^(2/3) zeta
+  double *nu0;                     // [1] radial frequency at sample 0 from the PN maps
   double *h;                       // [2][N+2] time domain hI | hII
   double *t;                       // [N+2]
   int *flags;                      // [4]: [0] non-positive Phidot count
   int *guard_blk;                  // [guard blocks]: last positive-Phidot index per block
-  double *ms_coef;                 // [mode-sum blocks][nmodes][3][kTaylorMax+1] Taylor coefficients
+  double *ms_coef;                 // per mode-sum block nmodes x 3 x (kTaylorMax+1) doubles; order p uses [nmodes][3][p+1] (float in mixed mode)
   int *ms_p;                       // [mode-sum blocks] Taylor order (-1: per-sample path)
   double *ms_eref;                 // [mode-sum blocks] expansion point
   const double *bessel_c;          // series coefficients c_m(k)

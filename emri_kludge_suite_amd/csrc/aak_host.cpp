@@ -529,6 +529,48 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   P.cos_thK = cos(qK);
   P.sin_phK = sin(phiK);
   P.cos_phK = cos(phiK);
+  P.cos_phS = cos(phiS);
+  P.sin_phS = sin(phiS);
+  {
+    // the per-sample orientation of L (KSParMap.cc:532-574, AAK.cc:425-433) on
+    // the basis (1, sin alpha, cos alpha)
+    const double cl = P.coslam, sl = P.sinlam;
+    const double g_cosqL[3] = {P.cosqK * cl, 0.0, P.sinqK * sl};
+    const double g_up[3] = {P.sinqK * P.sinphiK * cl, -P.cosphiK * sl, -P.cosqK * P.sinphiK * sl};
+    const double g_dw[3] = {P.sinqK * P.cosphiK * cl, P.sinphiK * sl, -P.cosqK * P.cosphiK * sl};
+    const double g_bd[3] = {0.0, (P.cosqK * P.Sdotn - P.cosqS) / P.sinqK * sl, P.sinqS * P.sin_dphiKS * sl};
+    for (int k = 0; k < 3; k++) {
+      P.g_cosqL[k] = g_cosqL[k];
+      P.g_phiLup[k] = g_up[k];
+      P.g_phiLdown[k] = g_dw[k];
+      P.g_betadown[k] = g_bd[k];
+    }
+    const double ci = P.cosiota_L, si = P.siniota_L;
+    const double Lk[3][3] = {{ci * P.sin_thK * P.cos_phK, ci * P.sin_thK * P.sin_phK, ci * P.cos_thK},
+                             {si * P.sin_phK, -si * P.cos_phK, 0.0},
+                             {-si * P.cos_thK * P.cos_phK, -si * P.cos_thK * P.sin_phK, si * P.sin_thK}};
+    const double *n = P.n_vec, *Sv = P.S_vec;
+    auto cross = [](const double *a, const double *b, double *c) {
+      c[0] = a[1] * b[2] - a[2] * b[1];
+      c[1] = a[2] * b[0] - a[0] * b[2];
+      c[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    double C[3][3], nxS[3];
+    for (int k = 0; k < 3; k++) cross(n, Lk[k], C[k]);
+    cross(n, Sv, nxS);
+    for (int k = 0; k < 3; k++) {
+      P.g_rc[k] = dot(C[k], nxS);
+      P.g_rs[k] = dot(Lk[k], nxS) - dot(Sv, C[k]);
+    }
+    P.g_nxL2[0] = dot(C[0], C[0]);
+    P.g_nxL2[1] = dot(C[1], C[1]);
+    P.g_nxL2[2] = dot(C[2], C[2]);
+    P.g_nxL2[3] = 2. * dot(C[0], C[1]);
+    P.g_nxL2[4] = 2. * dot(C[0], C[2]);
+    P.g_nxL2[5] = 2. * dot(C[1], C[2]);
+    P.nxS2 = dot(nxS, nxS);
+  }
   P.zeta = par[0] / sp.D / kGpc;
   P.Mphys = sp.M;
   // Bessel evaluation of the mode sum: series for e below kSeriesMaxE

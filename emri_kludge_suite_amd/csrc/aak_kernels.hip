@@ -9,8 +9,11 @@
 //                  the same ulp, so the fold becomes an exact int64 prefix sum;
 //                  binade crossings, exact ties and values near zero are
 //                  replayed serially in FP64
-//   k_modesum      per-sample harmonic sum with Bessel amplitudes, Doppler
-//                  phases, LISA response and Planck taper (AAK.cc:409-547)
+//   k_modesum      harmonic sum with Bessel amplitudes, Doppler phases, LISA
+//                  response and Planck taper (AAK.cc:409-547): per 256-sample
+//                  block the amplitudes are Taylor-expanded in e
+//                  (k_modesum_setup); blocks where that does not apply are
+//                  summed per sample (k_modesum_direct)
 //   k_weighted_dot PSD-weighted inner products of the rFFT output
 //                  (AAK_manager.cu:352-446 semantics)
 //
@@ -130,6 +133,11 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   B.rate[0][i] = (Th - R) / T / M;
   B.rate[1][i] = phidot;
   B.rate[2][i] = (Ph - Th) / T / M;
+  // amplitude of the mode sum (AAK.cc:481), with OmegaPhi of the prograde case
+  // from the maps already evaluated here
+  const double Om = (Y > 0) ? Ph / T / M : PN::omega_phi(v, e, Y, S, M);
+  B.amp[i] = pow(Om * P.Mphys * kMsunSec, 2. / 3.) * P.zeta;
+  if (i == 0) B.nu0[0] = R / T / (2. * kPi * M);
   if (i > P.i0 && i <= P.guard_last && !(phidot > 0.)) atomicAdd(&B.flags[0], 1);
 }
 
@@ -247,22 +255,68 @@ __device__ inline double fold_incr(const double *rate, int i0, int dir, int s, d
 
 __device__ inline int out_index(const FoldArgs &A, int s) { return A.i0 + A.dir * (s + 1); }
 
-// block-wide scans over kFoldThreads lanes through LDS (Hillis-Steele)
+// Block-wide exclusive scans over kFoldThreads lanes: inclusive shuffle scan
+// inside each 64-lane wave, wave totals through LDS.  `sh` needs
+// kFoldThreads / 64 entries.  Integer scans are exact; the double scan is an
+// approximation consumer (the fold's binade prediction) whose error the
+// bound E covers.
+template <class T>
+__device__ inline T wave_incl_sum(T v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const T u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
 template <class T>
 __device__ inline T block_excl_sum(T v, T *sh, T *total) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const T incl = wave_incl_sum(v);
+  if (lane == 63) sh[wave] = incl;
   __syncthreads();
-  for (int off = 1; off < kFoldThreads; off <<= 1) {
-    const T add = (tid >= off) ? sh[tid - off] : T(0);
-    __syncthreads();
-    sh[tid] += add;
-    __syncthreads();
+  T before = T(0), tot = T(0);
+#pragma unroll
+  for (int w = 0; w < kFoldThreads / 64; w++) {
+    const T x = sh[w];
+    if (w < wave) before += x;
+    tot += x;
   }
-  const T incl = sh[tid];
-  if (total) *total = sh[kFoldThreads - 1];
+  if (total) *total = tot;
   __syncthreads();
-  return incl - v;
+  return before + incl - v;
+}
+
+// Segmented inclusive scan of (flag, value) pairs over the block, lanes in
+// order: a set flag starts a new segment.  Returns the inclusive value of the
+// lane's segment prefix; shv/shf need kFoldThreads / 64 entries.
+__device__ inline long long block_seg_incl(long long v, int f, long long *shv, int *shf) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long uv = __shfl_up(v, o, 64);
+    const int uf = __shfl_up(f, o, 64);
+    if (lane >= o && !f) {
+      v += uv;
+      f |= uf;
+    }
+  }
+  if (lane == 63) {
+    shv[wave] = v;
+    shf[wave] = f;
+  }
+  __syncthreads();
+  if (!f) {
+    // carry in from the earlier waves, back to the nearest segment start
+    for (int w = wave - 1; w >= 0; w--) {
+      v += shv[w];
+      if (shf[w]) break;
+    }
+  }
+  __syncthreads();
+  return v;
 }
 
 // approximate block sums and prefix ranges
@@ -439,24 +493,12 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
     }
     tail += r[j];
   }
-  shl[tid] = tail;
-  shf[tid] = has_head;
+  // carry into the lane: the segmented inclusive scan of the previous lane
+  const long long seg_incl = block_seg_incl(tail, has_head, shl, shf);
+  long long carry = __shfl_up(seg_incl, 1, 64);
+  if ((tid & 63) == 63) shl[kFoldThreads / 64 + (tid >> 6)] = seg_incl;  // for the next wave's lane 0
   __syncthreads();
-  for (int off = 1; off < kFoldThreads; off <<= 1) {
-    long long v = 0;
-    int f = 0;
-    if (tid >= off) {
-      v = shl[tid - off];
-      f = shf[tid - off];
-    }
-    __syncthreads();
-    if (tid >= off && !shf[tid]) {
-      shl[tid] += v;
-      shf[tid] |= f;
-    }
-    __syncthreads();
-  }
-  const long long carry = (tid > 0) ? shl[tid - 1] : 0;
+  if ((tid & 63) == 0) carry = (tid > 0) ? shl[kFoldThreads / 64 + (tid >> 6) - 1] : 0;
   __syncthreads();
   // item and tie numbering
   const int nh = __popc(head_mask);
@@ -917,23 +959,6 @@ __device__ inline void bessel_five(int n, double x, double *J) {
   }
 }
 
-__device__ inline double nrm2_scaled(const double *v) {
-  double scale = 0.0, ssq = 1.0;
-  for (int i = 0; i < 3; i++) {
-    const double x = v[i];
-    if (x != 0.0) {
-      const double ax = fabs(x);
-      if (scale < ax) {
-        ssq = 1.0 + ssq * (scale / ax) * (scale / ax);
-        scale = ax;
-      } else {
-        ssq += (ax / scale) * (ax / scale);
-      }
-    }
-  }
-  return scale * sqrt(ssq);
-}
-
 // Per-sample part of the mode sum (AAK.cc:425-481): everything that does not
 // depend on the harmonic, folded into the weights of a_n, b_n, c_n in h_I and
 // h_II, so that h = Amp * sum_n [cos(nPhi)(alpha_n W_a + gamma_n W_c) +
@@ -943,49 +968,69 @@ struct SampleGeom {
   double W[6];  // a, b, c weights of h_I, then of h_II
 };
 
+__device__ inline double affine_sc(const double *c, double sa, double ca) { return c[0] + c[1] * sa + c[2] * ca; }
+
+// The harmonic-independent quantities need only be accurate (they set
+// amplitudes and the Doppler term), except the orientation angle's argument
+// 2 (gamma + beta), which is rounded as in the reference; the sample time and
+// the dense arrays are read as they are.
 __device__ inline void sample_geometry(const DenseParams &P, const DenseBuffers &B, int i, SampleGeom &G) {
+#pragma clang fp contract(fast)
   const double ti = sample_time(P, i);
-  const double e = B.e[i], v = B.v[i], M = B.M[i], S = B.S[i];
+  const double e = B.e[i];
   const double gim = B.phase[0][i], alp = B.phase[2][i];
   G.ti = ti;
   G.e = e;
   G.Phi = B.phase[1][i];
   G.gimdot = B.rate[0][i];
-  if (i == 0) {
-    G.nu = PN::drdm(v, e, P.coslam, S) / PN::dtdm(v, e, P.coslam, S) / (2. * kPi * M);
-  } else {
-    G.nu = B.rate[1][i] / 2. / kPi;
-  }
-  const double coslam = P.coslam, sinlam = P.sinlam;
+  G.nu = (i == 0) ? B.nu0[0] : B.rate[1][i] * (0.5 / kPi);
+  const double coslam = P.coslam;
   const double cosqS = P.cosqS, sinqS = P.sinqS, cosqK = P.cosqK, sinqK = P.sinqK;
-  const double cosphiK = P.cosphiK, sinphiK = P.sinphiK, halfsqrt3 = P.halfsqrt3;
-  const double cosalp = cos(alp), sinalp = sin(alp);
-  const double cosqL = cosqK * coslam + sinqK * sinlam * cosalp;
+  const double halfsqrt3 = P.halfsqrt3;
+  double sa, ca;
+  sincos(alp, &sa, &ca);
+  // orientation of L (AAK.cc:425-433)
+  const double cosqL = affine_sc(P.g_cosqL, sa, ca);
   const double sinqL = sqrt(1. - cosqL * cosqL);
-  const double phiLup = sinqK * sinphiK * coslam - cosphiK * sinlam * sinalp - cosqK * sinphiK * sinlam * cosalp;
-  const double phiLdown = sinqK * cosphiK * coslam + sinphiK * sinlam * sinalp - cosqK * cosphiK * sinlam * cosalp;
-  const double phiL = atan2(phiLup, phiLdown);
-  const double Ldotn = cosqL * cosqS + sinqL * sinqS * cos(phiL - P.phiS);
+  const double phiLup = affine_sc(P.g_phiLup, sa, ca), phiLdown = affine_sc(P.g_phiLdown, sa, ca);
+  // cos(phiL - phiS) with phiL = atan2(phiLup, phiLdown) (atan2(0, 0) = 0)
+  const double rL2 = phiLup * phiLup + phiLdown * phiLdown;
+  const double irL = rL2 > 0. ? rsqrt(rL2) : 0.;
+  const double cphiL = rL2 > 0. ? phiLdown * irL : 1., sphiL = phiLup * irL;
+  const double Ldotn = cosqL * cosqS + sinqL * sinqS * (cphiL * P.cos_phS + sphiL * P.sin_phS);
   const double Ldotn2 = Ldotn * Ldotn;
-  const double Sdotn = P.Sdotn;
-  const double betaup = -Sdotn + coslam * Ldotn;
-  const double betadown = sinqS * P.sin_dphiKS * sinlam * cosalp + (cosqK * Sdotn - cosqS) / sinqK * sinlam * sinalp;
+  const double betaup = -P.Sdotn + coslam * Ldotn;
+  const double betadown = affine_sc(P.g_betadown, sa, ca);
   const double beta = atan2(betaup, betadown);
-  const double gam = 2. * (gim + beta);
-  const double cos2gam = cos(gam), sin2gam = sin(gam);
+  // the reference's rounded argument 2 (gamma + beta) of the orientation angle
+  const double gb = gim + beta;
+  const double gam = 2. * gb;
+  double sin2gam, cos2gam;
+  sincos(gam, &sin2gam, &cos2gam);
   double cosorbphs = 0.0, FplusI, FcrosI, FplusII, FcrosII, h3;
   if (P.mich) {
+    // LISA orbit phase; the shifted angles by angle subtraction from one sincos
     const double orbphs = 2. * kPi * ti / kYear;
-    cosorbphs = cos(orbphs - P.phiS);
-    const double sinorbphs = sin(orbphs - P.phiS);
+    double so, co;
+    sincos(orbphs, &so, &co);
+    cosorbphs = co * P.cos_phS + so * P.sin_phS;             // cos(orbphs - phiS)
+    const double sinorbphs = so * P.cos_phS - co * P.sin_phS;  // sin(orbphs - phiS)
+    const double cosorbK = co * P.cos_phK + so * P.sin_phK;    // cos(orbphs - phiK)
     const double cosq = .5 * cosqS - halfsqrt3 * sinqS * cosorbphs;
-    const double phiw = orbphs + atan2(halfsqrt3 * cosqS + .5 * sinqS * cosorbphs, sinqS * sinorbphs);
-    const double psiup = .5 * cosqK - halfsqrt3 * sinqK * cos(orbphs - P.phiK) - cosq * P.psi_c0;
-    const double psidown = P.psi_c1 - halfsqrt3 * cos(orbphs) * P.psi_c2 - halfsqrt3 * sin(orbphs) * P.psi_c3;
-    const double psi = atan2(psiup, psidown);
+    // phiw = orbphs + atan2(Yw, Xw): double angles by (X^2 - Y^2, 2XY) / (X^2 + Y^2)
+    const double Yw = halfsqrt3 * cosqS + .5 * sinqS * cosorbphs, Xw = sinqS * sinorbphs;
+    const double rw = Xw * Xw + Yw * Yw;
+    const double irw = rw > 0. ? 1. / rw : 0.;
+    const double c2w = rw > 0. ? (Xw * Xw - Yw * Yw) * irw : 1., s2w = 2. * Xw * Yw * irw;
+    const double c2o = co * co - so * so, s2o = 2. * so * co;
+    const double cos2phi = c2o * c2w - s2o * s2w, sin2phi = s2o * c2w + c2o * s2w;
+    const double psiup = .5 * cosqK - halfsqrt3 * sinqK * cosorbK - cosq * P.psi_c0;
+    const double psidown = P.psi_c1 - halfsqrt3 * co * P.psi_c2 - halfsqrt3 * so * P.psi_c3;
+    const double rp = psiup * psiup + psidown * psidown;
+    const double irp = rp > 0. ? 1. / rp : 0.;
+    const double cos2psi = rp > 0. ? (psidown * psidown - psiup * psiup) * irp : 1.;
+    const double sin2psi = 2. * psidown * psiup * irp;
     const double cosq1 = .5 * (1. + cosq * cosq);
-    const double cos2phi = cos(2. * phiw), sin2phi = sin(2. * phiw);
-    const double cos2psi = cos(2. * psi), sin2psi = sin(2. * psi);
     FplusI = cosq1 * cos2phi * cos2psi - cosq * sin2phi * sin2psi;
     FcrosI = cosq1 * cos2phi * sin2psi + cosq * sin2phi * cos2psi;
     FplusII = cosq1 * sin2phi * cos2psi + cosq * cos2phi * sin2psi;
@@ -999,26 +1044,15 @@ __device__ inline void sample_geometry(const DenseParams &P, const DenseBuffers 
     h3 = 1.0;
   }
   G.cosorbphs = cosorbphs;
-  G.Amp = pow(PN::omega_phi(v, e, coslam, S, M) * P.Mphys * kMsunSec, 2. / 3.) * P.zeta;
-  // rotation to the wave frame (KSParMap.cc:532-574); n-independent
-  double L[3], nxL[3], nxS[3];
-  const double ca = cosalp, sa = sinalp;
-  L[0] = P.cosiota_L * P.sin_thK * P.cos_phK + P.siniota_L * (sa * P.sin_phK - ca * P.cos_thK * P.cos_phK);
-  L[1] = P.cosiota_L * P.sin_thK * P.sin_phK - P.siniota_L * (sa * P.cos_phK + ca * P.cos_thK * P.sin_phK);
-  L[2] = P.cosiota_L * P.cos_thK + P.siniota_L * ca * P.sin_thK;
-  const double *n = P.n_vec, *Sv = P.S_vec;
-  nxL[0] = n[1] * L[2] - n[2] * L[1];
-  nxL[1] = n[2] * L[0] - n[0] * L[2];
-  nxL[2] = n[0] * L[1] - n[1] * L[0];
-  nxS[0] = n[1] * Sv[2] - n[2] * Sv[1];
-  nxS[1] = n[2] * Sv[0] - n[0] * Sv[2];
-  nxS[2] = n[0] * Sv[1] - n[1] * Sv[0];
-  const double norm = nrm2_scaled(nxL) * nrm2_scaled(nxS);
-  const double cosrot = (0.0 + nxL[0] * nxS[0] + nxL[1] * nxS[1] + nxL[2] * nxS[2]) / norm;
-  double sinrot = 0.0 + L[0] * nxS[0] + L[1] * nxS[1] + L[2] * nxS[2];
-  sinrot -= 0.0 + Sv[0] * nxL[0] + Sv[1] * nxL[1] + Sv[2] * nxL[2];
-  sinrot /= norm;
-  const double rot0 = 2. * cosrot * cosrot - 1., rot1 = cosrot * sinrot;
+  G.Amp = B.amp[i];
+  // rotation to the wave frame (KSParMap.cc:532-574): cos(rot) = (nxL).(nxS) / N,
+  // sin(rot) = (L.(nxS) - S.(nxL)) / N, N = |nxL| |nxS|, all affine or quadratic
+  // in (sin alpha, cos alpha)
+  const double cn = affine_sc(P.g_rc, sa, ca), sn = affine_sc(P.g_rs, sa, ca);
+  const double *q2 = P.g_nxL2;
+  const double nxL2 = q2[0] + q2[1] * (sa * sa) + q2[2] * (ca * ca) + q2[3] * sa + q2[4] * ca + q2[5] * (sa * ca);
+  const double iN2 = 1. / (nxL2 * P.nxS2);
+  const double rot0 = 2. * (cn * cn) * iN2 - 1., rot1 = cn * sn * iN2;
   // h = Ap0 X + Ac0 Y per channel, with Ap0, Ac0 linear in (a, b, c)
   const double XI = h3 * (FplusI * rot0 - FcrosI * rot1), YI = h3 * (FplusI * rot1 + FcrosI * rot0);
   const double XII = h3 * (FplusII * rot0 - FcrosII * rot1), YII = h3 * (FplusII * rot1 + FcrosII * rot0);
@@ -1076,208 +1110,307 @@ __device__ inline void modesum_direct(const DenseParams &P, const DenseBuffers &
 // ------------------------------------------------------------------ Taylor form
 // Inside one block of samples e varies by a few 1e-5: the amplitude factors
 // alpha_n, beta_n, gamma_n are expanded in delta = e - e_ref once per block
-// (taylor_amplitudes) and evaluated per sample by Horner.  The order p is the
-// smallest with remainder bound 7 n (n delta)^{p+1}/(p+1)! <= 1e-17.
+// (k_modesum_setup) and evaluated per sample by Horner (k_modesum).  The order
+// p is the smallest with remainder bound 7 n (n delta)^{p+1}/(p+1)! <= 1e-17.
+// Blocks where no order up to kTaylorMax qualifies, or where the Bessel series
+// is not used (e > kSeriesMaxE, more than kTaylorModes harmonics), are summed
+// per sample by k_modesum_direct.
 constexpr int kMsThreads = 256;
 constexpr int kTaylorModes = 128;
+constexpr int kSetupThreads = 64;
+// doubles reserved per block in B.ms_coef per harmonic (orders up to kTaylorMax)
+constexpr int kCoefStride = 3 * (kTaylorMax + 1);
 
-// MIXED: the harmonic loop (rotation recurrence, amplitude polynomials,
-// response accumulation) runs in FP32 from FP64-reduced phase factors and
-// FP64 block coefficients; phases, their reduction mod 2 pi, the block
-// expansions and the final scaling stay FP64.
-// Block setup of the Taylor form, one workgroup per mode-sum block: the range
-// of e over the block's signal samples, the order p and the expansion of
-// alpha_n, beta_n, gamma_n for every harmonic (one thread each), written to
-// B.ms_coef[block][n][3][kTaylorMax+1]; B.ms_p[block] = p (-1: per-sample path).
-constexpr int kSetupThreads = 128;
+// Uniform, read-only data through the scalar path: the coefficients of a block
+// are the same for all its lanes, so they are fetched with scalar loads into
+// SGPRs and feed the FMAs directly.
+typedef const __attribute__((address_space(4))) double *cdptr;
+typedef const __attribute__((address_space(4))) float *cfptr;
+typedef const __attribute__((address_space(4))) int *ciptr;
 
+// B.ms_p[block]: the Taylor order, or -1 for the per-sample path.
+__device__ inline int block_order(const DenseBuffers &B, int blk) { return ((ciptr)B.ms_p)[blk]; }
+
+template <int PO, bool MIXED>
+__device__ void setup_harmonics(const DenseParams &P, const DenseBuffers &B, int blk, double eref, const double *sq,
+                                double *sJ) {
+  constexpr int NC = 3 * (PO + 1);
+  for (int n = threadIdx.x + 1; n <= P.nmodes; n += kSetupThreads) {
+    // series length: the bound for orders n+1, n+2 covers the higher orders
+    bessel_run<kSetupThreads>(n, PO, eref, B.bessel_c, B.bessel_lgf, P.bterms[n], sJ);
+    double coef[NC];
+    taylor_amplitudes_p<PO, kSetupThreads>(n, eref, sq, sJ, coef);
+    const size_t base = (size_t)blk * P.nmodes * kCoefStride;
+    if (MIXED) {
+      float *dst = reinterpret_cast<float *>(B.ms_coef + base) + (size_t)(n - 1) * NC;
+      for (int q = 0; q < NC; q++) dst[q] = (float)coef[q];
+    } else {
+      double *dst = B.ms_coef + base + (size_t)(n - 1) * NC;
+      for (int q = 0; q < NC; q++) dst[q] = coef[q];
+    }
+  }
+}
+
+// Block setup, one wave per mode-sum block: the range of e over the block's
+// signal samples, the order p and e_ref (every lane derives the same values
+// from a butterfly reduction), then one harmonic per lane.  The Bessel run of
+// a lane is staged in LDS (dynamic orders) and the expansion is formed in
+// registers.  A block with no signal sample gets p = 0 and no coefficients.
+template <bool MIXED>
 __global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, DenseBuffers B) {
-  __shared__ double s_sq[kTaylorMax + 1];
-  __shared__ double s_min[kSetupThreads / 64], s_max[kSetupThreads / 64];
-  __shared__ double s_eref;
-  __shared__ int s_p;
-  const int tid = threadIdx.x, blk = blockIdx.x;
+  __shared__ double s_J[kTaylorOrders][kSetupThreads];
+  const int lane = threadIdx.x, blk = blockIdx.x;
   const int n_end = min(P.N, P.ip + P.ib);
-  double emin = 1e300, emax = -1e300;
-  for (int q = tid; q < kMsThreads; q += kSetupThreads) {
+  double lo = 1e300, hi = -1e300;
+  for (int q = lane; q < kMsThreads; q += kSetupThreads) {
     const int i = blk * kMsThreads + q;
     if (i < n_end) {
       const double e = B.e[i];
-      emin = fmin(emin, e);
-      emax = fmax(emax, e);
+      lo = fmin(lo, e);
+      hi = fmax(hi, e);
     }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    emin = fmin(emin, __shfl_xor(emin, o));
-    emax = fmax(emax, __shfl_xor(emax, o));
+  for (int o = 1; o < 64; o <<= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o));
+    hi = fmax(hi, __shfl_xor(hi, o));
   }
-  if ((tid & 63) == 0) {
-    s_min[tid >> 6] = emin;
-    s_max[tid >> 6] = emax;
+  int p = -1;
+  double eref = 0.0;
+  const bool any = lo <= hi;
+  if (!any) {
+    p = 0;
+  } else if (P.bessel_series && P.nmodes <= kTaylorModes && hi <= kSeriesMaxE) {
+    eref = 0.5 * (lo + hi);
+    p = taylor_order(P.nmodes, 0.5 * (hi - lo));
   }
-  __syncthreads();
-  if (tid == 0) {
-    double lo = s_min[0], hi = s_max[0];
-    for (int w = 1; w < kSetupThreads / 64; w++) {
-      lo = fmin(lo, s_min[w]);
-      hi = fmax(hi, s_max[w]);
-    }
-    int p = -1;
-    double eref = 0.0;
-    if (lo <= hi && P.bessel_series && P.nmodes <= kTaylorModes && hi <= kSeriesMaxE) {
-      eref = 0.5 * (lo + hi);
-      p = taylor_order(P.nmodes, 0.5 * (hi - lo));
-      if (p >= 0) {
-        double sq[kTaylorMax + 1];
-        taylor_sqrt1me2(eref, p, sq);
-        for (int j = 0; j <= kTaylorMax; j++) s_sq[j] = j <= p ? sq[j] : 0.0;
-      }
-    }
-    s_eref = eref;
-    s_p = p;
+  if (lane == 0) {
     B.ms_p[blk] = p;
     B.ms_eref[blk] = eref;
   }
-  __syncthreads();
-  const int p = s_p;
-  if (p >= 0 && tid < P.nmodes) {
-    // series length: the host bound for orders n+1, n+2 covers the higher orders
-    double coef[3 * (kTaylorMax + 1)];
-    taylor_amplitudes(tid + 1, p, s_eref, s_sq, B.bessel_c, B.bessel_lgf, P.bterms[tid + 1], coef);
-    double *dst = B.ms_coef + ((size_t)blk * P.nmodes + tid) * (3 * (kTaylorMax + 1));
-    for (int q = 0; q < 3 * (kTaylorMax + 1); q++) dst[q] = coef[q];
+  if (p < 0 || !any) return;
+  double sq[kTaylorMax + 1];
+  taylor_sqrt1me2(eref, p, sq);
+  double *sJ = &s_J[0][lane];
+  switch (p) {
+    case 0: setup_harmonics<0, MIXED>(P, B, blk, eref, sq, sJ); break;
+    case 1: setup_harmonics<1, MIXED>(P, B, blk, eref, sq, sJ); break;
+    case 2: setup_harmonics<2, MIXED>(P, B, blk, eref, sq, sJ); break;
+    case 3: setup_harmonics<3, MIXED>(P, B, blk, eref, sq, sJ); break;
+    case 4: setup_harmonics<4, MIXED>(P, B, blk, eref, sq, sJ); break;
+    case 5: setup_harmonics<5, MIXED>(P, B, blk, eref, sq, sJ); break;
+    case 6: setup_harmonics<6, MIXED>(P, B, blk, eref, sq, sJ); break;
+    case 7: setup_harmonics<7, MIXED>(P, B, blk, eref, sq, sJ); break;
+    default: setup_harmonics<8, MIXED>(P, B, blk, eref, sq, sJ); break;
   }
 }
 
 int modesum_blocks(const DenseParams &P) { return (P.N + 2 + kMsThreads - 1) / kMsThreads; }
 
-// MIXED: the harmonic loop (rotation recurrence, amplitude polynomials,
-// response accumulation) runs in FP32 from FP64-reduced phase factors and
-// FP64 block coefficients; phases, their reduction mod 2 pi, the block
-// expansions and the final scaling stay FP64.
-template <bool MIXED>
-__global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuffers B) {
-  __shared__ double s_coef[MIXED ? 1 : kTaylorModes][3][kTaylorMax + 1];
-  __shared__ float s_coef_f[MIXED ? kTaylorModes : 1][3][kTaylorMax + 1];
-  const int tid = threadIdx.x;
-  const int i = blockIdx.x * kMsThreads + tid;
-  double *hI = B.h, *hII = B.h + (P.N + 2);
-  const int n_end = min(P.N, P.ip + P.ib);  // samples [0, n_end) carry the signal
-  const bool active = i < n_end;
-  const int p = B.ms_p[blockIdx.x];
-  const double eref = B.ms_eref[blockIdx.x];
-  if (p >= 0) {
-    const int nc = P.nmodes * 3 * (kTaylorMax + 1);
-    const double *src = B.ms_coef + (size_t)blockIdx.x * nc;
-    for (int q = tid; q < nc; q += kMsThreads) {
-      if (MIXED) (&s_coef_f[0][0][0])[q] = (float)src[q];
-      else (&s_coef[0][0][0])[q] = src[q];
-    }
+// Per-sample phase factors shared by both harmonic loops: u = exp(i (Phi + dA))
+// with Phi reduced mod 2 pi (Cody-Waite, three parts of 2 pi) and q0 = exp(i dB),
+// where the Doppler term of harmonic n is n dA + dB; q_n = q0 u^n by rotation.
+struct PhaseSeed {
+  double ur, ui, qr, qi, dA, dB;
+};
+
+__device__ inline void phase_seed(const DenseParams &P, const SampleGeom &G, PhaseSeed &S) {
+  const double kq = rint(G.Phi * (1.0 / (2.0 * kPi)));
+  double phr = fma(-kq, 6.283185307179586, G.Phi);
+  phr = fma(-kq, 2.4492935982947064e-16, phr);
+  phr = fma(-kq, -5.989539619436679e-33, phr);
+  S.dA = 0.0;
+  S.dB = 0.0;
+  if (P.mich) {
+    const double proj = kAUsec * P.sinqS * G.cosorbphs;
+    S.dA = 2. * kPi * G.nu * proj;
+    S.dB = 2. * G.gimdot * proj;
   }
-  __syncthreads();
-  if (i >= P.N + 2) return;
+  sincos(phr + S.dA, &S.ui, &S.ur);
+  sincos(S.dB, &S.qi, &S.qr);
+}
+
+// a_0 + sum_{j=1..PO} a_j d^j with the powers dp[j] = d^j shared by all
+// harmonics; the a_j come from SGPRs, so every term is one FMA with a scalar
+// operand (the highest power first).
+template <int PO, typename T, typename CP>
+__device__ inline T poly_pow(CP c, const T *dp) {
+  if (PO == 0) return c[0];
+  T x = c[PO] * dp[PO];
+#pragma unroll
+  for (int j = PO - 1; j >= 1; j--) x = fma(c[j], dp[j], x);
+  return x + c[0];
+}
+
+// FP64 harmonic loop at Taylor order PO.  The response factors are the same
+// for every harmonic, so only the three sums
+//   Sa = sum cos(s_n) alpha_n, Sg = sum cos(s_n) gamma_n, Sb = sum sin(s_n) beta_n
+// are accumulated.  The rotation gives exp(i theta), theta = n Phi + Doppler_n;
+// the reference's argument is its rounded s_n = rn(rn(n Phi) + Doppler_n), which
+// differs from theta by a few ulp of s_n (~1e-9 rad late in a year-long
+// inspiral).  E = theta - s_n = fma(n, Phi, -s_n) + Doppler_n, accurate to the
+// rounding of Doppler_n (~1e-14 rad), and exp(i s_n) = exp(i theta) (1 - i E)
+// to second order in E.
+template <int PO>
+__device__ inline void harmonics_fp64(int nmodes, cdptr cf, double Phi, double d, const PhaseSeed &S, double *sum) {
+  constexpr int NC = 3 * (PO + 1);
+  double dp[PO + 1];
+  dp[0] = 1.0;
+#pragma unroll
+  for (int j = 1; j <= PO; j++) dp[j] = dp[j - 1] * d;
+  double qr = S.qr, qi = S.qi, Sa = 0.0, Sb = 0.0, Sg = 0.0;
+  const double ur = S.ur, ui = S.ui;
+#pragma unroll 2
+  for (int nn = 1; nn <= nmodes; nn++, cf += NC) {
+    const double tr = fma(qr, ur, -(qi * ui));
+    qi = fma(qr, ui, qi * ur);
+    qr = tr;
+    const double dn = (double)nn;
+    const double D = fma(dn, S.dA, S.dB);
+    const double s_ = dn * Phi + D;  // not contracted: rn(rn(n Phi) + D)
+    const double E = fma(dn, Phi, -s_) + D;
+    const double cnp = fma(E, qi, qr), snp = fma(-E, qr, qi);
+    const double al = poly_pow<PO>(cf, dp);
+    const double be = poly_pow<PO>(cf + (PO + 1), dp);
+    const double ga = poly_pow<PO>(cf + 2 * (PO + 1), dp);
+    Sa = fma(cnp, al, Sa);
+    Sg = fma(cnp, ga, Sg);
+    Sb = fma(snp, be, Sb);
+  }
+  sum[0] = Sa;
+  sum[1] = Sb;
+  sum[2] = Sg;
+}
+
+// FP32 harmonic loop (mixed precision): rotation, amplitude polynomials and
+// the three sums in FP32 from FP64-reduced phase factors; no argument
+// correction.
+template <int PO>
+__device__ inline void harmonics_fp32(int nmodes, cfptr cf, float d, const PhaseSeed &S, double *sum) {
+  constexpr int NC = 3 * (PO + 1);
+  float dp[PO + 1];
+  dp[0] = 1.f;
+#pragma unroll
+  for (int j = 1; j <= PO; j++) dp[j] = dp[j - 1] * d;
+  float qr = (float)S.qr, qi = (float)S.qi, Sa = 0.f, Sb = 0.f, Sg = 0.f;
+  const float ur = (float)S.ur, ui = (float)S.ui;
+#pragma unroll 2
+  for (int nn = 1; nn <= nmodes; nn++, cf += NC) {
+    const float tr = fmaf(qr, ur, -(qi * ui));
+    qi = fmaf(qr, ui, qi * ur);
+    qr = tr;
+    const float al = poly_pow<PO>(cf, dp);
+    const float be = poly_pow<PO>(cf + (PO + 1), dp);
+    const float ga = poly_pow<PO>(cf + 2 * (PO + 1), dp);
+    Sa = fmaf(qr, al, Sa);
+    Sg = fmaf(qr, ga, Sg);
+    Sb = fmaf(qi, be, Sb);
+  }
+  sum[0] = Sa;
+  sum[1] = Sb;
+  sum[2] = Sg;
+}
+
+// Samples past the signal (or the two padding slots) get time and zeros.
+__device__ inline bool modesum_tail(const DenseParams &P, const DenseBuffers &B, int i, int n_end) {
+  double *hI = B.h, *hII = B.h + (P.N + 2);
   if (i >= P.N) {
     B.t[i] = 0.0;
     hI[i] = 0.0;
     hII[i] = 0.0;
-    return;
+    return true;
   }
-  if (!active) {
+  if (i >= n_end) {
     B.t[i] = sample_time(P, i);
     hI[i] = 0.0;
     hII[i] = 0.0;
-    return;
+    return true;
   }
+  return false;
+}
+
+// Taylor-form mode sum for blocks with B.ms_p >= 0.  MIXED: the harmonic loop
+// runs in FP32 (phases, their reduction mod 2 pi, the block expansions and the
+// final scaling stay FP64).
+template <bool MIXED>
+__global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuffers B) {
+  const int blk = blockIdx.x;
+  const int p = block_order(B, blk);
+  if (p < 0) return;  // k_modesum_direct sums this block
+  const int i = blk * kMsThreads + threadIdx.x;
+  if (i >= P.N + 2) return;
+  const int n_end = min(P.N, P.ip + P.ib);  // samples [0, n_end) carry the signal
+  if (modesum_tail(P, B, i, n_end)) return;
+  SampleGeom G;
+  sample_geometry(P, B, i, G);
+  B.t[i] = G.ti;
+  PhaseSeed S;
+  phase_seed(P, G, S);
+  const double d = G.e - B.ms_eref[blk];
+  const size_t base = (size_t)blk * P.nmodes * kCoefStride;
+  double sum[3];
+  if (MIXED) {
+    cfptr cf = (cfptr)(B.ms_coef + base);
+    const float fd = (float)d;
+    switch (p) {
+      case 0: harmonics_fp32<0>(P.nmodes, cf, fd, S, sum); break;
+      case 1: harmonics_fp32<1>(P.nmodes, cf, fd, S, sum); break;
+      case 2: harmonics_fp32<2>(P.nmodes, cf, fd, S, sum); break;
+      case 3: harmonics_fp32<3>(P.nmodes, cf, fd, S, sum); break;
+      case 4: harmonics_fp32<4>(P.nmodes, cf, fd, S, sum); break;
+      case 5: harmonics_fp32<5>(P.nmodes, cf, fd, S, sum); break;
+      case 6: harmonics_fp32<6>(P.nmodes, cf, fd, S, sum); break;
+      case 7: harmonics_fp32<7>(P.nmodes, cf, fd, S, sum); break;
+      default: harmonics_fp32<8>(P.nmodes, cf, fd, S, sum); break;
+    }
+  } else {
+    cdptr cf = (cdptr)(B.ms_coef + base);
+    switch (p) {
+      case 0: harmonics_fp64<0>(P.nmodes, cf, G.Phi, d, S, sum); break;
+      case 1: harmonics_fp64<1>(P.nmodes, cf, G.Phi, d, S, sum); break;
+      case 2: harmonics_fp64<2>(P.nmodes, cf, G.Phi, d, S, sum); break;
+      case 3: harmonics_fp64<3>(P.nmodes, cf, G.Phi, d, S, sum); break;
+      case 4: harmonics_fp64<4>(P.nmodes, cf, G.Phi, d, S, sum); break;
+      case 5: harmonics_fp64<5>(P.nmodes, cf, G.Phi, d, S, sum); break;
+      case 6: harmonics_fp64<6>(P.nmodes, cf, G.Phi, d, S, sum); break;
+      case 7: harmonics_fp64<7>(P.nmodes, cf, G.Phi, d, S, sum); break;
+      default: harmonics_fp64<8>(P.nmodes, cf, G.Phi, d, S, sum); break;
+    }
+  }
+  const double accI = G.W[0] * sum[0] + G.W[1] * sum[1] + G.W[2] * sum[2];
+  const double accII = G.W[3] * sum[0] + G.W[4] * sum[1] + G.W[5] * sum[2];
+  const double sc = (i <= P.ip) ? G.Amp : G.Amp / taper(P, i, G.ti);
+  B.h[i] = sc * accI;
+  B.h[P.N + 2 + i] = sc * accII;
+}
+
+// Per-sample mode sum for blocks with B.ms_p < 0 (the others return at once).
+__global__ __launch_bounds__(kMsThreads) void k_modesum_direct(DenseParams P, DenseBuffers B) {
+  const int blk = blockIdx.x;
+  if (block_order(B, blk) >= 0) return;
+  const int i = blk * kMsThreads + threadIdx.x;
+  if (i >= P.N + 2) return;
+  const int n_end = min(P.N, P.ip + P.ib);
+  if (modesum_tail(P, B, i, n_end)) return;
   SampleGeom G;
   sample_geometry(P, B, i, G);
   B.t[i] = G.ti;
   double accI, accII;
-  if (p < 0) {
-    modesum_direct(P, B, G, &accI, &accII);
-  } else {
-    const double d = G.e - eref;
-    // exp(i n Phi) by rotation from Phi reduced mod 2 pi (Cody-Waite, three
-    // parts of 2 pi); the Doppler term is linear in n: Doppler_n = n dA + dB
-    const double kq = rint(G.Phi * (1.0 / (2.0 * kPi)));
-    double phr = fma(-kq, 6.283185307179586, G.Phi);
-    phr = fma(-kq, 2.4492935982947064e-16, phr);
-    phr = fma(-kq, -5.989539619436679e-33, phr);
-    const double gdpi = G.gimdot / kPi;
-    double dA = 0.0, dB = 0.0;
-    if (P.mich) {
-      const double proj = kAUsec * P.sinqS * G.cosorbphs;
-      dA = 2. * kPi * G.nu * proj;
-      dB = 2. * G.gimdot * proj;
-    }
-    double ur, ui, qr, qi;
-    sincos(phr + dA, &ui, &ur);
-    sincos(dB, &qi, &qr);
-    accI = 0.0;
-    accII = 0.0;
-    if (MIXED) {
-      float fur = (float)ur, fui = (float)ui, fqr = (float)qr, fqi = (float)qi;
-      const float fd = (float)d;
-      const float W0 = (float)G.W[0], W1 = (float)G.W[1], W2 = (float)G.W[2];
-      const float W3 = (float)G.W[3], W4 = (float)G.W[4], W5 = (float)G.W[5];
-      float fI = 0.f, fII = 0.f;
-      for (int nn = 1; nn <= P.nmodes; nn++) {
-        const float tr = fqr * fur - fqi * fui;
-        fqi = fqr * fui + fqi * fur;
-        fqr = tr;
-        const float *cf = &s_coef_f[nn - 1][0][0];
-        float al = cf[p], be = cf[kTaylorMax + 1 + p], ga = cf[2 * (kTaylorMax + 1) + p];
-        for (int j = p - 1; j >= 0; j--) {
-          al = fmaf(al, fd, cf[j]);
-          be = fmaf(be, fd, cf[kTaylorMax + 1 + j]);
-          ga = fmaf(ga, fd, cf[2 * (kTaylorMax + 1) + j]);
-        }
-        fI += fqr * (al * W0 + ga * W2) + fqi * (be * W1);
-        fII += fqr * (al * W3 + ga * W5) + fqi * (be * W4);
-      }
-      accI = (double)fI;
-      accII = (double)fII;
-    } else {
-    for (int nn = 1; nn <= P.nmodes; nn++) {
-      // q = exp(i (n (Phi + dA) + dB))
-      const double tr = qr * ur - qi * ui;
-      qi = qr * ui + qi * ur;
-      qr = tr;
-      // the reference's rounded argument s = rn(rn(n Phi) + Doppler) differs
-      // from n Phi + Doppler by -(e1 + e2), both exact: e1 = n Phi - rn(n Phi),
-      // e2 = (rn(n Phi) + Doppler) - s
-      const double dn = (double)nn;
-      const double pn = dn * G.Phi;
-      const double e1 = fma(dn, G.Phi, -pn);
-      double s_ = pn, e2 = 0.0;
-      if (P.mich) {
-        const double fn = dn * G.nu + gdpi;
-        const double Doppler = 2. * kPi * fn * kAUsec * P.sinqS * G.cosorbphs;
-        s_ = pn + Doppler;
-        const double bb = s_ - pn;
-        e2 = (pn - (s_ - bb)) + (Doppler - bb);
-      }
-      const double del = -(e1 + e2);
-      const double cnp = qr - del * qi, snp = qi + del * qr;
-      double abg[3];
-      taylor_eval(&s_coef[nn - 1][0][0], p, d, abg);
-      const double al = abg[0], be = abg[1], ga = abg[2];
-      accI += cnp * (al * G.W[0] + ga * G.W[2]) + snp * (be * G.W[1]);
-      accII += cnp * (al * G.W[3] + ga * G.W[5]) + snp * (be * G.W[4]);
-    }
-    }
-  }
-  const double den = taper(P, i, G.ti);
-  hI[i] = G.Amp * accI / den;
-  hII[i] = G.Amp * accII / den;
+  modesum_direct(P, B, G, &accI, &accII);
+  const double sc = (i <= P.ip) ? G.Amp : G.Amp / taper(P, i, G.ti);
+  B.h[i] = sc * accI;
+  B.h[P.N + 2 + i] = sc * accII;
 }
 
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
-  const int n = P.N + 2;
-  hipLaunchKernelGGL(k_modesum_setup, dim3(modesum_blocks(P)), dim3(kSetupThreads), 0, s, P, B);
-  if (P.mixed)
-    hipLaunchKernelGGL(k_modesum<true>, dim3((n + kMsThreads - 1) / kMsThreads), dim3(kMsThreads), 0, s, P, B);
-  else
-    hipLaunchKernelGGL(k_modesum<false>, dim3((n + kMsThreads - 1) / kMsThreads), dim3(kMsThreads), 0, s, P, B);
+  const int nb = modesum_blocks(P);
+  if (P.mixed) {
+    hipLaunchKernelGGL(k_modesum_setup<true>, dim3(nb), dim3(kSetupThreads), 0, s, P, B);
+    hipLaunchKernelGGL(k_modesum<true>, dim3(nb), dim3(kMsThreads), 0, s, P, B);
+  } else {
+    hipLaunchKernelGGL(k_modesum_setup<false>, dim3(nb), dim3(kSetupThreads), 0, s, P, B);
+    hipLaunchKernelGGL(k_modesum<false>, dim3(nb), dim3(kMsThreads), 0, s, P, B);
+  }
+  hipLaunchKernelGGL(k_modesum_direct, dim3(nb), dim3(kMsThreads), 0, s, P, B);
 }
 
 // ------------------------------------------------------------------ likelihood
