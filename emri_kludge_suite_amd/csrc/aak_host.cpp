@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 
 #include "aak_bessel.hpp"
@@ -128,6 +129,13 @@ static double pn_p(double v, double e, double coslam, double S, double M, double
   return (1. - e * e) * pow(PN::omega_phi(v, e, coslam, S, M) * Mphys * kMsunSec, -2. / 3.);
 }
 
+static bool same_bits(double a, double b) {
+  uint64_t x, y;
+  std::memcpy(&x, &a, sizeof x);
+  std::memcpy(&y, &b, sizeof y);
+  return x == y;
+}
+
 int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double S_phys, const SparseTrajectory &tr,
                 KnotSet *ks, std::string *err) {
   const double timestep = cfg.dt;
@@ -198,6 +206,17 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
     eA[j + 1] = eA[j] + (1.5 * edot - .5 * edotm) * dt_large;
     vA[j + 1] = vA[j] + (1.5 * vdot - .5 * vdotm) * dt_large;
     clamp_step(j);
+    // Past a plunge the clamp resets (e, v) to the same point at every step:
+    // once a step starts from the state it produces, with the rates it had
+    // before, every later step repeats it exactly (fixed step, pure rates), so
+    // the remaining entries are that state.
+    if (same_bits(eA[j + 1], eA[j]) && same_bits(vA[j + 1], vA[j]) && same_bits(edot, edotm) && same_bits(vdot, vdotm)) {
+      for (int k = j + 2; k <= j_end; k++) {
+        eA[k] = eA[j + 1];
+        vA[k] = vA[j + 1];
+      }
+      break;
+    }
   }
   j_max_temp = j_max;
   for (int j = j0; j <= j_end; j++) {
@@ -305,75 +324,87 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
 // Natural cubic spline: symmetric tridiagonal system for c = y''/2 solved by
 // LDL^T (the scheme of GSL's cspline, which the reference links).  The
 // factorisation depends on the abscissae only, so the four arrays of one
-// waveform share it; each array's arithmetic is unchanged.
-int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs) {
+// waveform share it; each array's arithmetic is unchanged.  One forward pass
+// runs the factorisation, the four forward substitutions and the diagonal
+// scaling side by side (independent dependency chains), reusing each slope
+// (y[i+1] - y[i]) / h_i for the two right-hand sides that contain it; the
+// backward pass follows.  y_out, if given, receives a copy of the four arrays.
+int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs, double *const *y_out) {
   if (n < 3) return -1;
   const int sys = n - 2;
-  thread_local std::vector<double> od, dg, gi_, al, gam;
-  for (std::vector<double> *a : {&od, &dg, &gi_, &al, &gam})
-    if ((int)a->size() < sys + 1) a->resize(sys + 1);
-  for (int i = 0; i < sys; i++) {
-    const double hi = x[i + 1] - x[i], hip1 = x[i + 2] - x[i + 1];
-    od[i] = hip1;
-    dg[i] = 2.0 * (hip1 + hi);
-    gi_[i] = (hi != 0.0) ? 1.0 / hi : 0.0;
-  }
-  gi_[sys] = (x[sys + 1] - x[sys] != 0.0) ? 1.0 / (x[sys + 1] - x[sys]) : 0.0;
-  if (sys > 1) {
-    al[0] = dg[0];
-    gam[0] = od[0] / al[0];
-    for (int i = 1; i < sys - 1; i++) {
-      al[i] = dg[i] - od[i - 1] * gam[i - 1];
-      gam[i] = od[i] / al[i];
-    }
-    al[sys - 1] = dg[sys - 1] - od[sys - 2] * gam[sys - 2];
-  }
-  // right-hand side g_i = 3 (dy_{i+1}/h_{i+1} - dy_i/h_i) of array a
-  auto g = [&](int a, int i) {
-    const double *y = ys[a];
-    return 3.0 * ((y[i + 2] - y[i + 1]) * gi_[i + 1] - (y[i + 1] - y[i]) * gi_[i]);
-  };
+  thread_local std::vector<double> gam_v;
+  if ((int)gam_v.size() < sys) gam_v.resize(sys);
+  double *gam = gam_v.data();
+  const double *y0 = ys[0], *y1 = ys[1], *y2 = ys[2], *y3 = ys[3];
+  double *c0 = cs[0], *c1 = cs[1], *c2 = cs[2], *c3 = cs[3];
   for (int a = 0; a < 4; a++) {
     cs[a][0] = 0.0;
     cs[a][n - 1] = 0.0;
   }
-  if (sys == 1) {
-    for (int a = 0; a < 4; a++) cs[a][1] = g(a, 0) / dg[0];
-    return 0;
-  }
-  // the four arrays' recurrences run side by side (independent chains); each
-  // array's operations and their order are those of the single-array solve.
-  // z of array a is kept in c[a][1..sys] until the back substitution.
-  double z0 = g(0, 0), z1 = g(1, 0), z2 = g(2, 0), z3 = g(3, 0);
-  cs[0][1] = z0;
-  cs[1][1] = z1;
-  cs[2][1] = z2;
-  cs[3][1] = z3;
-  for (int i = 1; i < sys; i++) {
-    const double gm = gam[i - 1];
-    z0 = g(0, i) - gm * z0;
-    z1 = g(1, i) - gm * z1;
-    z2 = g(2, i) - gm * z2;
-    z3 = g(3, i) - gm * z3;
-    cs[0][1 + i] = z0;
-    cs[1][1 + i] = z1;
-    cs[2][1 + i] = z2;
-    cs[3][1 + i] = z3;
-  }
+  if (y_out)
+    for (int a = 0; a < 4; a++) {
+      y_out[a][0] = ys[a][0];
+      y_out[a][1] = ys[a][1];
+    }
+  // h_i = x[i+1] - x[i], gi_i = 1/h_i (0 for h_i = 0); the right-hand side of
+  // row i is g_i = 3 ((y[i+2] - y[i+1]) gi_{i+1} - (y[i+1] - y[i]) gi_i)
+  double h0 = x[1] - x[0];
+  double gi0 = (h0 != 0.0) ? 1.0 / h0 : 0.0;
+  double s0 = (y0[1] - y0[0]) * gi0, s1 = (y1[1] - y1[0]) * gi0;
+  double s2 = (y2[1] - y2[0]) * gi0, s3 = (y3[1] - y3[0]) * gi0;
+  double z0 = 0.0, z1 = 0.0, z2 = 0.0, z3 = 0.0, od_prev = 0.0, gam_prev = 0.0;
   for (int i = 0; i < sys; i++) {
-    const double a_ = al[i];
-    cs[0][1 + i] = cs[0][1 + i] / a_;
-    cs[1][1 + i] = cs[1][1 + i] / a_;
-    cs[2][1 + i] = cs[2][1 + i] / a_;
-    cs[3][1 + i] = cs[3][1 + i] / a_;
+    const double h1 = x[i + 2] - x[i + 1];
+    const double gi1 = (h1 != 0.0) ? 1.0 / h1 : 0.0;
+    const double od = h1, dg = 2.0 * (h1 + h0);
+    const double al = (i == 0) ? dg : dg - od_prev * gam_prev;
+    const double t0 = (y0[i + 2] - y0[i + 1]) * gi1, t1 = (y1[i + 2] - y1[i + 1]) * gi1;
+    const double t2 = (y2[i + 2] - y2[i + 1]) * gi1, t3 = (y3[i + 2] - y3[i + 1]) * gi1;
+    const double g0 = 3.0 * (t0 - s0), g1 = 3.0 * (t1 - s1), g2 = 3.0 * (t2 - s2), g3 = 3.0 * (t3 - s3);
+    if (i == 0) {
+      z0 = g0;
+      z1 = g1;
+      z2 = g2;
+      z3 = g3;
+    } else {
+      z0 = g0 - gam_prev * z0;
+      z1 = g1 - gam_prev * z1;
+      z2 = g2 - gam_prev * z2;
+      z3 = g3 - gam_prev * z3;
+    }
+    c0[1 + i] = z0 / al;
+    c1[1 + i] = z1 / al;
+    c2[1 + i] = z2 / al;
+    c3[1 + i] = z3 / al;
+    if (y_out) {
+      y_out[0][i + 2] = y0[i + 2];
+      y_out[1][i + 2] = y1[i + 2];
+      y_out[2][i + 2] = y2[i + 2];
+      y_out[3][i + 2] = y3[i + 2];
+    }
+    if (i < sys - 1) {
+      gam_prev = od / al;
+      gam[i] = gam_prev;
+    }
+    od_prev = od;
+    h0 = h1;
+    s0 = t0;
+    s1 = t1;
+    s2 = t2;
+    s3 = t3;
   }
-  // c[sys] = z[sys-1] already in place; c[1+i] = z[i] - gam[i] c[i+2]
+  // c[1+i] = z_i / al_i - gam_i c[i+2]
+  double n0 = c0[sys], n1 = c1[sys], n2 = c2[sys], n3 = c3[sys];
   for (int i = sys - 2; i >= 0; i--) {
     const double gm = gam[i];
-    cs[0][1 + i] = cs[0][1 + i] - gm * cs[0][i + 2];
-    cs[1][1 + i] = cs[1][1 + i] - gm * cs[1][i + 2];
-    cs[2][1 + i] = cs[2][1 + i] - gm * cs[2][i + 2];
-    cs[3][1 + i] = cs[3][1 + i] - gm * cs[3][i + 2];
+    n0 = c0[1 + i] - gm * n0;
+    n1 = c1[1 + i] - gm * n1;
+    n2 = c2[1 + i] - gm * n2;
+    n3 = c3[1 + i] - gm * n3;
+    c0[1 + i] = n0;
+    c1[1 + i] = n1;
+    c2[1 + i] = n2;
+    c3[1 + i] = n3;
   }
   return 0;
 }
@@ -391,12 +422,13 @@ size_t table_size(const KnotSet &ks) { return (size_t)kTableArrays * ks.K + ks.s
 void fill_table(const KnotSet &ks, const SparseTrajectory &tr, double *tab, KnotGrid *grid) {
   const int K = ks.K;
   const double *ys[4] = {ks.ky(0), ks.ky(1), ks.ky(2), ks.ky(3)};
-  double *cs[4];
+  double *cs[4], *yo[4];
   for (int a = 0; a < 4; a++) {
-    std::memcpy(tab + (size_t)a * K, ys[a], sizeof(double) * K);
+    yo[a] = tab + (size_t)a * K;
     cs[a] = tab + (size_t)(4 + a) * K;
   }
-  spline_natural4(ks.kt(), ys, K, cs);
+  if (spline_natural4(ks.kt(), ys, K, cs, yo) != 0)
+    for (int a = 0; a < 4; a++) std::memcpy(yo[a], ys[a], sizeof(double) * K);
   std::memcpy(tab + (size_t)kTableArrays * K, tr.dt_map, sizeof(double) * ks.steps);
   grid->K = K;
   grid->head_off = ks.j0 - ks.j_start;

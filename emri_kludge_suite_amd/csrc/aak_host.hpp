@@ -78,7 +78,8 @@ int sparse_stage(const WaveConfig &cfg, const SourceParams &sp, SparseTrajectory
 int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double S_phys, const SparseTrajectory &tr,
                 KnotSet *ks, std::string *err);
 // Natural cubic splines of four arrays on one abscissa (one factorisation).
-int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs);
+int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs,
+                    double *const *y_out = nullptr);
 int spline_natural(const double *x, const double *y, int n, double *c);
 size_t table_size(const KnotSet &ks);
 void fill_table(const KnotSet &ks, const SparseTrajectory &tr, double *tab, KnotGrid *grid);

@@ -235,7 +235,7 @@ struct PN {
     return dphidm(v, e, -cosi, -s) / dtdm(v, e, -cosi, -s) / M;
   }
   // Radiation-reaction rates of v and e (KSParMap.cc:86-144).
-  AAK_HD static double dvdt_p(double v, double e, double Y, double m, double M, double q, double pw, double lv) {
+  AAK_HD __attribute__((always_inline)) static double dvdt_p(double v, double e, double Y, double m, double M, double q, double pw, double lv) {
     const double v2 = v * v, v3 = v2 * v, v6 = v3 * v3, e2 = e * e, e4 = e2 * e2, e6 = e4 * e2;
     const double Y2 = Y * Y, q2 = q * q, EG = kEulerGamma, P2 = kPi2Trunc, PI = kPi;
     return (32. * pw * m * v6 * v3 *
@@ -259,7 +259,7 @@ struct PN {
                    (107. * (256. + 1832. * e2 + 1308. * e4 + 69. * e6) * lv) / 1680.))) /
            (5. * M * M);
   }
-  AAK_HD static double dedt_p(double v, double e, double Y, double m, double M, double q, double pw, double lv) {
+  AAK_HD __attribute__((always_inline)) static double dedt_p(double v, double e, double Y, double m, double M, double q, double pw, double lv) {
     const double v2 = v * v, v3 = v2 * v, v6 = v3 * v3, e2 = e * e, e4 = e2 * e2;
     const double Y2 = Y * Y, q2 = q * q, EG = kEulerGamma, P2 = kPi2Trunc, PI = kPi;
     return -(e * pw * m * v6 * v2 *
@@ -289,7 +289,7 @@ struct PN {
   AAK_HD static double dedt(double v, double e, double Y, double m, double M, double q) {
     return dedt_p(v, e, Y, m, M, q, pow(1. - e * e, 1.5), log(v));
   }
-  AAK_HD static void ak_rates(double v, double e, double Y, double m, double M, double q, double *edot, double *vdot) {
+  AAK_HD __attribute__((always_inline)) static void ak_rates(double v, double e, double Y, double m, double M, double q, double *edot, double *vdot) {
     const double pw = pow(1. - e * e, 1.5), lv = log(v);
     *edot = dedt_p(v, e, Y, m, M, q, pw, lv);
     *vdot = dvdt_p(v, e, Y, m, M, q, pw, lv);

@@ -93,6 +93,24 @@ def test_production_prior_host_stage_bit_identical(L):
         done += 1
 
 
+def test_heavy_plunging_host_stage_bit_identical(L):
+    # sources that plunge early in a long knot grid: past the plunge the AK
+    # evolution repeats one clamped state, which the product fills in without
+    # re-evaluating; the knots must still equal the oracle's full evolution
+    rng = np.random.default_rng(3)
+    done = 0
+    for _ in range(400):
+        pars = _draw(rng, production=True)
+        K, traj, kn, info, dp = _product_plan(L, pars, 2 ** 21, 15.0, False, 4_000_000)
+        if K < 100_000 or info[9] != 1:
+            continue
+        _assert_same_plan(L, pars, 2 ** 21, 15.0)
+        done += 1
+        if done == 3:
+            break
+    assert done == 3
+
+
 def test_plunging_backint_host_stage_bit_identical(L):
     pars = [0.5, 0.9, 7.2, 0.3, 3e5, 30.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
     K, info = _assert_same_plan(L, pars, 2 ** 22, 8.0, backint=True)
