@@ -1,16 +1,19 @@
 // aak_capi.cpp -- the C ABI (include/gpuAAK.h, include/aak_testing.h).
 //
 // One handle owns:
-//   * a compute stream and a copy stream (default priority),
-//   * the device buffers of one waveform (dense arrays, phases, h, t, fold
-//     and guard work, Bessel tables) and two device knot tables, so the
-//     upload of the next waveform's table overlaps the current one's kernels,
-//   * the rocFFT real-forward plan (both channels in one batched call) and
-//     the weighted-data buffers,
+//   * a copy stream and one or more lanes (default priority streams), each
+//     with the device buffers of one waveform in flight (dense arrays, phases,
+//     h, t, fold and guard work, mode-sum coefficients, spectrum) and two
+//     device knot tables, so the upload of the next waveform's table overlaps
+//     the current one's kernels; lane 0 serves the single-waveform calls,
+//     batches spread their templates over AAK_LANES lanes (default 2) so that
+//     one template's narrow kernels overlap another's wide ones,
+//   * a rocFFT real-forward plan per lane (both channels in one batched call),
+//     the weighted-data buffers and the Bessel tables,
 //   * for batches, a pool of host threads and a ring of plan slots whose knot
 //     tables live in pinned host memory.
 //
-// Per waveform the compute stream runs, with no host round trip:
+// Per waveform a lane's stream runs, with no host round trip:
 //   clear counters -> (wait upload) rates -> Phidot guard -> exact folds ->
 //   fill -> mode sum [-> rFFT -> weighted reduction].
 // gpu_gen_AAK / Likelihood keep the reference's one-waveform-per-call
@@ -107,11 +110,13 @@ struct RingSlot {
 
 }  // namespace
 
-struct aak_handle_s {
-  WaveConfig cfg;
-  int N = 0, nbins = 0;
-  hipStream_t stream = nullptr, cstream = nullptr;
-  // one waveform's device buffers
+// Device state of one waveform in flight: a compute stream with its own dense
+// arrays, fold work, mode-sum coefficients, time-domain and spectrum buffers,
+// two knot-table slots and a rocFFT execution context.  Batches spread their
+// templates over several lanes so that one template's serial or narrow
+// kernels (the fold chain, small reductions) overlap another's wide ones.
+struct Lane {
+  hipStream_t stream = nullptr;
   double *d_table[2] = {nullptr, nullptr};
   size_t table_cap[2] = {0, 0};
   hipEvent_t ev_upload[2] = {nullptr, nullptr}, ev_rates[2] = {nullptr, nullptr};
@@ -121,28 +126,38 @@ struct aak_handle_s {
   double *d_t = nullptr;      // [N+2]
   int *d_flags = nullptr;     // [4]: non-positive Phidot count, fold fallbacks
   int *d_guard = nullptr;
-  double *d_bessel = nullptr; // c_m(k) table | log k!
   FoldWork fw{};
   int fold_nblk_cap = 0;
   void *d_foldmem = nullptr;
-  // likelihood
   double2 *d_hf = nullptr;    // [2][nbins]
-  double2 *d_data = nullptr;  // [2][nbins]
-  double *d_w = nullptr;      // [2][nbins]
   double *d_partials = nullptr, *d_like = nullptr;
-  double *d_like_batch = nullptr;
-  double *d_phase = nullptr;  // [8][N] phase output, allocated on first use
-  double *d_ms_coef = nullptr; // mode-sum block Taylor coefficients (grown with nmodes)
+  double *d_ms_coef = nullptr;  // mode-sum block Taylor coefficients (grown with nmodes)
   size_t ms_coef_cap = 0;
   int *d_ms_p = nullptr;
   double *d_ms_eref = nullptr;
-  size_t like_batch_cap = 0;
-  PinnedBuf like_host;
   rocfft_plan plan = nullptr;
   rocfft_execution_info exec = nullptr;
   void *d_work = nullptr;
+  bool ready = false;
+};
+
+constexpr int kMaxLanes = 4;
+
+struct aak_handle_s {
+  WaveConfig cfg;
+  int N = 0, nbins = 0;
+  hipStream_t cstream = nullptr;  // knot-table uploads of every lane
+  Lane lane[kMaxLanes];
+  int nlanes = 2;                 // lanes used by batches (lane 0 serves single calls)
+  double *d_bessel = nullptr;     // c_m(k) table | log k!
+  double2 *d_data = nullptr;      // [2][nbins] weighted data
+  double *d_w = nullptr;          // [2][nbins] weights
+  double *d_like_batch = nullptr;
+  double *d_phase = nullptr;      // [8][N] phase output, allocated on first use
+  size_t like_batch_cap = 0;
+  PinnedBuf like_host;
   int have_data = 0;
-  int state = 0;  // 0 empty, 1 time domain, 2 weighted spectrum
+  int state = 0;  // 0 empty, 1 time domain, 2 weighted spectrum (lane 0)
   int info[10] = {0};
   // single-waveform staging
   WaveformPlan single;
@@ -169,61 +184,53 @@ struct aak_handle_s {
   long long b_n = 0, b_next = 0;
   std::vector<long long> b_order;  // templates in decreasing estimated host cost
 
-  void mark(int k) {
-    if (profiling) (void)hipEventRecord(evs[k], stream);
+  void mark(const Lane &L, int k) {
+    if (profiling) (void)hipEventRecord(evs[k], L.stream);
   }
-  DenseBuffers bufs(int slot) const {
+  DenseBuffers bufs(const Lane &ln, int slot) const {
     DenseBuffers B;
     const size_t L = (size_t)N + 1;
-    B.knots = d_table[slot];
-    B.e = d_dense;
-    B.v = d_dense + L;
-    B.M = d_dense + 2 * L;
-    B.S = d_dense + 3 * L;
+    B.knots = ln.d_table[slot];
+    B.e = ln.d_dense;
+    B.v = ln.d_dense + L;
+    B.M = ln.d_dense + 2 * L;
+    B.S = ln.d_dense + 3 * L;
     for (int p = 0; p < 3; p++) {
-      B.rate[p] = d_dense + (4 + p) * L;
-      B.phase[p] = d_dense + (7 + p) * L;
+      B.rate[p] = ln.d_dense + (4 + p) * L;
+      B.phase[p] = ln.d_dense + (7 + p) * L;
     }
-    B.amp = d_dense + 10 * L;
-    B.nu0 = d_dense + 11 * L;
-    B.h = d_h;
-    B.t = d_t;
-    B.flags = d_flags;
-    B.guard_blk = d_guard;
-    B.ms_coef = d_ms_coef;
-    B.ms_p = d_ms_p;
-    B.ms_eref = d_ms_eref;
+    B.amp = ln.d_dense + 10 * L;
+    B.nu0 = ln.d_dense + 11 * L;
+    B.h = ln.d_h;
+    B.t = ln.d_t;
+    B.flags = ln.d_flags;
+    B.guard_blk = ln.d_guard;
+    B.ms_coef = ln.d_ms_coef;
+    B.ms_p = ln.d_ms_p;
+    B.ms_eref = ln.d_ms_eref;
     B.bessel_c = d_bessel;
     B.bessel_lgf = d_bessel + (size_t)(kBesselKMax + 1) * kBesselMMax;
     return B;
   }
 };
 
-static int alloc_handle(aak_handle h) {
+static int alloc_lane(aak_handle h, Lane &L) {
+  if (L.ready) return 0;
   const int N = h->N;
-  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
   for (int k = 0; k < 2; k++) {
-    HIPCHK(hipEventCreateWithFlags(&h->ev_upload[k], hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&h->ev_rates[k], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&L.ev_upload[k], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&L.ev_rates[k], hipEventDisableTiming));
   }
-  HIPCHK(hipMalloc(&h->d_dense, sizeof(double) * (11 * ((size_t)N + 1) + 1)));
-  HIPCHK(hipMalloc(&h->d_h, sizeof(double) * 2 * ((size_t)N + 2)));
-  HIPCHK(hipMalloc(&h->d_t, sizeof(double) * ((size_t)N + 2)));
-  HIPCHK(hipMalloc(&h->d_flags, sizeof(int) * 4));
-  HIPCHK(hipMalloc(&h->d_guard, sizeof(int) * ((size_t)N / 4096 + 4)));
-  HIPCHK(hipMalloc(&h->d_ms_p, sizeof(int) * ((size_t)N / 256 + 4)));
-  HIPCHK(hipMalloc(&h->d_ms_eref, sizeof(double) * ((size_t)N / 256 + 4)));
-  {
-    const size_t nc = (size_t)(kBesselKMax + 1) * kBesselMMax, nl = kBesselKMax + 1;
-    std::vector<double> tab(nc + nl);
-    bessel_series_table(tab.data());
-    bessel_log_factorials(tab.data() + nc);
-    HIPCHK(hipMalloc(&h->d_bessel, sizeof(double) * tab.size()));
-    HIPCHK(hipMemcpy(h->d_bessel, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
-  }
+  HIPCHK(hipMalloc(&L.d_dense, sizeof(double) * (11 * ((size_t)N + 1) + 1)));
+  HIPCHK(hipMalloc(&L.d_h, sizeof(double) * 2 * ((size_t)N + 2)));
+  HIPCHK(hipMalloc(&L.d_t, sizeof(double) * ((size_t)N + 2)));
+  HIPCHK(hipMalloc(&L.d_flags, sizeof(int) * 4));
+  HIPCHK(hipMalloc(&L.d_guard, sizeof(int) * ((size_t)N / 4096 + 4)));
+  HIPCHK(hipMalloc(&L.d_ms_p, sizeof(int) * ((size_t)N / 256 + 4)));
+  HIPCHK(hipMalloc(&L.d_ms_eref, sizeof(double) * ((size_t)N / 256 + 4)));
   const int nblk = (N + kFoldBlock - 1) / kFoldBlock + 1;
-  h->fold_nblk_cap = nblk;
+  L.fold_nblk_cap = nblk;
   size_t bytes = 0;
   auto carve = [&](size_t n) {
     size_t off = bytes;
@@ -236,25 +243,23 @@ static int alloc_handle(aak_handle h) {
   const size_t o_items = carve(sizeof(FoldItem) * 3 * (size_t)nblk * kFoldMaxItems);
   const size_t o_ties = carve(sizeof(FoldTie) * 3 * (size_t)nblk * kFoldMaxTies);
   const size_t o_st = carve(sizeof(int) * 3);
-  HIPCHK(hipMalloc(&h->d_foldmem, bytes));
-  char *base = (char *)h->d_foldmem;
-  h->fw.bsum = (double *)(base + o_bsum);
-  h->fw.bmin = (double *)(base + o_bmin);
-  h->fw.bmax = (double *)(base + o_bmax);
-  h->fw.xstart = (double *)(base + o_xs);
-  h->fw.E = (double *)(base + o_E);
-  h->fw.nitems = (int *)(base + o_ni);
-  h->fw.nties = (int *)(base + o_nt);
-  h->fw.items = (FoldItem *)(base + o_items);
-  h->fw.ties = (FoldTie *)(base + o_ties);
-  h->fw.status = (int *)(base + o_st);
-  h->fw.fallbacks = h->d_flags + 1;
+  HIPCHK(hipMalloc(&L.d_foldmem, bytes));
+  char *base = (char *)L.d_foldmem;
+  L.fw.bsum = (double *)(base + o_bsum);
+  L.fw.bmin = (double *)(base + o_bmin);
+  L.fw.bmax = (double *)(base + o_bmax);
+  L.fw.xstart = (double *)(base + o_xs);
+  L.fw.E = (double *)(base + o_E);
+  L.fw.nitems = (int *)(base + o_ni);
+  L.fw.nties = (int *)(base + o_nt);
+  L.fw.items = (FoldItem *)(base + o_items);
+  L.fw.ties = (FoldTie *)(base + o_ties);
+  L.fw.status = (int *)(base + o_st);
+  L.fw.fallbacks = L.d_flags + 1;
   const int nb = h->nbins;
-  HIPCHK(hipMalloc(&h->d_hf, sizeof(double2) * 2 * (size_t)nb));
-  HIPCHK(hipMalloc(&h->d_data, sizeof(double2) * 2 * (size_t)nb));
-  HIPCHK(hipMalloc(&h->d_w, sizeof(double) * 2 * (size_t)nb));
-  HIPCHK(hipMalloc(&h->d_partials, sizeof(double) * 2 * 2048));
-  HIPCHK(hipMalloc(&h->d_like, sizeof(double) * 2));
+  HIPCHK(hipMalloc(&L.d_hf, sizeof(double2) * 2 * (size_t)nb));
+  HIPCHK(hipMalloc(&L.d_partials, sizeof(double) * 2 * 2048));
+  HIPCHK(hipMalloc(&L.d_like, sizeof(double) * 2));
   // rocFFT: two real sequences of length N (distance N+2) -> two half spectra
   std::call_once(g_fft_once, []() { rocfft_setup(); });
   rocfft_plan_description desc = nullptr;
@@ -263,22 +268,57 @@ static int alloc_handle(aak_handle h) {
   const size_t in_dist = (size_t)N + 2, out_dist = (size_t)nb;
   if (rocfft_plan_description_set_data_layout(desc, rocfft_array_type_real, rocfft_array_type_hermitian_interleaved,
                                               nullptr, nullptr, 1, &in_stride, in_dist, 1, &out_stride,
-                                              out_dist) != rocfft_status_success)
+                                              out_dist) != rocfft_status_success) {
+    rocfft_plan_description_destroy(desc);
     return fail(-101, "rocfft layout");
-  const size_t len = (size_t)N;
-  if (rocfft_plan_create(&h->plan, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                         rocfft_precision_double, 1, &len, 2, desc) != rocfft_status_success)
-    return fail(-101, "rocfft plan");
-  rocfft_plan_description_destroy(desc);
-  size_t wsize = 0;
-  rocfft_plan_get_work_buffer_size(h->plan, &wsize);
-  rocfft_execution_info_create(&h->exec);
-  if (wsize) {
-    HIPCHK(hipMalloc(&h->d_work, wsize));
-    rocfft_execution_info_set_work_buffer(h->exec, h->d_work, wsize);
   }
-  rocfft_execution_info_set_stream(h->exec, h->stream);
+  const size_t len = (size_t)N;
+  const rocfft_status ps = rocfft_plan_create(&L.plan, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                                              rocfft_precision_double, 1, &len, 2, desc);
+  rocfft_plan_description_destroy(desc);
+  if (ps != rocfft_status_success) return fail(-101, "rocfft plan");
+  size_t wsize = 0;
+  rocfft_plan_get_work_buffer_size(L.plan, &wsize);
+  if (rocfft_execution_info_create(&L.exec) != rocfft_status_success) return fail(-101, "rocfft execution info");
+  if (wsize) {
+    HIPCHK(hipMalloc(&L.d_work, wsize));
+    rocfft_execution_info_set_work_buffer(L.exec, L.d_work, wsize);
+  }
+  rocfft_execution_info_set_stream(L.exec, L.stream);
+  L.ready = true;
   return 0;
+}
+
+static void free_lane(Lane &L) {
+  if (L.stream) (void)hipStreamSynchronize(L.stream);
+  if (L.plan) rocfft_plan_destroy(L.plan);
+  if (L.exec) rocfft_execution_info_destroy(L.exec);
+  void *ptrs[] = {L.d_table[0], L.d_table[1], L.d_dense, L.d_h, L.d_t, L.d_flags, L.d_guard, L.d_foldmem,
+                  L.d_hf, L.d_partials, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_eref};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int k = 0; k < 2; k++) {
+    if (L.ev_upload[k]) (void)hipEventDestroy(L.ev_upload[k]);
+    if (L.ev_rates[k]) (void)hipEventDestroy(L.ev_rates[k]);
+  }
+  if (L.stream) (void)hipStreamDestroy(L.stream);
+  L = Lane();
+}
+
+static int alloc_handle(aak_handle h) {
+  HIPCHK(hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
+  {
+    const size_t nc = (size_t)(kBesselKMax + 1) * kBesselMMax, nl = kBesselKMax + 1;
+    std::vector<double> tab(nc + nl);
+    bessel_series_table(tab.data());
+    bessel_log_factorials(tab.data() + nc);
+    HIPCHK(hipMalloc(&h->d_bessel, sizeof(double) * tab.size()));
+    HIPCHK(hipMemcpy(h->d_bessel, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  }
+  const int nb = h->nbins;
+  HIPCHK(hipMalloc(&h->d_data, sizeof(double2) * 2 * (size_t)nb));
+  HIPCHK(hipMalloc(&h->d_w, sizeof(double) * 2 * (size_t)nb));
+  return alloc_lane(h, h->lane[0]);
 }
 
 static void stop_workers(aak_handle h) {
@@ -298,13 +338,9 @@ static void stop_workers(aak_handle h) {
 
 static void free_handle(aak_handle h) {
   stop_workers(h);
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->cstream) (void)hipStreamSynchronize(h->cstream);
-  if (h->plan) rocfft_plan_destroy(h->plan);
-  if (h->exec) rocfft_execution_info_destroy(h->exec);
-  void *ptrs[] = {h->d_table[0], h->d_table[1], h->d_dense, h->d_h, h->d_t, h->d_flags, h->d_guard,
-                  h->d_bessel, h->d_foldmem, h->d_hf, h->d_data, h->d_w, h->d_partials, h->d_like,
-                  h->d_like_batch, h->d_work, h->d_phase, h->d_ms_coef, h->d_ms_p, h->d_ms_eref};
+  for (Lane &L : h->lane) free_lane(L);
+  void *ptrs[] = {h->d_bessel, h->d_data, h->d_w, h->d_like_batch, h->d_phase};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   h->single_pin.release();
@@ -313,88 +349,80 @@ static void free_handle(aak_handle h) {
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t &e : h->batch_ev)
     if (e) (void)hipEventDestroy(e);
-  for (int k = 0; k < 2; k++) {
-    if (h->ev_upload[k]) (void)hipEventDestroy(h->ev_upload[k]);
-    if (h->ev_rates[k]) (void)hipEventDestroy(h->ev_rates[k]);
-  }
   if (h->cstream) (void)hipStreamDestroy(h->cstream);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
-// Buffers after a possible growth of the mode-sum coefficient buffer.
-static DenseBuffers B2(aak_handle h, int slot) { return h->bufs(slot); }
-
-// Enqueue the device stage of one planned waveform.  The knot table goes to
-// device table `slot` on the copy stream; `copied` (optional) is recorded when
-// the host table may be reused.
-static int enqueue_waveform(aak_handle h, const WaveformPlan &plan, int slot, hipEvent_t copied,
+// Enqueue the device stage of one planned waveform on lane L.  The knot table
+// goes to the lane's table `slot` on the copy stream; `copied` (optional) is
+// recorded when the host table may be reused.
+static int enqueue_waveform(aak_handle h, Lane &L, const WaveformPlan &plan, int slot, hipEvent_t copied,
                             const DenseParams *Pover = nullptr) {
   const DenseParams &P = Pover ? *Pover : plan.P;
   const size_t need = plan.tab_n;
-  if (need > h->table_cap[slot]) {
+  if (need > L.table_cap[slot]) {
     // growing: no kernel may still read the old table
-    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipStreamSynchronize(L.stream));
     HIPCHK(hipStreamSynchronize(h->cstream));
-    if (h->d_table[slot]) HIPCHK(hipFree(h->d_table[slot]));
-    h->d_table[slot] = nullptr;
-    h->table_cap[slot] = 0;
+    if (L.d_table[slot]) HIPCHK(hipFree(L.d_table[slot]));
+    L.d_table[slot] = nullptr;
+    L.table_cap[slot] = 0;
     const size_t cap = need + need / 4 + 4096;
-    HIPCHK(hipMalloc(&h->d_table[slot], sizeof(double) * cap));
-    h->table_cap[slot] = cap;
-    h->rates_recorded[slot] = false;
+    HIPCHK(hipMalloc(&L.d_table[slot], sizeof(double) * cap));
+    L.table_cap[slot] = cap;
+    L.rates_recorded[slot] = false;
   }
-  if (h->rates_recorded[slot]) HIPCHK(hipStreamWaitEvent(h->cstream, h->ev_rates[slot], 0));
-  HIPCHK(hipMemcpyAsync(h->d_table[slot], plan.tab, sizeof(double) * need, hipMemcpyHostToDevice, h->cstream));
-  HIPCHK(hipEventRecord(h->ev_upload[slot], h->cstream));
+  if (!Pover) {
+    // block Taylor coefficients: blocks x nmodes x 3 x (kTaylorMax+1) doubles
+    const size_t nc = (size_t)modesum_blocks(P) * (size_t)(P.nmodes > 0 ? P.nmodes : 1) * 3 * (kTaylorMax + 1);
+    if (nc > L.ms_coef_cap) {
+      HIPCHK(hipStreamSynchronize(L.stream));
+      if (L.d_ms_coef) HIPCHK(hipFree(L.d_ms_coef));
+      L.d_ms_coef = nullptr;
+      L.ms_coef_cap = 0;
+      HIPCHK(hipMalloc(&L.d_ms_coef, sizeof(double) * nc));
+      L.ms_coef_cap = nc;
+    }
+  }
+  if (L.rates_recorded[slot]) HIPCHK(hipStreamWaitEvent(h->cstream, L.ev_rates[slot], 0));
+  HIPCHK(hipMemcpyAsync(L.d_table[slot], plan.tab, sizeof(double) * need, hipMemcpyHostToDevice, h->cstream));
+  HIPCHK(hipEventRecord(L.ev_upload[slot], h->cstream));
   if (copied) HIPCHK(hipEventRecord(copied, h->cstream));
-  const DenseBuffers B = h->bufs(slot);
-  launch_clear_flags(B, h->stream);
-  HIPCHK(hipStreamWaitEvent(h->stream, h->ev_upload[slot], 0));
-  h->mark(0);
-  launch_rates(P, B, h->stream);
-  HIPCHK(hipEventRecord(h->ev_rates[slot], h->stream));
-  h->rates_recorded[slot] = true;
-  h->mark(1);
-  launch_phidot_guard(P, B, h->stream);
+  const DenseBuffers B = h->bufs(L, slot);
+  launch_clear_flags(B, L.stream);
+  HIPCHK(hipStreamWaitEvent(L.stream, L.ev_upload[slot], 0));
+  h->mark(L, 0);
+  launch_rates(P, B, L.stream);
+  HIPCHK(hipEventRecord(L.ev_rates[slot], L.stream));
+  L.rates_recorded[slot] = true;
+  h->mark(L, 1);
+  launch_phidot_guard(P, B, L.stream);
   const int nfwd = P.last_fold - P.i0 + 1;
   for (int pass = 0; pass < 2; pass++) {
     const int dir = pass == 0 ? 1 : -1;
     const int nsteps = pass == 0 ? nfwd : P.i0;
     if (nsteps <= 0) continue;
-    if (fold_exact(P, B, h->fw, h->fold_nblk_cap, dir, nsteps, h->stream)) return fail(-7, "fold block capacity");
+    if (fold_exact(P, B, L.fw, L.fold_nblk_cap, dir, nsteps, L.stream)) return fail(-7, "fold block capacity");
   }
-  launch_fold_fill(P, B, h->stream);
-  h->mark(2);
-  if (!Pover) {
-    // block Taylor coefficients: blocks x nmodes x 3 x (kTaylorMax+1) doubles
-    const size_t need = (size_t)modesum_blocks(P) * (size_t)(P.nmodes > 0 ? P.nmodes : 1) * 3 * (kTaylorMax + 1);
-    if (need > h->ms_coef_cap) {
-      HIPCHK(hipStreamSynchronize(h->stream));
-      if (h->d_ms_coef) HIPCHK(hipFree(h->d_ms_coef));
-      h->d_ms_coef = nullptr;
-      h->ms_coef_cap = 0;
-      HIPCHK(hipMalloc(&h->d_ms_coef, sizeof(double) * need));
-      h->ms_coef_cap = need;
-    }
-  }
+  launch_fold_fill(P, B, L.stream);
+  h->mark(L, 2);
   if (Pover) {  // dense stage only
     HIPCHK(hipGetLastError());
     return 0;
   }
-  launch_modesum(P, B2(h, slot), h->stream);
-  h->mark(3);
+  launch_modesum(P, B, L.stream);
+  h->mark(L, 3);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-static int enqueue_likelihood(aak_handle h, double *out_dev, bool keep_spectrum) {
-  void *in[1] = {h->d_h};
-  void *outb[1] = {h->d_hf};
-  if (rocfft_execute(h->plan, in, outb, h->exec) != rocfft_status_success) return fail(-101, "rocfft execute");
-  h->mark(4);
-  launch_weighted_dot(h->d_hf, h->d_data, h->d_w, h->nbins, keep_spectrum ? h->d_hf : nullptr, h->d_partials,
-                      out_dev, h->stream);
-  h->mark(5);
+static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_spectrum) {
+  void *in[1] = {L.d_h};
+  void *outb[1] = {L.d_hf};
+  if (rocfft_execute(L.plan, in, outb, L.exec) != rocfft_status_success) return fail(-101, "rocfft execute");
+  h->mark(L, 4);
+  launch_weighted_dot(L.d_hf, h->d_data, h->d_w, h->nbins, keep_spectrum ? L.d_hf : nullptr, L.d_partials, out_dev,
+                      L.stream);
+  h->mark(L, 5);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -493,6 +521,10 @@ int aak_create(double T_fit, int init_length, int length, double init_dt, double
   h->cfg.backint = backint ? 1 : 0;
   h->N = length;
   h->nbins = length / 2 + 1;
+  if (const char *env = std::getenv("AAK_LANES")) {
+    const int v = std::atoi(env);
+    h->nlanes = v < 1 ? 1 : (v > kMaxLanes ? kMaxLanes : v);
+  }
   const int rc = alloc_handle(h);
   if (rc) {
     std::string keep = g_err;
@@ -539,11 +571,12 @@ int aak_input_data(aak_handle h, const double *hI_f, const double *hII_f, const 
   if (!h) return fail(-1, "null handle");
   if (len != h->nbins) return fail(-2, "input_data: len must equal length/2+1");
   const size_t nb = (size_t)h->nbins;
-  HIPCHK(hipMemcpyAsync(h->d_data, hI_f, sizeof(double2) * nb, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(h->d_data + nb, hII_f, sizeof(double2) * nb, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(h->d_w, ASDinv1, sizeof(double) * nb, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(h->d_w + nb, ASDinv2, sizeof(double) * nb, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  hipStream_t st = h->lane[0].stream;
+  HIPCHK(hipMemcpyAsync(h->d_data, hI_f, sizeof(double2) * nb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(h->d_data + nb, hII_f, sizeof(double2) * nb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(h->d_w, ASDinv1, sizeof(double) * nb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(h->d_w + nb, ASDinv2, sizeof(double) * nb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
   h->have_data = 1;
   return 0;
 }
@@ -564,11 +597,12 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
     h->state = 0;
     return fail(rc, err);
   }
-  rc = enqueue_waveform(h, plan, 0, nullptr);
+  Lane &L = h->lane[0];
+  rc = enqueue_waveform(h, L, plan, 0, nullptr);
   if (rc) return rc;
   int flags[4] = {0, 0, 0, 0};
-  HIPCHK(hipMemcpyAsync(flags, h->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpyAsync(flags, L.d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, L.stream));
+  HIPCHK(hipStreamSynchronize(L.stream));
   read_stage_times(h, 0, 3);
   h->state = 1;
   const DenseParams &P = plan.P;
@@ -604,17 +638,18 @@ int aak_gen_phases(aak_handle h, const double *params, double *t, double *phase_
   // the phase output applies the Phidot guard to every forward sample (AAKPhase.cc:233-241)
   DenseParams P = plan.P;
   P.guard_last = P.N - 1;
-  rc = enqueue_waveform(h, plan, 0, nullptr, &P);
+  Lane &L = h->lane[0];
+  rc = enqueue_waveform(h, L, plan, 0, nullptr, &P);
   if (rc) return rc;
-  const DenseBuffers B = h->bufs(0);
-  launch_phase_out(P, B, h->d_phase, h->stream);
+  const DenseBuffers B = h->bufs(L, 0);
+  launch_phase_out(P, B, h->d_phase, L.stream);
   HIPCHK(hipGetLastError());
   double *dst[8] = {t, phase_r, phase_theta, phase_phi, omega_r, omega_theta, omega_phi, ecc};
   for (int a = 0; a < 8; a++)
     if (dst[a])
       HIPCHK(hipMemcpyAsync(dst[a], h->d_phase + (size_t)a * h->N, sizeof(double) * h->N, hipMemcpyDeviceToHost,
-                            h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+                            L.stream));
+  HIPCHK(hipStreamSynchronize(L.stream));
   h->state = 0;
   return 0;
 }
@@ -623,11 +658,12 @@ int aak_likelihood(aak_handle h, double *like_out) {
   if (!h) return fail(-1, "null handle");
   if (!h->have_data) return fail(-2, "Likelihood before input_data");
   if (h->state != 1) return fail(-2, "Likelihood needs a waveform from gpu_gen_AAK");
-  h->mark(3);
-  int rc = enqueue_likelihood(h, h->d_like, true);
+  Lane &L = h->lane[0];
+  h->mark(L, 3);
+  int rc = enqueue_likelihood(h, L, L.d_like, true);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(like_out, h->d_like, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpyAsync(like_out, L.d_like, sizeof(double) * 2, hipMemcpyDeviceToHost, L.stream));
+  HIPCHK(hipStreamSynchronize(L.stream));
   read_stage_times(h, 3, 5);
   h->state = 2;
   return 0;
@@ -640,8 +676,12 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   if (n == 0) return 0;
   int rc = start_pool(h);
   if (rc) return rc;
+  const int nl = h->nlanes;
+  for (int l = 1; l < nl; l++) {
+    rc = alloc_lane(h, h->lane[l]);
+    if (rc) return rc;
+  }
   if ((size_t)n * 2 > h->like_batch_cap) {
-    HIPCHK(hipStreamSynchronize(h->stream));
     if (h->d_like_batch) HIPCHK(hipFree(h->d_like_batch));
     h->d_like_batch = nullptr;
     h->like_batch_cap = 0;
@@ -670,6 +710,8 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
                    [](const std::pair<double, long long> &a, const std::pair<double, long long> &b) {
                      return a.first > b.first;
                    });
+  // the two cheapest go first so the GPU starts after one short host stage
+  if (n > 2) std::rotate(cost.begin(), cost.end() - 2, cost.end());
   {
     std::lock_guard<std::mutex> lk(h->mu);
     h->b_order.resize((size_t)n);
@@ -697,10 +739,12 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
     st[i] = slot.rc < 0 ? slot.rc : 0;
     if (st[i] == 0 && err_rc == 0) {
       if (prof) h->evs = h->batch_ev.data() + 6 * i;
-      rc = enqueue_waveform(h, slot.plan, (int)(done & 1), slot.copied);
+      // templates alternate over the lanes; each lane alternates its two table slots
+      Lane &L = h->lane[done % nl];
+      rc = enqueue_waveform(h, L, slot.plan, (int)((done / nl) & 1), slot.copied);
       if (!rc) {
         slot.copy_pending = true;
-        rc = enqueue_likelihood(h, h->d_like_batch + 2 * i, false);
+        rc = enqueue_likelihood(h, L, h->d_like_batch + 2 * i, false);
       }
       if (rc) {
         err_rc = rc;
@@ -721,14 +765,16 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   h->evs = h->ev;
   h->state = 0;
   if (err_rc) {
-    (void)hipStreamSynchronize(h->stream);
+    for (int l = 0; l < nl; l++) (void)hipStreamSynchronize(h->lane[l].stream);
     (void)hipStreamSynchronize(h->cstream);
     if (status) std::memcpy(status, st.data(), sizeof(int) * (size_t)n);
     return fail(err_rc, err_msg);
   }
+  for (int l = 1; l < nl; l++) HIPCHK(hipStreamSynchronize(h->lane[l].stream));
+  Lane &L0 = h->lane[0];
   HIPCHK(hipMemcpyAsync(h->like_host.p, h->d_like_batch, sizeof(double) * 2 * (size_t)n, hipMemcpyDeviceToHost,
-                        h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+                        L0.stream));
+  HIPCHK(hipStreamSynchronize(L0.stream));
   HIPCHK(hipStreamSynchronize(h->cstream));
   if (prof) {
     for (int k = 0; k < 5; k++) h->batch_ms[k] = 0.0;
@@ -754,15 +800,16 @@ int aak_get_waveform(aak_handle h, double *t, double *hI, double *hII) {
   if (!h) return fail(-1, "null handle");
   if (h->state == 0) return fail(-2, "GetWaveform before gpu_gen_AAK");
   const size_t n = (size_t)h->N + 2;
-  HIPCHK(hipMemcpyAsync(t, h->d_t, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  const Lane &L = h->lane[0];
+  HIPCHK(hipMemcpyAsync(t, L.d_t, sizeof(double) * n, hipMemcpyDeviceToHost, L.stream));
   if (h->state == 2) {
-    HIPCHK(hipMemcpyAsync(hI, h->d_hf, sizeof(double2) * h->nbins, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(hII, h->d_hf + h->nbins, sizeof(double2) * h->nbins, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(hI, L.d_hf, sizeof(double2) * h->nbins, hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(hipMemcpyAsync(hII, L.d_hf + h->nbins, sizeof(double2) * h->nbins, hipMemcpyDeviceToHost, L.stream));
   } else {
-    HIPCHK(hipMemcpyAsync(hI, h->d_h, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(hII, h->d_h + n, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(hI, L.d_h, sizeof(double) * n, hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(hipMemcpyAsync(hII, L.d_h + n, sizeof(double) * n, hipMemcpyDeviceToHost, L.stream));
   }
-  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipStreamSynchronize(L.stream));
   return 0;
 }
 
@@ -925,8 +972,9 @@ int aak_test_get_dense(aak_handle h, double *out) {
   if (!h) return fail(-1, "null handle");
   if (h->state == 0) return fail(-2, "no waveform generated");
   const size_t L = (size_t)h->N + 1;
-  HIPCHK(hipMemcpyAsync(out, h->d_dense, sizeof(double) * 10 * L, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  const Lane &ln = h->lane[0];
+  HIPCHK(hipMemcpyAsync(out, ln.d_dense, sizeof(double) * 10 * L, hipMemcpyDeviceToHost, ln.stream));
+  HIPCHK(hipStreamSynchronize(ln.stream));
   return 0;
 }
 

@@ -329,9 +329,32 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
 // scaling side by side (independent dependency chains), reusing each slope
 // (y[i+1] - y[i]) / h_i for the two right-hand sides that contain it; the
 // backward pass follows.  y_out, if given, receives a copy of the four arrays.
+//
+// A tail where all four arrays are constant (a source that plunged: the knots
+// past the plunge hold the clamped state) has right-hand sides exactly +0, so
+// the forward values there decay geometrically and become exactly +0 within a
+// few hundred rows; from that row on every coefficient is +0 and the backward
+// pass leaves the earlier ones unchanged (c - gam * (+0) = c), so the solve
+// stops there and fills the rest.
 int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs, double *const *y_out) {
   if (n < 3) return -1;
   const int sys = n - 2;
+  // first index of the tail on which every array equals its last value, bit for bit
+  int t_const = n - 1;
+  {
+    uint64_t last[4];
+    for (int a = 0; a < 4; a++) std::memcpy(&last[a], &ys[a][n - 1], sizeof(uint64_t));
+    while (t_const > 0) {
+      bool same = true;
+      for (int a = 0; a < 4 && same; a++) {
+        uint64_t u;
+        std::memcpy(&u, &ys[a][t_const - 1], sizeof(uint64_t));
+        same = (u == last[a]);
+      }
+      if (!same) break;
+      t_const--;
+    }
+  }
   thread_local std::vector<double> gam_v;
   if ((int)gam_v.size() < sys) gam_v.resize(sys);
   double *gam = gam_v.data();
@@ -353,6 +376,7 @@ int spline_natural4(const double *x, const double *const *ys, int n, double *con
   double s0 = (y0[1] - y0[0]) * gi0, s1 = (y1[1] - y1[0]) * gi0;
   double s2 = (y2[1] - y2[0]) * gi0, s3 = (y3[1] - y3[0]) * gi0;
   double z0 = 0.0, z1 = 0.0, z2 = 0.0, z3 = 0.0, od_prev = 0.0, gam_prev = 0.0;
+  int stop = sys;  // rows [stop, sys) are +0
   for (int i = 0; i < sys; i++) {
     const double h1 = x[i + 2] - x[i + 1];
     const double gi1 = (h1 != 0.0) ? 1.0 / h1 : 0.0;
@@ -392,10 +416,21 @@ int spline_natural4(const double *x, const double *const *ys, int n, double *con
     s1 = t1;
     s2 = t2;
     s3 = t3;
+    if (i >= t_const && z0 == 0.0 && z1 == 0.0 && z2 == 0.0 && z3 == 0.0) {
+      stop = i + 1;
+      break;
+    }
+  }
+  if (stop < sys) {
+    for (int a = 0; a < 4; a++) {
+      for (int k = stop; k < sys; k++) cs[a][1 + k] = 0.0;
+      if (y_out)
+        for (int k = stop + 2; k < n; k++) y_out[a][k] = ys[a][k];
+    }
   }
   // c[1+i] = z_i / al_i - gam_i c[i+2]
-  double n0 = c0[sys], n1 = c1[sys], n2 = c2[sys], n3 = c3[sys];
-  for (int i = sys - 2; i >= 0; i--) {
+  double n0 = c0[stop], n1 = c1[stop], n2 = c2[stop], n3 = c3[stop];
+  for (int i = stop - 2; i >= 0; i--) {
     const double gm = gam[i];
     n0 = c0[1 + i] - gm * n0;
     n1 = c1[1 + i] - gm * n1;

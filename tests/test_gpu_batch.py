@@ -82,6 +82,25 @@ def test_batch_repeatable_and_thread_count_independent():
     assert np.array_equal(np.isnan(a), np.isnan(b))
 
 
+@pytest.mark.timeout(300)
+def test_batch_lane_count_independent(monkeypatch):
+    # batches spread templates over AAK_LANES concurrent streams (read when a
+    # handle is created); the results must not depend on the lane count
+    print("case batch-lanes N=2^20 dt=15 n_max=12 seed 7, lanes 1/2/3", flush=True)
+    P = _prior(np.random.default_rng(7), 16)
+    out = {}
+    for lanes in (1, 2, 3):
+        monkeypatch.setenv("AAK_LANES", str(lanes))
+        g = _handle_with_data(N2, DT2, 12, seed=4)
+        out[lanes] = g.LikelihoodBatch(P)
+        del g
+    ok = ~np.isnan(out[1][:, 0])
+    assert ok.sum() >= 8
+    for lanes in (2, 3):
+        assert np.array_equal(np.isnan(out[1]), np.isnan(out[lanes]))
+        assert np.array_equal(out[1][ok], out[lanes][ok]), lanes
+
+
 def test_batch_profiled_stage_times():
     print("case batch-profiling N=2^18 dt=10", flush=True)
     g = _handle_with_data(2 ** 18, 10.0, 8)
