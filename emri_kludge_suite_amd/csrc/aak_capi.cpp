@@ -24,6 +24,7 @@
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -87,8 +88,19 @@ struct PinnedBuf {
 
 struct PinnedSink : TableSink {
   PinnedBuf *buf;
-  explicit PinnedSink(PinnedBuf *b) : buf(b) {}
-  double *reserve(size_t n) override { return buf->reserve(n) ? nullptr : buf->p; }
+  std::atomic<size_t> *hint;  // largest table seen by the pool (nullptr: none)
+  explicit PinnedSink(PinnedBuf *b, std::atomic<size_t> *h = nullptr) : buf(b), hint(h) {}
+  double *reserve(size_t n) override {
+    if (n > buf->cap && hint) {
+      // a slot that has to grow goes straight to the largest size seen, so each
+      // slot of the ring allocates pinned memory at most about once
+      size_t seen = hint->load();
+      while (n > seen && !hint->compare_exchange_weak(seen, n)) {
+      }
+      if (seen > n) n = seen;
+    }
+    return buf->reserve(n) ? nullptr : buf->p;
+  }
 };
 
 // One plan slot of the batch pool.  Workers take a free slot, fill it with
@@ -183,6 +195,7 @@ struct aak_handle_s {
   const double *b_params = nullptr;
   long long b_n = 0, b_next = 0;
   std::vector<long long> b_order;  // templates in decreasing estimated host cost
+  std::atomic<size_t> pin_hint{0};  // largest knot table (doubles) planned by the pool
 
   void mark(const Lane &L, int k) {
     if (profiling) (void)hipEventRecord(evs[k], L.stream);
@@ -460,7 +473,7 @@ static void worker_main(aak_handle h) {
     }
     SourceParams sp;
     std::memcpy(&sp, params + 14 * (size_t)i, sizeof(SourceParams));
-    PinnedSink sink(&slot->pin);
+    PinnedSink sink(&slot->pin, &h->pin_hint);
     std::string err;
     const int rc = plan_waveform(h->cfg, sp, &slot->plan, &err, &sink);
     {
@@ -901,6 +914,7 @@ int aak_test_knot_table(const double *params, double T_fit, int length, double d
   grid[0] = plan.P.grid.K;
   grid[1] = plan.P.grid.head_off;
   grid[2] = plan.P.grid.steps;
+  grid[3] = plan.P.grid.flat;
   return (int)plan.tab_n;
 }
 

@@ -20,10 +20,10 @@ constexpr int kTableArrays = 8;
 // exactly as the host evolution computes them (AAK.cc:114-121, 224-231), so
 // host and device agree bit for bit without shipping K times.
 struct KnotGrid {
-  int K;           // knots
+  int K;           // knots in the table
   int head_off;    // knot index of dt_map[0]  (j0 - j_start)
   int steps;       // head length
-  int pad;
+  int flat;        // 1: the evolution's knots beyond K-1 bound flat intervals (value y[K-1])
   double dt_large, t_head_last;  // spacing, dt_map[steps-1]
 };
 

@@ -336,7 +336,8 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
 // few hundred rows; from that row on every coefficient is +0 and the backward
 // pass leaves the earlier ones unchanged (c - gam * (+0) = c), so the solve
 // stops there and fills the rest.
-int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs, double *const *y_out) {
+int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs, double *const *y_out,
+                    int *flat_from) {
   if (n < 3) return -1;
   const int sys = n - 2;
   // first index of the tail on which every array equals its last value, bit for bit
@@ -421,11 +422,15 @@ int spline_natural4(const double *x, const double *const *ys, int n, double *con
       break;
     }
   }
+  // knots from J on bound only flat intervals (equal values, zero coefficients)
+  const int J = (stop < sys) ? std::max(stop + 1, t_const) : n - 1;
+  if (flat_from) *flat_from = J;
+  const int fill_end = flat_from ? J + 1 : n;  // the caller keeps knots [0, J] only
   if (stop < sys) {
     for (int a = 0; a < 4; a++) {
-      for (int k = stop; k < sys; k++) cs[a][1 + k] = 0.0;
+      for (int m = stop + 1; m < std::min(fill_end, n - 1); m++) cs[a][m] = 0.0;
       if (y_out)
-        for (int k = stop + 2; k < n; k++) y_out[a][k] = ys[a][k];
+        for (int k = stop + 2; k < fill_end; k++) y_out[a][k] = ys[a][k];
     }
   }
   // c[1+i] = z_i / al_i - gam_i c[i+2]
@@ -462,13 +467,22 @@ void fill_table(const KnotSet &ks, const SparseTrajectory &tr, double *tab, Knot
     yo[a] = tab + (size_t)a * K;
     cs[a] = tab + (size_t)(4 + a) * K;
   }
-  if (spline_natural4(ks.kt(), ys, K, cs, yo) != 0)
+  int J = K - 1;
+  if (spline_natural4(ks.kt(), ys, K, cs, yo, &J) != 0) {
     for (int a = 0; a < 4; a++) std::memcpy(yo[a], ys[a], sizeof(double) * K);
-  std::memcpy(tab + (size_t)kTableArrays * K, tr.dt_map, sizeof(double) * ks.steps);
-  grid->K = K;
+    J = K - 1;
+  }
+  // A flat tail (from knot J on) is not stored: the device returns y[J] + 0 past
+  // knot J, which is what the cubic of every flat interval evaluates to.
+  const int Kt = std::max(J + 1, 2);
+  if (Kt < K)
+    for (int a = 1; a < kTableArrays; a++)
+      std::memmove(tab + (size_t)a * Kt, tab + (size_t)a * K, sizeof(double) * Kt);
+  std::memcpy(tab + (size_t)kTableArrays * Kt, tr.dt_map, sizeof(double) * ks.steps);
+  grid->K = Kt;
   grid->head_off = ks.j0 - ks.j_start;
   grid->steps = ks.steps;
-  grid->pad = 0;
+  grid->flat = (Kt < K) ? 1 : 0;
   grid->dt_large = ks.dt_large;
   grid->t_head_last = tr.dt_map[ks.steps - 1];
 }
@@ -476,6 +490,7 @@ void fill_table(const KnotSet &ks, const SparseTrajectory &tr, double *tab, Knot
 double table_eval(const double *tab, const KnotGrid &g, int array, double t) {
   const int K = g.K;
   const double *head = tab + (size_t)kTableArrays * K;
+  if (g.flat && t >= knot_time(g, head, K - 1)) return tab[(size_t)array * K + K - 1] + 0.0;
   const int k = knot_index(g, head, t);
   const double x0 = knot_time(g, head, k), x1 = knot_time(g, head, k + 1);
   const double *y = tab + (size_t)array * K, *c = tab + (size_t)(4 + array) * K;
@@ -523,6 +538,7 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
     plan->tab = plan->table.data();
   }
   fill_table(ks, plan->traj, plan->tab, &P.grid);
+  plan->tab_n = (size_t)kTableArrays * P.grid.K + ks.steps;
   const int N = cfg.length;
   P.N = N;
   P.i0 = ks.i0;

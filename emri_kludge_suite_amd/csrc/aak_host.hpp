@@ -78,8 +78,11 @@ int sparse_stage(const WaveConfig &cfg, const SourceParams &sp, SparseTrajectory
 int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double S_phys, const SparseTrajectory &tr,
                 KnotSet *ks, std::string *err);
 // Natural cubic splines of four arrays on one abscissa (one factorisation).
+// y_out (optional) receives a copy of the four arrays.  With flat_from the
+// constant tail is not written: *flat_from = first knot from which every
+// interval is flat (equal values, zero coefficients), n - 1 if none.
 int spline_natural4(const double *x, const double *const *ys, int n, double *const *cs,
-                    double *const *y_out = nullptr);
+                    double *const *y_out = nullptr, int *flat_from = nullptr);
 int spline_natural(const double *x, const double *y, int n, double *c);
 size_t table_size(const KnotSet &ks);
 void fill_table(const KnotSet &ks, const SparseTrajectory &tr, double *tab, KnotGrid *grid);

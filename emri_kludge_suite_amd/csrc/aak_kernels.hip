@@ -53,6 +53,12 @@ __device__ inline void spline_sample(const DenseParams &P, const double *tab, do
   const KnotGrid &g = P.grid;
   const int K = g.K;
   const double *head = tab + (size_t)kTableArrays * K;
+  if (g.flat && t >= knot_time(g, head, K - 1)) {
+    // past the table: flat intervals, whose cubic is y[K-1] + 0
+#pragma unroll
+    for (int a = 0; a < 4; a++) y[a] = tab[(size_t)a * K + K - 1] + 0.0;
+    return;
+  }
   const int k = knot_index(g, head, t);
   const double x0 = knot_time(g, head, k), x1 = knot_time(g, head, k + 1);
   const double dl = t - x0;
@@ -110,7 +116,10 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   if (i > P.N) return;
   const double t = sample_time(P, i);
   double y[4];
-  if (staged) {
+  if (g.flat && t >= knot_time(g, head, K - 1)) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) y[a] = tab[(size_t)a * K + K - 1] + 0.0;
+  } else if (staged) {
     const int k = knot_search(s_x, 0, cnt, t);
     const double dl = t - s_x[k];
 #pragma unroll
@@ -255,6 +264,24 @@ __device__ inline double fold_incr(const double *rate, int i0, int dir, int s, d
 
 __device__ inline int out_index(const FoldArgs &A, int s) { return A.i0 + A.dir * (s + 1); }
 
+// A block's increments are read with consecutive lanes on consecutive steps
+// (coalesced) and staged in LDS; each lane then takes its run of
+// kFoldPerThread consecutive steps from LDS.  One pad double per run keeps
+// both access patterns free of bank conflicts.
+static_assert(kFoldPerThread == 16, "fold_lds_index assumes runs of 16");
+constexpr int kFoldLds = kFoldBlock + kFoldBlock / kFoldPerThread;
+__device__ inline int fold_lds_index(int l) { return l + (l >> 4); }
+
+__device__ inline void stage_increments(const FoldArgs &A, int p, int b, double *sd) {
+#pragma unroll 4
+  for (int q = 0; q < kFoldPerThread; q++) {
+    const int l = threadIdx.x + q * kFoldThreads;
+    const int s = b * kFoldBlock + l;
+    sd[fold_lds_index(l)] = (s < A.nsteps) ? fold_incr(A.rate[p], A.i0, A.dir, s, A.dt) : 0.0;
+  }
+  __syncthreads();
+}
+
 // Block-wide exclusive scans over kFoldThreads lanes: inclusive shuffle scan
 // inside each 64-lane wave, wave totals through LDS.  `sh` needs
 // kFoldThreads / 64 entries.  Integer scans are exact; the double scan is an
@@ -322,13 +349,15 @@ __device__ inline long long block_seg_incl(long long v, int f, long long *shv, i
 // approximate block sums and prefix ranges
 __global__ __launch_bounds__(kFoldThreads) void k_fold_sums(FoldArgs A, FoldWork W) {
   __shared__ double sh[kFoldThreads];
+  __shared__ double sd[kFoldLds];
   const int p = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
   const int base = b * kFoldBlock + tid * kFoldPerThread;
+  stage_increments(A, p, b, sd);
   double run = 0.0, mn = 0.0, mx = 0.0;
   for (int j = 0; j < kFoldPerThread; j++) {
     const int s = base + j;
     if (s < A.nsteps) {
-      run += fold_incr(A.rate[p], A.i0, A.dir, s, A.dt);
+      run += sd[fold_lds_index(tid * kFoldPerThread + j)];
       mn = fmin(mn, run);
       mx = fmax(mx, run);
     }
@@ -336,24 +365,24 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_sums(FoldArgs A, FoldWork
   double tot;
   const double off = block_excl_sum(run, sh, &tot);
   double lmn = off + mn, lmx = off + mx;
-  sh[tid] = lmn;
-  __syncthreads();
-  for (int o = kFoldThreads / 2; o > 0; o >>= 1) {
-    if (tid < o) sh[tid] = fmin(sh[tid], sh[tid + o]);
-    __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    lmn = fmin(lmn, __shfl_xor(lmn, o, 64));
+    lmx = fmax(lmx, __shfl_xor(lmx, o, 64));
   }
-  const double bmn = sh[0];
-  __syncthreads();
-  sh[tid] = lmx;
-  __syncthreads();
-  for (int o = kFoldThreads / 2; o > 0; o >>= 1) {
-    if (tid < o) sh[tid] = fmax(sh[tid], sh[tid + o]);
-    __syncthreads();
+  if ((tid & 63) == 0) {
+    sh[tid >> 6] = lmn;
+    sh[kFoldThreads / 64 + (tid >> 6)] = lmx;
   }
+  __syncthreads();
   if (tid == 0) {
+    double bmn = sh[0], bmx = sh[kFoldThreads / 64];
+    for (int w = 1; w < kFoldThreads / 64; w++) {
+      bmn = fmin(bmn, sh[w]);
+      bmx = fmax(bmx, sh[kFoldThreads / 64 + w]);
+    }
     W.bsum[p * A.nblk + b] = tot;
     W.bmin[p * A.nblk + b] = bmn;
-    W.bmax[p * A.nblk + b] = sh[0];
+    W.bmax[p * A.nblk + b] = bmx;
   }
 }
 
@@ -419,16 +448,17 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
   __shared__ long long shl[kFoldThreads];
   __shared__ int shi[kFoldThreads];
   __shared__ int shf[kFoldThreads];
+  __shared__ double sd[kFoldLds];
   const int p = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
   if (W.status[p]) return;
   const int base = b * kFoldBlock + tid * kFoldPerThread;
   const double E = W.E[p];
+  stage_increments(A, p, b, sd);
   double d[kFoldPerThread];
   double tsum = 0.0;
 #pragma unroll
   for (int j = 0; j < kFoldPerThread; j++) {
-    const int s = base + j;
-    d[j] = (s < A.nsteps) ? fold_incr(A.rate[p], A.i0, A.dir, s, A.dt) : 0.0;
+    d[j] = sd[fold_lds_index(tid * kFoldPerThread + j)];
     tsum += d[j];
   }
   double dummy;
@@ -519,6 +549,9 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
   int item = item_base - 1, tcount = tie_base;
   int c_item = -1, c_k = 0, c_tf = 0;  // WRITE: the item of the current run
   long long c_m = 0;
+  double outv[kFoldPerThread];  // WRITE: values of the run steps, NaN elsewhere
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) outv[j] = __builtin_nan("");
 #pragma unroll
   for (int j = 0; j < kFoldPerThread; j++) {
     if (key[j] == -2) break;
@@ -559,7 +592,20 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
       long long ups = 0;
       for (int q = c_tf; q < tcount; q++) ups += ties[q].up;
       const long long m = c_m + P + ups;
-      A.out[p][out_index(A, base + j)] = ldexp((double)m, c_k - 53);
+      outv[j] = ldexp((double)m, c_k - 53);
+    }
+  }
+  if (WRITE) {
+    // coalesced stores of the run steps; single steps were written by the chain
+#pragma unroll
+    for (int j = 0; j < kFoldPerThread; j++) sd[fold_lds_index(tid * kFoldPerThread + j)] = outv[j];
+    __syncthreads();
+#pragma unroll 4
+    for (int q = 0; q < kFoldPerThread; q++) {
+      const int l = tid + q * kFoldThreads;
+      const int s = b * kFoldBlock + l;
+      const double v = sd[fold_lds_index(l)];
+      if (s < A.nsteps && !__builtin_isnan(v)) A.out[p][out_index(A, s)] = v;
     }
   }
 }

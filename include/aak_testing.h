@@ -21,8 +21,9 @@ int aak_test_host_plan(const double *params, double T_fit, int length, double dt
                        int *info, double *dparams);
 
 /* Device knot table of the host stage (y of e, v, M, S | their natural-spline
- * coefficients c | head knot times); grid[0..2] = K, head offset, head length.
- * Returns the table length. */
+ * coefficients c | head knot times); grid[0..3] = table knots K, head offset,
+ * head length, flat flag (1: the knots after the table bound flat intervals of
+ * value y[K-1] and are not stored).  Returns the table length. */
 int aak_test_knot_table(const double *params, double T_fit, int length, double dt, int LISA, int backint,
                         double *table, int cap, int *grid);
 

@@ -120,44 +120,71 @@ def test_plunging_backint_host_stage_bit_identical(L):
 def _knot_table(L, pars, N, dt, backint=False):
     cap = 8 * 4_000_000 + 200
     tab = np.zeros(cap)
-    grid = np.zeros(3, dtype=np.int32)
+    grid = np.zeros(4, dtype=np.int32)
     n = L.lib().aak_test_knot_table(L.dptr(np.array(pars)), 1.0, N, dt, 1, int(backint), L.dptr(tab), cap,
                                     L.iptr(grid))
     assert n > 0, L.last_error()
     return tab[:n], grid
 
 
-@pytest.mark.parametrize("case", ["config1", "backint"])
-def test_knot_table_matches_the_natural_spline(L, case):
+def _table_case(case, L):
     if case == "config1":
-        pars, N, dt, backint = _draw(np.random.default_rng(1)), 2 ** 18, 10.0, False
-    else:
-        pars, N, dt, backint = [0.5, 0.9, 7.2, 0.3, 3e5, 30.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0], 2 ** 22, 8.0, True
+        return _draw(np.random.default_rng(1)), 2 ** 18, 10.0, False
+    if case == "backint":
+        return [0.5, 0.9, 7.2, 0.3, 3e5, 30.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0], 2 ** 22, 8.0, True
+    return _flat_tail_source(L), 2 ** 21, 15.0, False
+
+
+def _flat_tail_source(L):
+    # a source of the production prior that plunges early in a long knot grid:
+    # its knots past the plunge hold one clamped state, which the table leaves
+    # out.  First draw (seed 3) whose table drops at least half the knots.
+    rng = np.random.default_rng(3)
+    for _ in range(400):
+        pars = _draw(rng, production=True)
+        K, *_ = _product_plan(L, pars, 2 ** 21, 15.0, False, 4_000_000)
+        if K < 50_000:
+            continue
+        tab, grid = _knot_table(L, pars, 2 ** 21, 15.0, False)
+        if grid[3] == 1 and grid[0] < K // 2:
+            return pars
+    pytest.skip("no flat-tail source among the draws")
+
+
+@pytest.mark.parametrize("case", ["config1", "backint", "flat_tail"])
+def test_knot_table_matches_the_natural_spline(L, case):
+    pars, N, dt, backint = _table_case(case, L)
     K, _, kn, info, _ = _product_plan(L, pars, N, dt, backint)
     cap = 4_000_000
     tab, grid = _knot_table(L, pars, N, dt, backint)
-    assert grid[0] == K
+    Kt = grid[0]
+    assert (Kt == K and grid[3] == 0) or (case == "flat_tail" and grid[3] == 1 and 2 <= Kt < K)
     t = np.ascontiguousarray(kn[:K])
     for a in range(4):
         y = np.ascontiguousarray(kn[(a + 1) * cap:(a + 1) * cap + K])
         c = np.zeros(K)
         ol.oracle().o_spline_natural(ol._ptr(t), ol._ptr(y), K, ol._ptr(c))
-        assert np.array_equal(tab[a * K:(a + 1) * K], y)
-        assert np.array_equal(tab[(4 + a) * K:(5 + a) * K], c), "spline coefficients differ for array %d" % a
+        assert np.array_equal(tab[a * Kt:(a + 1) * Kt], y[:Kt])
+        assert np.array_equal(tab[(4 + a) * Kt:(5 + a) * Kt], c[:Kt]), "spline coefficients differ for array %d" % a
+        if Kt < K:
+            # the knots left out bound flat intervals: same value, zero coefficients
+            assert np.all(y[Kt - 1:].view(np.int64) == y[Kt - 1:Kt].view(np.int64)), a
+            assert np.all(c[Kt - 1:].view(np.int64) == 0), a
     # the head holds the dt_map knot times; the rest are regenerated on the device
     steps, off = grid[2], grid[1]
-    assert np.array_equal(tab[8 * K:8 * K + steps], kn[off:off + steps] if off + steps <= K else tab[8 * K:8 * K + steps])
+    assert np.array_equal(tab[8 * Kt:8 * Kt + steps],
+                          kn[off:off + steps] if off + steps <= K else tab[8 * Kt:8 * Kt + steps])
 
 
-@pytest.mark.parametrize("case", ["config1", "backint"])
+@pytest.mark.parametrize("case", ["config1", "backint", "flat_tail"])
 def test_device_spline_evaluation_matches_the_oracle(L, case):
     # knot interval from the regenerated knot times and the cubic from (y, c): the
     # interval of a bisection over the knots and the oracle's spline value, bit for bit
-    if case == "config1":
-        pars, N, dt, backint = _draw(np.random.default_rng(1)), 2 ** 18, 10.0, False
-    else:
-        pars, N, dt, backint = [0.5, 0.9, 7.2, 0.3, 3e5, 30.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0], 2 ** 22, 8.0, True
+    # (with a flat tail left out of the table, its values and no interval past it)
+    pars, N, dt, backint = _table_case(case, L)
     K, _, kn, info, _ = _product_plan(L, pars, N, dt, backint)
+    _, grid = _knot_table(L, pars, N, dt, backint)
+    Kt = grid[0]
     cap = 4_000_000
     t = np.ascontiguousarray(kn[:K])
     rng = np.random.default_rng(5)
@@ -169,7 +196,8 @@ def test_device_spline_evaluation_matches_the_oracle(L, case):
     assert L.lib().aak_test_knot_eval(L.dptr(np.array(pars)), 1.0, N, dt, 1, int(backint), L.dptr(ts), ts.size,
                                       L.iptr(k_out), L.dptr(y_out)) == 0
     ref_k = np.clip(np.searchsorted(t, ts, side="right") - 1, 0, K - 2)
-    assert np.array_equal(k_out, ref_k)
+    inside = ts < t[Kt - 1]
+    assert np.array_equal(k_out[inside], ref_k[inside])
     for a in range(4):
         y = np.ascontiguousarray(kn[(a + 1) * cap:(a + 1) * cap + K])
         c = np.zeros(K)
