@@ -48,6 +48,7 @@ def lib():
         "aak_test_taylor_amplitudes": (_i, [_i, _i, _d, _d, _d, _pd]),
         "aak_likelihood_batch": (_i, [C.c_void_p, _pd, _i, _pd, _pi]),
         "aak_set_host_threads": (_i, [C.c_void_p, _i]),
+        "aak_set_lanes": (_i, [C.c_void_p, _i]),
         "aak_set_precision": (_i, [C.c_void_p, _i]),
         "aak_gen_phases": (_i, [C.c_void_p, _pd] + [_pd] * 8),
         "aak_get_batch_timings": (_i, [C.c_void_p, _pd, _i]),

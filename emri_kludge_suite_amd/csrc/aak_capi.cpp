@@ -108,7 +108,8 @@ struct PinnedSink : TableSink {
 // slots in completion order (a slow host stage does not hold up the GPU) and
 // frees them once their table upload is enqueued.
 struct RingSlot {
-  WaveformPlan plan;
+  DenseParams P;                // plan of the template held
+  size_t tab_n = 0;             // knot table length (in pin)
   PinnedBuf pin;
   hipEvent_t copied = nullptr;  // upload of this slot's table finished
   bool copy_pending = false;
@@ -146,6 +147,7 @@ struct Lane {
   double *d_ms_coef = nullptr;  // mode-sum block Taylor coefficients (grown with nmodes)
   size_t ms_coef_cap = 0;
   int *d_ms_p = nullptr;
+  int *d_ms_list = nullptr;
   double *d_ms_eref = nullptr;
   rocfft_plan plan = nullptr;
   rocfft_execution_info exec = nullptr;
@@ -196,6 +198,8 @@ struct aak_handle_s {
   long long b_n = 0, b_next = 0;
   std::vector<long long> b_order;  // templates in decreasing estimated host cost
   std::atomic<size_t> pin_hint{0};  // largest knot table (doubles) planned by the pool
+  size_t knot_hint = 0;             // largest evolution array length planned by the pool (under mu)
+  std::vector<size_t> worker_cap;   // evolution array length of each worker (under mu)
 
   void mark(const Lane &L, int k) {
     if (profiling) (void)hipEventRecord(evs[k], L.stream);
@@ -220,6 +224,7 @@ struct aak_handle_s {
     B.guard_blk = ln.d_guard;
     B.ms_coef = ln.d_ms_coef;
     B.ms_p = ln.d_ms_p;
+    B.ms_list = ln.d_ms_list;
     B.ms_eref = ln.d_ms_eref;
     B.bessel_c = d_bessel;
     B.bessel_lgf = d_bessel + (size_t)(kBesselKMax + 1) * kBesselMMax;
@@ -241,6 +246,7 @@ static int alloc_lane(aak_handle h, Lane &L) {
   HIPCHK(hipMalloc(&L.d_flags, sizeof(int) * 4));
   HIPCHK(hipMalloc(&L.d_guard, sizeof(int) * ((size_t)N / 4096 + 4)));
   HIPCHK(hipMalloc(&L.d_ms_p, sizeof(int) * ((size_t)N / 256 + 4)));
+  HIPCHK(hipMalloc(&L.d_ms_list, sizeof(int) * ((size_t)N / 256 + 4)));
   HIPCHK(hipMalloc(&L.d_ms_eref, sizeof(double) * ((size_t)N / 256 + 4)));
   const int nblk = (N + kFoldBlock - 1) / kFoldBlock + 1;
   L.fold_nblk_cap = nblk;
@@ -307,7 +313,7 @@ static void free_lane(Lane &L) {
   if (L.plan) rocfft_plan_destroy(L.plan);
   if (L.exec) rocfft_execution_info_destroy(L.exec);
   void *ptrs[] = {L.d_table[0], L.d_table[1], L.d_dense, L.d_h, L.d_t, L.d_flags, L.d_guard, L.d_foldmem,
-                  L.d_hf, L.d_partials, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_eref};
+                  L.d_hf, L.d_partials, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_list, L.d_ms_eref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; k++) {
@@ -368,10 +374,10 @@ static void free_handle(aak_handle h) {
 // Enqueue the device stage of one planned waveform on lane L.  The knot table
 // goes to the lane's table `slot` on the copy stream; `copied` (optional) is
 // recorded when the host table may be reused.
-static int enqueue_waveform(aak_handle h, Lane &L, const WaveformPlan &plan, int slot, hipEvent_t copied,
-                            const DenseParams *Pover = nullptr) {
-  const DenseParams &P = Pover ? *Pover : plan.P;
-  const size_t need = plan.tab_n;
+static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, const double *tab, size_t tab_n, int slot,
+                            hipEvent_t copied, const DenseParams *Pover = nullptr) {
+  const DenseParams &P = Pover ? *Pover : Pplan;
+  const size_t need = tab_n;
   if (need > L.table_cap[slot]) {
     // growing: no kernel may still read the old table
     HIPCHK(hipStreamSynchronize(L.stream));
@@ -397,7 +403,7 @@ static int enqueue_waveform(aak_handle h, Lane &L, const WaveformPlan &plan, int
     }
   }
   if (L.rates_recorded[slot]) HIPCHK(hipStreamWaitEvent(h->cstream, L.ev_rates[slot], 0));
-  HIPCHK(hipMemcpyAsync(L.d_table[slot], plan.tab, sizeof(double) * need, hipMemcpyHostToDevice, h->cstream));
+  HIPCHK(hipMemcpyAsync(L.d_table[slot], tab, sizeof(double) * need, hipMemcpyHostToDevice, h->cstream));
   HIPCHK(hipEventRecord(L.ev_upload[slot], h->cstream));
   if (copied) HIPCHK(hipEventRecord(copied, h->cstream));
   const DenseBuffers B = h->bufs(L, slot);
@@ -449,23 +455,40 @@ static void read_stage_times(aak_handle h, int first, int last) {
 }
 
 // ---------------------------------------------------------------- batch pool
-static void worker_main(aak_handle h) {
+static void worker_main(aak_handle h, int id) {
+  // the evolution arrays of the worker's plans, reused across templates and
+  // batches; the slot keeps the parameters and the knot table
+  WaveformPlan plan;
   for (;;) {
     long long i;
     const double *params;
     RingSlot *slot;
     int sidx;
+    size_t grow_to = 0;
     {
       std::unique_lock<std::mutex> lk(h->mu);
-      h->cv_work.wait(lk, [&] {
-        return h->stop || (h->b_params && h->b_next < h->b_n && !h->free_slots.empty());
-      });
+      auto has_work = [&] { return h->b_params && h->b_next < h->b_n && !h->free_slots.empty(); };
+      h->cv_work.wait(lk, [&] { return h->stop || has_work() || plan.knots.eA.size() < h->knot_hint; });
       if (h->stop) return;
-      i = h->b_order[(size_t)h->b_next++];
-      params = h->b_params;
-      sidx = h->free_slots.back();
-      h->free_slots.pop_back();
-      slot = h->ring[(size_t)sidx].get();
+      if (has_work()) {
+        i = h->b_order[(size_t)h->b_next++];
+        params = h->b_params;
+        sidx = h->free_slots.back();
+        h->free_slots.pop_back();
+        slot = h->ring[(size_t)sidx].get();
+      } else {
+        grow_to = h->knot_hint;
+      }
+    }
+    if (grow_to) {
+      // idle: grow to the largest evolution planned so far, outside any template
+      reserve_knots(&plan.knots, grow_to);
+      {
+        std::lock_guard<std::mutex> lk(h->mu);
+        h->worker_cap[(size_t)id] = plan.knots.eA.size();
+      }
+      h->cv_ready.notify_all();
+      continue;
     }
     if (slot->copy_pending) {
       (void)hipEventSynchronize(slot->copied);
@@ -475,14 +498,23 @@ static void worker_main(aak_handle h) {
     std::memcpy(&sp, params + 14 * (size_t)i, sizeof(SourceParams));
     PinnedSink sink(&slot->pin, &h->pin_hint);
     std::string err;
-    const int rc = plan_waveform(h->cfg, sp, &slot->plan, &err, &sink);
+    const int rc = plan_waveform(h->cfg, sp, &plan, &err, &sink);
+    slot->P = plan.P;
+    slot->tab_n = plan.tab_n;
+    bool hint_grew = false;
     {
       std::lock_guard<std::mutex> lk(h->mu);
+      h->worker_cap[(size_t)id] = plan.knots.eA.size();
+      if (plan.knots.eA.size() > h->knot_hint) {
+        h->knot_hint = plan.knots.eA.size();
+        hint_grew = true;
+      }
       slot->rc = rc;
       slot->tpl = i;
       h->ready_slots.push_back(sidx);
     }
-    h->cv_ready.notify_one();
+    h->cv_ready.notify_all();
+    if (hint_grew) h->cv_work.notify_all();  // idle workers grow to the new size
   }
 }
 
@@ -505,7 +537,8 @@ static int start_pool(aak_handle h) {
     HIPCHK(hipEventCreateWithFlags(&slot->copied, hipEventDisableTiming));
     h->ring.push_back(std::move(slot));
   }
-  for (int t = 0; t < T; t++) h->workers.emplace_back(worker_main, h);
+  h->worker_cap.assign((size_t)T, 0);
+  for (int t = 0; t < T; t++) h->workers.emplace_back(worker_main, h, t);
   return 0;
 }
 
@@ -571,6 +604,13 @@ int aak_set_precision(aak_handle h, int mode) {
   return 0;
 }
 
+int aak_set_lanes(aak_handle h, int n) {
+  if (!h) return fail(-1, "null handle");
+  if (n < 1 || n > kMaxLanes) return fail(-2, "lanes must be in [1, 4]");
+  h->nlanes = n;
+  return 0;
+}
+
 int aak_set_host_threads(aak_handle h, int n) {
   if (!h) return fail(-1, "null handle");
   if (n < 0 || n > 256) return fail(-2, "host threads must be in [0, 256]");
@@ -611,7 +651,7 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
     return fail(rc, err);
   }
   Lane &L = h->lane[0];
-  rc = enqueue_waveform(h, L, plan, 0, nullptr);
+  rc = enqueue_waveform(h, L, plan.P, plan.tab, plan.tab_n, 0, nullptr);
   if (rc) return rc;
   int flags[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(flags, L.d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, L.stream));
@@ -652,7 +692,7 @@ int aak_gen_phases(aak_handle h, const double *params, double *t, double *phase_
   DenseParams P = plan.P;
   P.guard_last = P.N - 1;
   Lane &L = h->lane[0];
-  rc = enqueue_waveform(h, L, plan, 0, nullptr, &P);
+  rc = enqueue_waveform(h, L, plan.P, plan.tab, plan.tab_n, 0, nullptr, &P);
   if (rc) return rc;
   const DenseBuffers B = h->bufs(L, 0);
   launch_phase_out(P, B, h->d_phase, L.stream);
@@ -694,6 +734,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
     rc = alloc_lane(h, h->lane[l]);
     if (rc) return rc;
   }
+
   if ((size_t)n * 2 > h->like_batch_cap) {
     if (h->d_like_batch) HIPCHK(hipFree(h->d_like_batch));
     h->d_like_batch = nullptr;
@@ -754,7 +795,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
       if (prof) h->evs = h->batch_ev.data() + 6 * i;
       // templates alternate over the lanes; each lane alternates its two table slots
       Lane &L = h->lane[done % nl];
-      rc = enqueue_waveform(h, L, slot.plan, (int)((done / nl) & 1), slot.copied);
+      rc = enqueue_waveform(h, L, slot.P, slot.pin.p, slot.tab_n, (int)((done / nl) & 1), slot.copied);
       if (!rc) {
         slot.copy_pending = true;
         rc = enqueue_likelihood(h, L, h->d_like_batch + 2 * i, false);
@@ -789,6 +830,20 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
                         L0.stream));
   HIPCHK(hipStreamSynchronize(L0.stream));
   HIPCHK(hipStreamSynchronize(h->cstream));
+  {
+    // with the pool idle and every upload done, bring each slot to the largest
+    // table planned so far, so later batches do not allocate pinned memory
+    const size_t hint = h->pin_hint.load();
+    for (auto &slot : h->ring)
+      if (slot->pin.cap < hint && slot->pin.reserve(hint)) return fail(-100, "pinned host allocation failed");
+    // and every worker's evolution arrays at the largest planned so far
+    std::unique_lock<std::mutex> lk(h->mu);
+    h->cv_ready.wait(lk, [&] {
+      for (size_t c : h->worker_cap)
+        if (c < h->knot_hint) return false;
+      return true;
+    });
+  }
   if (prof) {
     for (int k = 0; k < 5; k++) h->batch_ms[k] = 0.0;
     float ms = 0.f;

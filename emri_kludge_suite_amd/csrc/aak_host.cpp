@@ -136,6 +136,11 @@ static bool same_bits(double a, double b) {
   return x == y;
 }
 
+void reserve_knots(KnotSet *ks, size_t L) {
+  for (std::vector<double> *a : {&ks->eA, &ks->vA, &ks->t, &ks->e, &ks->v, &ks->M, &ks->S})
+    if (a->size() < L) a->resize(L);
+}
+
 int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double S_phys, const SparseTrajectory &tr,
                 KnotSet *ks, std::string *err) {
   const double timestep = cfg.dt;
@@ -154,8 +159,7 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
   // Arrays over the whole index range j in [0, 2 j0); every entry is written
   // before it is read, and storage is reused across waveforms.
   const size_t L = (size_t)2 * j0;
-  for (std::vector<double> *a : {&ks->eA, &ks->vA, &ks->t, &ks->e, &ks->v, &ks->M, &ks->S})
-    if (a->size() < L) a->resize(L);
+  reserve_knots(ks, L);
   double *eA = ks->eA.data(), *vA = ks->vA.data(), *tf = ks->t.data(), *ef = ks->e.data(), *vf = ks->v.data(),
          *Mf = ks->M.data(), *Sf = ks->S.data();
   double edot = 0, vdot = 0, edotm, vdotm;

@@ -77,6 +77,8 @@ struct TableSink {
 int sparse_stage(const WaveConfig &cfg, const SourceParams &sp, SparseTrajectory *out, std::string *err);
 int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double S_phys, const SparseTrajectory &tr,
                 KnotSet *ks, std::string *err);
+// Grow the evolution arrays of ks to L entries (no-op when they are larger).
+void reserve_knots(KnotSet *ks, size_t L);
 // Natural cubic splines of four arrays on one abscissa (one factorisation).
 // y_out (optional) receives a copy of the four arrays.  With flat_from the
 // constant tail is not written: *flat_from = first knot from which every

@@ -1129,30 +1129,6 @@ __device__ inline double harmonic_phase(const DenseParams &P, const SampleGeom &
   return nn * G.Phi;
 }
 
-// One sample, Bessel functions per harmonic (series or Miller) and one
-// sincos per harmonic: used where the Taylor form is not applicable.
-__device__ inline void modesum_direct(const DenseParams &P, const DenseBuffers &B, const SampleGeom &G, double *hI,
-                                      double *hII) {
-  const double e = G.e, sq1me2 = sqrt(1 - e * e), gdpi = G.gimdot / kPi;
-  double accI = 0.0, accII = 0.0, J[5];
-  for (int nn = 1; nn <= P.nmodes; nn++) {
-    const double nPhi = harmonic_phase(P, G, (double)nn, gdpi);
-    if (P.bessel_series)
-      bessel_five_series(nn, e, B.bessel_c, B.bessel_lgf, P.bterms[nn], P.bterms[0], J);
-    else
-      bessel_five(nn, nn * e, J);
-    double snp, cnp;
-    sincos(nPhi, &snp, &cnp);
-    const double al = -nn * (J[0] - 2. * e * J[1] + 2. / nn * J[2] + 2. * e * J[3] - J[4]);
-    const double be = -nn * sq1me2 * (J[0] - 2. * J[2] + J[4]);
-    const double ga = 2. * J[2];
-    accI += cnp * (al * G.W[0] + ga * G.W[2]) + snp * (be * G.W[1]);
-    accII += cnp * (al * G.W[3] + ga * G.W[5]) + snp * (be * G.W[4]);
-  }
-  *hI = accI;
-  *hII = accII;
-}
-
 // ------------------------------------------------------------------ Taylor form
 // Inside one block of samples e varies by a few 1e-5: the amplitude factors
 // alpha_n, beta_n, gamma_n are expanded in delta = e - e_ref once per block
@@ -1232,6 +1208,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, 
   if (lane == 0) {
     B.ms_p[blk] = p;
     B.ms_eref[blk] = eref;
+    if (p < 0) B.ms_list[atomicAdd(&B.flags[2], 1)] = blk;
   }
   if (p < 0 || !any) return;
   double sq[kTaylorMax + 1];
@@ -1429,22 +1406,66 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
   B.h[P.N + 2 + i] = sc * accII;
 }
 
-// Per-sample mode sum for blocks with B.ms_p < 0 (the others return at once).
+// Per-sample mode sum for the blocks listed by k_modesum_setup (B.ms_p < 0):
+// eight lanes per sample, each summing every eighth harmonic (the per-harmonic
+// Bessel and phase evaluation is the long part), partial sums combined by
+// shuffles.  A fixed grid walks the list, 32 samples per workgroup.
+constexpr int kDirectGroups = 8;
+constexpr int kDirectSamples = kMsThreads / kDirectGroups;
+constexpr int kDirectGrid = 1024;
+
+__device__ inline void modesum_direct_part(const DenseParams &P, const DenseBuffers &B, const SampleGeom &G, int g,
+                                           double *hI, double *hII) {
+  const double e = G.e, sq1me2 = sqrt(1 - e * e), gdpi = G.gimdot / kPi;
+  double accI = 0.0, accII = 0.0, J[5];
+  for (int nn = 1 + g; nn <= P.nmodes; nn += kDirectGroups) {
+    const double nPhi = harmonic_phase(P, G, (double)nn, gdpi);
+    if (P.bessel_series)
+      bessel_five_series(nn, e, B.bessel_c, B.bessel_lgf, P.bterms[nn], P.bterms[0], J);
+    else
+      bessel_five(nn, nn * e, J);
+    double snp, cnp;
+    sincos(nPhi, &snp, &cnp);
+    const double al = -nn * (J[0] - 2. * e * J[1] + 2. / nn * J[2] + 2. * e * J[3] - J[4]);
+    const double be = -nn * sq1me2 * (J[0] - 2. * J[2] + J[4]);
+    const double ga = 2. * J[2];
+    accI += cnp * (al * G.W[0] + ga * G.W[2]) + snp * (be * G.W[1]);
+    accII += cnp * (al * G.W[3] + ga * G.W[5]) + snp * (be * G.W[4]);
+  }
+  *hI = accI;
+  *hII = accII;
+}
+
 __global__ __launch_bounds__(kMsThreads) void k_modesum_direct(DenseParams P, DenseBuffers B) {
-  const int blk = blockIdx.x;
-  if (block_order(B, blk) >= 0) return;
-  const int i = blk * kMsThreads + threadIdx.x;
-  if (i >= P.N + 2) return;
+  const int count = B.flags[2];
+  const int units = count * (kMsThreads / kDirectSamples);
+  const int g = threadIdx.x % kDirectGroups, sl = threadIdx.x / kDirectGroups;
   const int n_end = min(P.N, P.ip + P.ib);
-  if (modesum_tail(P, B, i, n_end)) return;
-  SampleGeom G;
-  sample_geometry(P, B, i, G);
-  B.t[i] = G.ti;
-  double accI, accII;
-  modesum_direct(P, B, G, &accI, &accII);
-  const double sc = (i <= P.ip) ? G.Amp : G.Amp / taper(P, i, G.ti);
-  B.h[i] = sc * accI;
-  B.h[P.N + 2 + i] = sc * accII;
+  for (int u = blockIdx.x; u < units; u += gridDim.x) {
+    const int blk = B.ms_list[u / (kMsThreads / kDirectSamples)];
+    const int i = blk * kMsThreads + (u % (kMsThreads / kDirectSamples)) * kDirectSamples + sl;
+    // every lane takes part in the shuffles below; inactive ones add zeros
+    bool work = i < P.N + 2;
+    if (work && g == 0) work = !modesum_tail(P, B, i, n_end);
+    else if (work) work = (i < P.N && i < n_end);
+    double accI = 0.0, accII = 0.0;
+    SampleGeom G;
+    if (work) {
+      sample_geometry(P, B, i, G);
+      modesum_direct_part(P, B, G, g, &accI, &accII);
+    }
+#pragma unroll
+    for (int o = 1; o < kDirectGroups; o <<= 1) {
+      accI += __shfl_xor(accI, o, 64);
+      accII += __shfl_xor(accII, o, 64);
+    }
+    if (work && g == 0) {
+      B.t[i] = G.ti;
+      const double sc = (i <= P.ip) ? G.Amp : G.Amp / taper(P, i, G.ti);
+      B.h[i] = sc * accI;
+      B.h[P.N + 2 + i] = sc * accII;
+    }
+  }
 }
 
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
@@ -1456,7 +1477,7 @@ void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) 
     hipLaunchKernelGGL(k_modesum_setup<false>, dim3(nb), dim3(kSetupThreads), 0, s, P, B);
     hipLaunchKernelGGL(k_modesum<false>, dim3(nb), dim3(kMsThreads), 0, s, P, B);
   }
-  hipLaunchKernelGGL(k_modesum_direct, dim3(nb), dim3(kMsThreads), 0, s, P, B);
+  hipLaunchKernelGGL(k_modesum_direct, dim3(kDirectGrid), dim3(kMsThreads), 0, s, P, B);
 }
 
 // ------------------------------------------------------------------ likelihood

@@ -115,6 +115,10 @@ class GPUAAK:
         """Threads of the batch host pool (0: AAK_HOST_THREADS or the core count, at most 16)."""
         _lib.check(_lib.lib().aak_set_host_threads(self._h, int(n)), "set_host_threads")
 
+    def set_lanes(self, n):
+        """Concurrent streams of LikelihoodBatch (1..4); results do not depend on it."""
+        _lib.check(_lib.lib().aak_set_lanes(self._h, int(n)), "set_lanes")
+
     def set_nmax(self, n_max):
         """Override the harmonic count (0 restores (int)(30 e0), min 4)."""
         _lib.check(_lib.lib().aak_set_nmax(self._h, int(n_max)), "set_nmax")

@@ -80,6 +80,11 @@ int aak_set_precision(aak_handle h, int mode);
  * concurrency, at most 16); no counterpart in the reference. */
 int aak_set_host_threads(aak_handle h, int n);
 
+/* Concurrent streams ("lanes", 1..4) over which aak_likelihood_batch spreads
+ * its templates, each with its own device buffers (default AAK_LANES or 2);
+ * no counterpart in the reference.  Results do not depend on it. */
+int aak_set_lanes(aak_handle h, int n);
+
 /* AAK phases and frequencies of one source, on the handle's grid: the
  * reference's CPU AAKphase (AAKpy.cc:53-88, AAKPhase.cc:13-374; Python
  * AAKwrapper.phase, AAKwrapper.pyx:90-133).  params: the 14 gpu_gen_AAK
