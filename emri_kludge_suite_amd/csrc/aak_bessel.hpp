@@ -170,29 +170,42 @@ AAK_HD void taylor_sqrt1me2(double e_ref, int p, double *sq) {
 // signed binomial sums of the derivative formula are formed as iterated
 // differences, D_j(k) = D_{j-1}(k-1) - D_{j-1}(k+1) with D_0 = J, on one row that
 // shrinks by two orders per step.
-template <int PO, int S>
-AAK_HD void taylor_amplitudes_p(int n, double e_ref, const double *sq, const double *Jr, double *coef) {
+template <int PO, int S, class Out>
+AAK_HD void taylor_amplitudes_p(int n, double e_ref, const double *sq, const double *Jr, Out out) {
   constexpr int L = 5 + 2 * PO;
   double row[L];
   for (int q = 0; q < L; q++) row[q] = Jr[q * S];
-  double t[5][PO + 1];  // t[k][j] = (n/2)^j / j! D_j(n - 2 + k)
+  // coefficients are emitted order by order as their differences appear:
+  // t_k = (n/2)^j / j! D_j(n - 2 + k) of the current order, u[j] = t_0 - 2 t_2 + t_4
+  // of every order so far (the beta expansion convolves them with sq)
+  double u[PO + 1];
+  double t1_prev = 0.0, t3_prev = 0.0;
   double hn = 1.0;
+  const double dn = (double)n, two_n = 2. / dn;
   for (int j = 0; j <= PO; j++) {
-    for (int k = 0; k < 5; k++) t[k][j] = hn * row[k + PO - j];
+    const double t0 = hn * row[PO - j], t1 = hn * row[1 + PO - j], t2 = hn * row[2 + PO - j];
+    const double t3 = hn * row[3 + PO - j], t4 = hn * row[4 + PO - j];
+    const double e1 = e_ref * t1 + (j ? t1_prev : 0.0);
+    const double e3 = e_ref * t3 + (j ? t3_prev : 0.0);
+    out(0, j, -dn * (t0 - 2. * e1 + two_n * t2 + 2. * e3 - t4));
+    u[j] = t0 - 2. * t2 + t4;
+    double bsum = 0.0;
+    for (int q = 0; q <= j; q++) bsum += sq[q] * u[j - q];
+    out(1, j, -dn * bsum);
+    out(2, j, 2. * t2);
+    t1_prev = t1;
+    t3_prev = t3;
     for (int q = 0; q + 2 < L - 2 * j; q++) row[q] = row[q] - row[q + 2];
     hn = hn * (0.5 * n) * (1.0 / (j + 1));
   }
-  const double dn = (double)n, two_n = 2. / dn;
-  for (int j = 0; j <= PO; j++) {
-    const double e1 = e_ref * t[1][j] + (j ? t[1][j - 1] : 0.0);
-    const double e3 = e_ref * t[3][j] + (j ? t[3][j - 1] : 0.0);
-    coef[j] = -dn * (t[0][j] - 2. * e1 + two_n * t[2][j] + 2. * e3 - t[4][j]);
-    double bsum = 0.0;
-    for (int q = 0; q <= j; q++) bsum += sq[q] * (t[0][j - q] - 2. * t[2][j - q] + t[4][j - q]);
-    coef[PO + 1 + j] = -dn * bsum;
-    coef[2 * (PO + 1) + j] = 2. * t[2][j];
-  }
 }
+
+// Store of taylor_amplitudes_p into coef[f * (PO + 1) + j].
+template <int PO, class T>
+struct CoefStore {
+  T *coef;
+  AAK_HD void operator()(int f, int j, double v) const { coef[f * (PO + 1) + j] = (T)v; }
+};
 
 // Runtime-order form of taylor_amplitudes_p (coef holds 3 (p + 1) values).
 AAK_HD void taylor_amplitudes(int n, int p, double e_ref, const double *sq, const double *ctab, const double *lgf,
@@ -200,15 +213,15 @@ AAK_HD void taylor_amplitudes(int n, int p, double e_ref, const double *sq, cons
   double Jr[kTaylorOrders];
   bessel_run(n, p, e_ref, ctab, lgf, M, Jr);
   switch (p) {
-    case 0: taylor_amplitudes_p<0, 1>(n, e_ref, sq, Jr, coef); break;
-    case 1: taylor_amplitudes_p<1, 1>(n, e_ref, sq, Jr, coef); break;
-    case 2: taylor_amplitudes_p<2, 1>(n, e_ref, sq, Jr, coef); break;
-    case 3: taylor_amplitudes_p<3, 1>(n, e_ref, sq, Jr, coef); break;
-    case 4: taylor_amplitudes_p<4, 1>(n, e_ref, sq, Jr, coef); break;
-    case 5: taylor_amplitudes_p<5, 1>(n, e_ref, sq, Jr, coef); break;
-    case 6: taylor_amplitudes_p<6, 1>(n, e_ref, sq, Jr, coef); break;
-    case 7: taylor_amplitudes_p<7, 1>(n, e_ref, sq, Jr, coef); break;
-    default: taylor_amplitudes_p<8, 1>(n, e_ref, sq, Jr, coef); break;
+    case 0: taylor_amplitudes_p<0, 1>(n, e_ref, sq, Jr, CoefStore<0, double>{coef}); break;
+    case 1: taylor_amplitudes_p<1, 1>(n, e_ref, sq, Jr, CoefStore<1, double>{coef}); break;
+    case 2: taylor_amplitudes_p<2, 1>(n, e_ref, sq, Jr, CoefStore<2, double>{coef}); break;
+    case 3: taylor_amplitudes_p<3, 1>(n, e_ref, sq, Jr, CoefStore<3, double>{coef}); break;
+    case 4: taylor_amplitudes_p<4, 1>(n, e_ref, sq, Jr, CoefStore<4, double>{coef}); break;
+    case 5: taylor_amplitudes_p<5, 1>(n, e_ref, sq, Jr, CoefStore<5, double>{coef}); break;
+    case 6: taylor_amplitudes_p<6, 1>(n, e_ref, sq, Jr, CoefStore<6, double>{coef}); break;
+    case 7: taylor_amplitudes_p<7, 1>(n, e_ref, sq, Jr, CoefStore<7, double>{coef}); break;
+    default: taylor_amplitudes_p<8, 1>(n, e_ref, sq, Jr, CoefStore<8, double>{coef}); break;
   }
 }
 

@@ -8,14 +8,16 @@
 //     the current one's kernels; lane 0 serves the single-waveform calls,
 //     batches spread their templates over AAK_LANES lanes (default 2) so that
 //     one template's narrow kernels overlap another's wide ones,
-//   * a rocFFT real-forward plan per lane (both channels in one batched call),
-//     the weighted-data buffers and the Bessel tables,
+//   * the inner-product stage: for power-of-two lengths the fused four-step
+//     FFT + weighted reduction (aak_fft.hip, data stored in its row order),
+//     otherwise a rocFFT real-forward plan per lane (both channels in one
+//     batched call) and the separate reduction; the Bessel tables,
 //   * for batches, a pool of host threads and a ring of plan slots whose knot
 //     tables live in pinned host memory.
 //
 // Per waveform a lane's stream runs, with no host round trip:
 //   clear counters -> (wait upload) rates -> Phidot guard -> exact folds ->
-//   fill -> mode sum [-> rFFT -> weighted reduction].
+//   fill -> mode sum [-> FFT + weighted reduction].
 // gpu_gen_AAK / Likelihood keep the reference's one-waveform-per-call
 // semantics; aak_likelihood_batch computes many likelihoods against the same
 // data with the host stages of later templates running on the pool while the
@@ -42,6 +44,7 @@
 #include "../../include/gpuAAK.h"
 #include "aak_bessel.hpp"
 #include "aak_dense.hpp"
+#include "aak_fft.hpp"
 #include "aak_geodesic.hpp"
 #include "aak_host.hpp"
 
@@ -143,6 +146,7 @@ struct Lane {
   int fold_nblk_cap = 0;
   void *d_foldmem = nullptr;
   double2 *d_hf = nullptr;    // [2][nbins]
+  double2 *d_fy = nullptr;    // [2][N/2] four-step FFT scratch (fused FFT path)
   double *d_partials = nullptr, *d_like = nullptr;
   double *d_ms_coef = nullptr;  // mode-sum block Taylor coefficients (grown with nmodes)
   size_t ms_coef_cap = 0;
@@ -164,8 +168,11 @@ struct aak_handle_s {
   Lane lane[kMaxLanes];
   int nlanes = 2;                 // lanes used by batches (lane 0 serves single calls)
   double *d_bessel = nullptr;     // c_m(k) table | log k!
-  double2 *d_data = nullptr;      // [2][nbins] weighted data
-  double *d_w = nullptr;          // [2][nbins] weights
+  double2 *d_data = nullptr;      // [2][nbins] weighted data (fused FFT path: in its row order)
+  double *d_w = nullptr;          // [2][nbins] weights (same order as d_data)
+  bool fused_fft = false;         // power-of-two length: fused FFT + reduction, else rocFFT
+  FftArgs fft{};
+  double2 *d_fft_tw = nullptr;    // twiddle tables of the fused FFT
   double *d_like_batch = nullptr;
   double *d_phase = nullptr;      // [8][N] phase output, allocated on first use
   size_t like_batch_cap = 0;
@@ -277,8 +284,14 @@ static int alloc_lane(aak_handle h, Lane &L) {
   L.fw.fallbacks = L.d_flags + 1;
   const int nb = h->nbins;
   HIPCHK(hipMalloc(&L.d_hf, sizeof(double2) * 2 * (size_t)nb));
-  HIPCHK(hipMalloc(&L.d_partials, sizeof(double) * 2 * 2048));
+  const int nparts = std::max(2048, h->fused_fft ? fft_partials(h->fft) : 0);
+  HIPCHK(hipMalloc(&L.d_partials, sizeof(double) * 2 * (size_t)nparts));
   HIPCHK(hipMalloc(&L.d_like, sizeof(double) * 2));
+  if (h->fused_fft) {
+    HIPCHK(hipMalloc(&L.d_fy, sizeof(double2) * 2 * (size_t)h->fft.M));
+    L.ready = true;
+    return 0;
+  }
   // rocFFT: two real sequences of length N (distance N+2) -> two half spectra
   std::call_once(g_fft_once, []() { rocfft_setup(); });
   rocfft_plan_description desc = nullptr;
@@ -313,7 +326,8 @@ static void free_lane(Lane &L) {
   if (L.plan) rocfft_plan_destroy(L.plan);
   if (L.exec) rocfft_execution_info_destroy(L.exec);
   void *ptrs[] = {L.d_table[0], L.d_table[1], L.d_dense, L.d_h, L.d_t, L.d_flags, L.d_guard, L.d_foldmem,
-                  L.d_hf, L.d_partials, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_list, L.d_ms_eref};
+                  L.d_hf, L.d_fy, L.d_partials, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_list,
+                  L.d_ms_eref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; k++) {
@@ -337,6 +351,42 @@ static int alloc_handle(aak_handle h) {
   const int nb = h->nbins;
   HIPCHK(hipMalloc(&h->d_data, sizeof(double2) * 2 * (size_t)nb));
   HIPCHK(hipMalloc(&h->d_w, sizeof(double) * 2 * (size_t)nb));
+  if (h->fused_fft) {
+    FftArgs &F = h->fft;
+    // W_L^e = exp(-2 pi i e / L) for L = R, C, and W_M, W_N split as
+    // W^e = W^(1024 (e >> 10)) W^(e & 1023), in extended precision
+    const int nhi = F.M / 1024 + 2;
+    const size_t o_R = 0, o_C = o_R + F.R, o_Mlo = o_C + F.C, o_Mhi = o_Mlo + 1024, o_Nlo = o_Mhi + nhi,
+                 o_Nhi = o_Nlo + 1024, total = o_Nhi + nhi;
+    std::vector<double2> tw(total);
+    auto root = [](long long e, long long L) {
+      const long double pi = 3.141592653589793238462643383279502884L;
+      const long double a = -2.0L * pi * (long double)(e % L) / (long double)L;
+      return make_double2((double)cosl(a), (double)sinl(a));
+    };
+    for (int e = 0; e < F.R; e++) tw[o_R + e] = root(e, F.R);
+    for (int e = 0; e < F.C; e++) tw[o_C + e] = root(e, F.C);
+    for (int e = 0; e < 1024; e++) {
+      tw[o_Mlo + e] = root(e, F.M);
+      tw[o_Nlo + e] = root(e, F.N);
+    }
+    for (int e = 0; e < nhi; e++) {
+      tw[o_Mhi + e] = root(1024LL * e, F.M);
+      tw[o_Nhi + e] = root(1024LL * e, F.N);
+    }
+    HIPCHK(hipMalloc(&h->d_fft_tw, sizeof(double2) * total));
+    HIPCHK(hipMemcpy(h->d_fft_tw, tw.data(), sizeof(double2) * total, hipMemcpyHostToDevice));
+    F.twR = h->d_fft_tw + o_R;
+    F.twC = h->d_fft_tw + o_C;
+    F.twMlo = h->d_fft_tw + o_Mlo;
+    F.twMhi = h->d_fft_tw + o_Mhi;
+    F.twNlo = h->d_fft_tw + o_Nlo;
+    F.twNhi = h->d_fft_tw + o_Nhi;
+    F.Dp[0] = h->d_data;
+    F.Dp[1] = h->d_data + nb;
+    F.wp[0] = h->d_w;
+    F.wp[1] = h->d_w + nb;
+  }
   return alloc_lane(h, h->lane[0]);
 }
 
@@ -359,7 +409,7 @@ static void free_handle(aak_handle h) {
   stop_workers(h);
   if (h->cstream) (void)hipStreamSynchronize(h->cstream);
   for (Lane &L : h->lane) free_lane(L);
-  void *ptrs[] = {h->d_bessel, h->d_data, h->d_w, h->d_like_batch, h->d_phase};
+  void *ptrs[] = {h->d_bessel, h->d_data, h->d_w, h->d_fft_tw, h->d_like_batch, h->d_phase};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   h->single_pin.release();
@@ -435,6 +485,15 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
 }
 
 static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_spectrum) {
+  if (h->fused_fft) {
+    launch_fft_pass1(h->fft, L.d_h, L.d_fy, L.stream);
+    h->mark(L, 4);
+    launch_fft_pass2(h->fft, L.d_fy, L.d_partials, keep_spectrum ? L.d_hf : nullptr, L.stream);
+    launch_dot_final(L.d_partials, fft_partials(h->fft), out_dev, L.stream);
+    h->mark(L, 5);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   void *in[1] = {L.d_h};
   void *outb[1] = {L.d_hf};
   if (rocfft_execute(L.plan, in, outb, L.exec) != rocfft_status_success) return fail(-101, "rocfft execute");
@@ -571,6 +630,18 @@ int aak_create(double T_fit, int init_length, int length, double init_dt, double
     const int v = std::atoi(env);
     h->nlanes = v < 1 ? 1 : (v > kMaxLanes ? kMaxLanes : v);
   }
+  {
+    // AAK_FFT=rocfft keeps the library transform for every length
+    const char *env = std::getenv("AAK_FFT");
+    int R = 0, C = 0;
+    if (!(env && std::strcmp(env, "rocfft") == 0) && fft_supported(length, &R, &C)) {
+      h->fused_fft = true;
+      h->fft.N = length;
+      h->fft.M = length / 2;
+      h->fft.R = R;
+      h->fft.C = C;
+    }
+  }
   const int rc = alloc_handle(h);
   if (rc) {
     std::string keep = g_err;
@@ -625,10 +696,32 @@ int aak_input_data(aak_handle h, const double *hI_f, const double *hII_f, const 
   if (len != h->nbins) return fail(-2, "input_data: len must equal length/2+1");
   const size_t nb = (size_t)h->nbins;
   hipStream_t st = h->lane[0].stream;
-  HIPCHK(hipMemcpyAsync(h->d_data, hI_f, sizeof(double2) * nb, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(h->d_data + nb, hII_f, sizeof(double2) * nb, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(h->d_w, ASDinv1, sizeof(double) * nb, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(h->d_w + nb, ASDinv2, sizeof(double) * nb, hipMemcpyHostToDevice, st));
+  if (h->fused_fft) {
+    // bin k = k1 + R k2 (k < M) goes to k1 C + k2, the Nyquist bin stays at M
+    const FftArgs &F = h->fft;
+    std::vector<double2> d(2 * nb);
+    std::vector<double> w(2 * nb);
+    const double2 *src_d[2] = {reinterpret_cast<const double2 *>(hI_f), reinterpret_cast<const double2 *>(hII_f)};
+    const double *src_w[2] = {ASDinv1, ASDinv2};
+    for (int ch = 0; ch < 2; ch++) {
+      double2 *dd = d.data() + ch * nb;
+      double *ww = w.data() + ch * nb;
+      for (int k1 = 0; k1 < F.R; k1++)
+        for (int k2 = 0; k2 < F.C; k2++) {
+          dd[(size_t)k1 * F.C + k2] = src_d[ch][k1 + (size_t)F.R * k2];
+          ww[(size_t)k1 * F.C + k2] = src_w[ch][k1 + (size_t)F.R * k2];
+        }
+      dd[F.M] = src_d[ch][F.M];
+      ww[F.M] = src_w[ch][F.M];
+    }
+    HIPCHK(hipMemcpyAsync(h->d_data, d.data(), sizeof(double2) * 2 * nb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(h->d_w, w.data(), sizeof(double) * 2 * nb, hipMemcpyHostToDevice, st));
+  } else {
+    HIPCHK(hipMemcpyAsync(h->d_data, hI_f, sizeof(double2) * nb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(h->d_data + nb, hII_f, sizeof(double2) * nb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(h->d_w, ASDinv1, sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(h->d_w + nb, ASDinv2, sizeof(double) * nb, hipMemcpyHostToDevice, st));
+  }
   HIPCHK(hipStreamSynchronize(st));
   h->have_data = 1;
   return 0;
@@ -1035,6 +1128,24 @@ int aak_test_taylor_amplitudes(int n_max, int n, double e_ref, double del, doubl
 
 int aak_test_inspiral_rates(double p, double e, double cosi, double a, double *out) {
   return inspiral_rates<HostCx>(p, e, cosi, a, out);
+}
+
+int aak_test_inner_product(aak_handle h, const double *hI, const double *hII, double *like, double *spec,
+                           int *fused) {
+  if (!h) return fail(-1, "null handle");
+  if (!h->have_data) return fail(-2, "inner product before input_data");
+  const size_t n = (size_t)h->N, ld = n + 2;
+  Lane &L = h->lane[0];
+  HIPCHK(hipMemcpyAsync(L.d_h, hI, sizeof(double) * n, hipMemcpyHostToDevice, L.stream));
+  HIPCHK(hipMemcpyAsync(L.d_h + ld, hII, sizeof(double) * n, hipMemcpyHostToDevice, L.stream));
+  const int rc = enqueue_likelihood(h, L, L.d_like, spec != nullptr);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(like, L.d_like, sizeof(double) * 2, hipMemcpyDeviceToHost, L.stream));
+  if (spec) HIPCHK(hipMemcpyAsync(spec, L.d_hf, sizeof(double2) * 2 * h->nbins, hipMemcpyDeviceToHost, L.stream));
+  HIPCHK(hipStreamSynchronize(L.stream));
+  h->state = 0;  // the time-domain buffers no longer hold a generated waveform
+  if (fused) *fused = h->fused_fft ? 1 : 0;
+  return 0;
 }
 
 int aak_test_get_dense(aak_handle h, double *out) {

@@ -174,5 +174,7 @@ void launch_clear_flags(const DenseBuffers &B, hipStream_t s);
 void launch_phase_out(const DenseParams &P, const DenseBuffers &B, double *out, hipStream_t s);
 void launch_weighted_dot(const double2 *hf, const double2 *data, const double *w, int nbins, double2 *hw_out,
                          double *partials, double *out, hipStream_t s);
+// 4 x the sums of nparts (dh, hh) pairs into out[0..1].
+void launch_dot_final(const double *partials, int nparts, double *out, hipStream_t s);
 
 }  // namespace aak

@@ -1,0 +1,31 @@
+// aak_fft.hpp -- fused real FFT + noise-weighted inner products (aak_fft.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace aak {
+
+// N = 2M real samples per channel, M = R C (powers of two).  Tables: W_R^e,
+// W_C^e (e < R, C), W_M^e = twMhi[e >> 10] twMlo[e & 1023] for e < M and
+// W_N^k = twNhi[k >> 10] twNlo[k & 1023] for k <= M.  Dp, wp: weighted data and
+// weights of each channel in the order of the second pass, position
+// k1 * C + k2 for bin k = k1 + R k2 (k < M), position M for the Nyquist bin.
+struct FftArgs {
+  int N, M, R, C;
+  const double2 *twR, *twC, *twMlo, *twMhi, *twNlo, *twNhi;
+  const double2 *Dp[2];
+  const double *wp[2];
+};
+
+// Lengths handled (powers of two, 2^8 <= N <= 2^23) and their R, C.
+bool fft_supported(int N, int *R, int *C);
+// Partial-sum pairs written by launch_fft_dot.
+int fft_partials(const FftArgs &F);
+// Both channels of h ([2][N+2]).  Pass 1 writes the 2M-complex scratch Y;
+// pass 2 writes fft_partials(F) pairs (sum of <h w, d w>, sum of |h w|^2) to
+// partials and, if hw_out is not null, the weighted spectra h~ w ([2][M+1],
+// natural order).
+void launch_fft_pass1(const FftArgs &F, const double *h, double2 *Y, hipStream_t s);
+void launch_fft_pass2(const FftArgs &F, const double2 *Y, double *partials, double2 *hw_out, hipStream_t s);
+
+}  // namespace aak

@@ -143,9 +143,11 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   B.rate[1][i] = phidot;
   B.rate[2][i] = (Ph - Th) / T / M;
   // amplitude of the mode sum (AAK.cc:481), with OmegaPhi of the prograde case
-  // from the maps already evaluated here
+  // from the maps already evaluated here; x^(2/3) as cbrt(x)^2 (within a few
+  // ulp of pow: the amplitude scales the waveform and feeds no phase)
   const double Om = (Y > 0) ? Ph / T / M : PN::omega_phi(v, e, Y, S, M);
-  B.amp[i] = pow(Om * P.Mphys * kMsunSec, 2. / 3.) * P.zeta;
+  const double cr = cbrt(Om * P.Mphys * kMsunSec);
+  B.amp[i] = (cr * cr) * P.zeta;
   if (i == 0) B.nu0[0] = R / T / (2. * kPi * M);
   if (i > P.i0 && i <= P.guard_last && !(phidot > 0.)) atomicAdd(&B.flags[0], 1);
 }
@@ -1160,16 +1162,13 @@ __device__ void setup_harmonics(const DenseParams &P, const DenseBuffers &B, int
   for (int n = threadIdx.x + 1; n <= P.nmodes; n += kSetupThreads) {
     // series length: the bound for orders n+1, n+2 covers the higher orders
     bessel_run<kSetupThreads>(n, PO, eref, B.bessel_c, B.bessel_lgf, P.bterms[n], sJ);
-    double coef[NC];
-    taylor_amplitudes_p<PO, kSetupThreads>(n, eref, sq, sJ, coef);
     const size_t base = (size_t)blk * P.nmodes * kCoefStride;
-    if (MIXED) {
-      float *dst = reinterpret_cast<float *>(B.ms_coef + base) + (size_t)(n - 1) * NC;
-      for (int q = 0; q < NC; q++) dst[q] = (float)coef[q];
-    } else {
-      double *dst = B.ms_coef + base + (size_t)(n - 1) * NC;
-      for (int q = 0; q < NC; q++) dst[q] = coef[q];
-    }
+    // each coefficient is stored as soon as it is formed
+    if (MIXED)
+      taylor_amplitudes_p<PO, kSetupThreads>(
+          n, eref, sq, sJ, CoefStore<PO, float>{reinterpret_cast<float *>(B.ms_coef + base) + (size_t)(n - 1) * NC});
+    else
+      taylor_amplitudes_p<PO, kSetupThreads>(n, eref, sq, sJ, CoefStore<PO, double>{B.ms_coef + base + (size_t)(n - 1) * NC});
   }
 }
 
@@ -1537,6 +1536,10 @@ __global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *partial
     out[0] = 4.0 * s_dh[0];
     out[1] = 4.0 * s_hh[0];
   }
+}
+
+void launch_dot_final(const double *partials, int nparts, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nparts, out);
 }
 
 void launch_weighted_dot(const double2 *hf, const double2 *data, const double *w, int nbins, double2 *hw_out,

@@ -149,6 +149,25 @@ class GPUAAK:
         keys = ("rates", "fold", "modesum", "rfft", "reduce")
         return dict(zip(keys, (float(x) for x in buf)))
 
+    def inner_product_stage(self, hI, hII):
+        """Runs the inner-product stage alone on two real series (test hook).
+
+        Returns ((<h|d>, <h|h>), weighted spectra [2][length/2+1] complex,
+        True if the fused FFT ran).  Invalidates the last waveform.
+        """
+        n = self.length
+        a = np.ascontiguousarray(hI, dtype=np.float64)
+        b = np.ascontiguousarray(hII, dtype=np.float64)
+        if a.shape != (n,) or b.shape != (n,):
+            raise ValueError("hI and hII must have the handle's length")
+        like = np.zeros(2)
+        spec = np.zeros(4 * (n // 2 + 1))
+        fused = np.zeros(1, dtype=np.int32)
+        _lib.check(_lib.lib().aak_test_inner_product(self._h, _lib.dptr(a), _lib.dptr(b), _lib.dptr(like),
+                                                     _lib.dptr(spec), _lib.iptr(fused)), "inner_product_stage")
+        spec = spec[0::2] + 1j * spec[1::2]
+        return (float(like[0]), float(like[1])), spec.reshape(2, n // 2 + 1), bool(fused[0])
+
     def dense_arrays(self):
         """Device dense arrays of the last waveform (test hook)."""
         L = self.length + 1

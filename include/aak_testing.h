@@ -45,9 +45,17 @@ int aak_test_taylor_amplitudes(int n_max, int n, double e_ref, double del, doubl
 /* One evaluation of the inspiral rates (dp/dt, de/dt, dcos(iota)/dt). */
 int aak_test_inspiral_rates(double p, double e, double cosi, double a, double *out);
 
+/* The inner-product stage alone on two given real series of length `length`
+ * (the handle's data and weights from aak_input_data): like[0..1] =
+ * (<h|d>, <h|h>) as aak_Likelihood returns them, spec (optional) =
+ * [2][length/2+1] weighted complex spectra, *fused (optional) = 1 when the
+ * fused FFT ran, 0 for the rocFFT path.  Invalidates the last waveform. */
+struct aak_handle_s;
+int aak_test_inner_product(struct aak_handle_s *h, const double *hI, const double *hII, double *like, double *spec,
+                           int *fused);
+
 /* Copies the dense arrays of the last aak_gpu_gen_AAK to the host:
  * out[10*(length+1)] = e | v | M | S | gimdot | Phidot | alpdot | gim | Phi | alp. */
-struct aak_handle_s;
 int aak_test_get_dense(struct aak_handle_s *h, double *out);
 
 #ifdef __cplusplus
