@@ -102,6 +102,7 @@ struct DenseParams {
   double t_plunge, t_zero;
   KnotGrid grid;
   int mixed;                      // 1: FP32 harmonic loop (mixed precision)
+  int store_vms;                  // 1: keep v, M, S per sample (read by no later stage)
   int bessel_series;              // 1: series + recurrence, 0: Miller from above
   unsigned char bterms[kMaxModes + 1];  // series length per harmonic ([0]: orders 0..4)
 };

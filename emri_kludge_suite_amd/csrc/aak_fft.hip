@@ -16,36 +16,111 @@
 // order (row k1, then k2), so every read is contiguous.  HBM traffic per
 // likelihood is about 2 x (16 + 16 + 16 + 16 + 8) MB at N = 2^21.
 //
-// Each FFT runs in LDS as a Stockham autosort transform (radix 4, one radix-2
-// stage when log2 of the length is odd); twiddles come from tables computed on
-// the host in extended precision.
+// Each FFT is a Stockham autosort transform of radix-16 stages (a radix-8,
+// -4 or -2 stage finishes lengths that are not powers of 16) on registers,
+// exchanging through LDS between stages; LDS rows carry one pad element per 16
+// so that the stride-16 writes of the first stage spread over all banks.
+// Twiddles come from tables computed on the host in extended precision.
 #include <hip/hip_runtime.h>
 
 #include "aak_fft.hpp"
 
 namespace aak {
 
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ inline double2 cmul_mi(double2 a) { return make_double2(a.y, -a.x); }  // a * (-i)
 
-// One Stockham stage of radix RS on L points held in x (LDS), NT threads.
+// cos(pi/8), sin(pi/8), sqrt(1/2)
+constexpr double kC8 = 0.92387953251128675613;
+constexpr double kS8 = 0.38268343236508977173;
+constexpr double kR2 = 0.70710678118654752440;
+
+__device__ inline void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3) {
+  const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmul_mi(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a1 = cadd(t1, t3);
+  a2 = csub(t0, t2);
+  a3 = csub(t1, t3);
+}
+
+// Forward DFT of RS points, v in natural order, o out in natural order.
+template <int RS>
+__device__ inline void dft(double2 *v, double2 *o) {
+  if constexpr (RS == 2) {
+    o[0] = cadd(v[0], v[1]);
+    o[1] = csub(v[0], v[1]);
+  } else if constexpr (RS == 4) {
+    dft4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int r = 0; r < 4; r++) o[r] = v[r];
+  } else if constexpr (RS == 8) {
+    // n = 2 n1 + n2, k = k1 + 4 k2
+    dft4(v[0], v[2], v[4], v[6]);
+    dft4(v[1], v[3], v[5], v[7]);
+    v[3] = cmul(v[3], make_double2(kR2, -kR2));
+    v[5] = cmul_mi(v[5]);
+    v[7] = cmul(v[7], make_double2(-kR2, -kR2));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+      o[k1] = cadd(v[2 * k1], v[2 * k1 + 1]);
+      o[k1 + 4] = csub(v[2 * k1], v[2 * k1 + 1]);
+    }
+  } else {
+    static_assert(RS == 16, "radix 2, 4, 8 or 16");
+    // n = 4 n1 + n2, k = k1 + 4 k2
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // v[n2 + 4 k1] *= W16^(n2 k1)
+    v[5] = cmul(v[5], make_double2(kC8, -kS8));
+    v[9] = cmul(v[9], make_double2(kR2, -kR2));
+    v[13] = cmul(v[13], make_double2(kS8, -kC8));
+    v[6] = cmul(v[6], make_double2(kR2, -kR2));
+    v[10] = cmul_mi(v[10]);
+    v[14] = cmul(v[14], make_double2(-kR2, -kR2));
+    v[7] = cmul(v[7], make_double2(kS8, -kC8));
+    v[11] = cmul(v[11], make_double2(-kR2, -kR2));
+    v[15] = cmul(v[15], make_double2(-kC8, kS8));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; k2++) o[k1 + 4 * k2] = v[4 * k1 + k2];
+  }
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+// radix of the stage that faces `rem` remaining points: 16 while that leaves a
+// power of 16 or at least 4 behind, then 8, 4 or 2
+constexpr int stage_radix(int rem) {
+  return (ilog2(rem) >= 4 && ilog2(rem) != 5) ? 16 : (ilog2(rem) == 5 ? 8 : rem);
+}
+
+__device__ inline int pad(int i) { return i + (i >> 4); }
+
+// One Stockham stage of radix RS on L points held in x (LDS, padded), NT threads.
 template <int L, int NT, int NS, int RS>
 __device__ inline void fft_stage(double2 *x, int t, const double2 *tw) {
-  constexpr int NB = L / RS;                  // butterflies
-  constexpr int PER = (NB + NT - 1) / NT;     // per thread
-  double2 v[PER][RS];
+  constexpr int NB = L / RS;
+  constexpr int PER = (NB + NT - 1) / NT;
+  double2 o[PER][RS];
 #pragma unroll
   for (int u = 0; u < PER; u++) {
     const int j = t + u * NT;
     if (NB % NT == 0 || j < NB) {
       const int jm = j & (NS - 1);
+      double2 v[RS];
 #pragma unroll
-      for (int r = 0; r < RS; r++) {
-        double2 a = x[j + r * NB];
-        if (NS > 1 && r > 0) a = cmul(a, tw[(r * jm * (L / (RS * NS))) & (L - 1)]);
-        v[u][r] = a;
+      for (int r = 0; r < RS; r++) v[r] = x[pad(j + r * NB)];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < RS; r++) v[r] = cmul(v[r], tw[(r * jm * (L / (RS * NS))) & (L - 1)]);
       }
+      dft<RS>(v, o[u]);
     }
   }
   __syncthreads();
@@ -53,22 +128,9 @@ __device__ inline void fft_stage(double2 *x, int t, const double2 *tw) {
   for (int u = 0; u < PER; u++) {
     const int j = t + u * NT;
     if (NB % NT == 0 || j < NB) {
-      const int jm = j & (NS - 1);
-      const int d = (j / NS) * NS * RS + jm;
-      if (RS == 4) {
-        const double2 a0 = make_double2(v[u][0].x + v[u][2].x, v[u][0].y + v[u][2].y);
-        const double2 a1 = make_double2(v[u][0].x - v[u][2].x, v[u][0].y - v[u][2].y);
-        const double2 a2 = make_double2(v[u][1].x + v[u][3].x, v[u][1].y + v[u][3].y);
-        // (v1 - v3) * (-i)
-        const double2 a3 = make_double2(v[u][1].y - v[u][3].y, v[u][3].x - v[u][1].x);
-        x[d] = make_double2(a0.x + a2.x, a0.y + a2.y);
-        x[d + NS] = make_double2(a1.x + a3.x, a1.y + a3.y);
-        x[d + 2 * NS] = make_double2(a0.x - a2.x, a0.y - a2.y);
-        x[d + 3 * NS] = make_double2(a1.x - a3.x, a1.y - a3.y);
-      } else {
-        x[d] = make_double2(v[u][0].x + v[u][1].x, v[u][0].y + v[u][1].y);
-        x[d + NS] = make_double2(v[u][0].x - v[u][1].x, v[u][0].y - v[u][1].y);
-      }
+      const int d = (j / NS) * NS * RS + (j & (NS - 1));
+#pragma unroll
+      for (int r = 0; r < RS; r++) x[pad(d + r * NS)] = o[u][r];
     }
   }
   __syncthreads();
@@ -77,17 +139,34 @@ __device__ inline void fft_stage(double2 *x, int t, const double2 *tw) {
 template <int L, int NT, int NS>
 __device__ inline void fft_stages(double2 *x, int t, const double2 *tw) {
   if constexpr (NS < L) {
-    constexpr int RS = ((L / NS) % 4 == 0) ? 4 : 2;
+    constexpr int RS = stage_radix(L / NS);
     fft_stage<L, NT, NS, RS>(x, t, tw);
     fft_stages<L, NT, NS * RS>(x, t, tw);
   }
 }
 
-// Forward L-point DFT of x in place (natural order in and out); tw = W_L^e.
-// Every thread of the block must call it (it synchronises the block).
+// Whole L-point FFT whose first stage (no twiddles) reads straight from global
+// memory: butterfly j reads src[j + r L / RS], contiguous across the threads
+// for each r.
 template <int L, int NT>
-__device__ inline void fft_lds(double2 *x, int t, const double2 *tw) {
-  fft_stages<L, NT, 1>(x, t, tw);
+__device__ inline void fft_first_global(const double2 *src, double2 *x, int t, const double2 *tw) {
+  constexpr int RS = stage_radix(L);
+  constexpr int NB = L / RS;
+  constexpr int PER = (NB + NT - 1) / NT;
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int j = t + u * NT;
+    if (NB % NT == 0 || j < NB) {
+      double2 v[RS], o[RS];
+#pragma unroll
+      for (int r = 0; r < RS; r++) v[r] = src[j + r * NB];
+      dft<RS>(v, o);
+#pragma unroll
+      for (int r = 0; r < RS; r++) x[pad(j * RS + r)] = o[r];
+    }
+  }
+  __syncthreads();
+  fft_stages<L, NT, RS>(x, t, tw);
 }
 
 __device__ inline double2 tw2(const double2 *hi, const double2 *lo, long long e) {
@@ -98,48 +177,64 @@ __device__ inline double2 tw2(const double2 *hi, const double2 *lo, long long e)
 // columns per workgroup: 8 (one 128-byte line per row) while 8 columns fit LDS
 template <int LR>
 constexpr int col_cols() { return LR <= 1024 ? 8 : 4; }
-constexpr int kColNT = 64;      // threads per column
+// LDS stride of a column: padded length, plus one so the columns of a row
+// start on different banks
+template <int LR>
+constexpr int col_stride() { return LR + LR / 16 + 1; }
+// threads per column or row: one radix-16 butterfly each in the first stage
+template <int L>
+constexpr int fft_threads() { return L / 16 > 64 ? L / 16 : 64; }
 
 template <int LR>
-__global__ __launch_bounds__(col_cols<LR>() * kColNT) void k_fft_cols(FftArgs F, const double *h, double2 *Y) {
-  constexpr int kColCols = col_cols<LR>();
-  constexpr int LP = LR + 2;
-  __shared__ double2 s[kColCols * LP];
-  const int ch = blockIdx.y, b0 = blockIdx.x * kColCols, tid = threadIdx.x;
+__global__ __launch_bounds__(col_cols<LR>() * fft_threads<LR>()) void k_fft_cols(FftArgs F, const double *h,
+                                                                                  double2 *Y) {
+  constexpr int CC = col_cols<LR>(), CS = col_stride<LR>(), NT = fft_threads<LR>();
+  __shared__ double2 s[CC * CS];
+  const int ch = blockIdx.y, b0 = blockIdx.x * CC, tid = threadIdx.x;
+  constexpr int PT = (LR + NT - 1) / NT;  // elements per thread in the staging loops
+  constexpr bool kFull = LR % NT == 0;
   const double2 *z = reinterpret_cast<const double2 *>(h + (size_t)ch * (F.N + 2));
-  for (int idx = tid; idx < kColCols * LR; idx += kColCols * kColNT) {
-    const int a = idx / kColCols, c = idx % kColCols;
-    s[c * LP + a] = z[(size_t)a * F.C + b0 + c];
+  {
+    // every load in flight before the first LDS store
+    double2 v[PT];
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      const int idx = tid + q * CC * NT;
+      if (kFull || idx < CC * LR) v[q] = z[(size_t)(idx / CC) * F.C + b0 + idx % CC];
+    }
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      const int idx = tid + q * CC * NT;
+      if (kFull || idx < CC * LR) s[(idx % CC) * CS + pad(idx / CC)] = v[q];
+    }
   }
   __syncthreads();
-  fft_lds<LR, kColNT>(s + (tid / kColNT) * LP, tid % kColNT, F.twR);
+  fft_stages<LR, NT, 1>(s + (tid / NT) * CS, tid % NT, F.twR);
   double2 *Yc = Y + (size_t)ch * F.M;
-  for (int idx = tid; idx < kColCols * LR; idx += kColCols * kColNT) {
-    const int k1 = idx / kColCols, c = idx % kColCols, b = b0 + c;
-    Yc[(size_t)k1 * F.C + b] = cmul(s[c * LP + k1], tw2(F.twMhi, F.twMlo, (long long)b * k1));
+#pragma unroll
+  for (int q = 0; q < PT; q++) {
+    const int idx = tid + q * CC * NT;
+    const int k1 = idx / CC, c = idx % CC, b = b0 + c;
+    if (kFull || idx < CC * LR)
+      Yc[(size_t)k1 * F.C + b] = cmul(s[c * CS + pad(k1)], tw2(F.twMhi, F.twMlo, (long long)b * k1));
   }
 }
 
 // ---------------------------------------------------------------- pass 2
-constexpr int kRowThreads = 256;
-
+// One workgroup per row pair, half of it per row: block 0 takes rows 0 and R/2
+// (each its own mirror), block p > 0 rows p and R - p.
 template <int LC>
-__global__ __launch_bounds__(kRowThreads) void k_fft_rows(FftArgs F, const double2 *Y, double *partials,
-                                                          double2 *hw_out) {
-  constexpr int LP = LC + 2;
-  constexpr int NT = kRowThreads / 2;
-  __shared__ double2 s[2 * LP];
-  __shared__ double s_red[2][kRowThreads / 64];
-  const int ch = blockIdx.y, p = blockIdx.x, tid = threadIdx.x;
-  const int A = p, B = (F.R - p) & (F.R - 1);
+__global__ __launch_bounds__(2 * fft_threads<LC>()) void k_fft_rows(FftArgs F, const double2 *Y, double *partials,
+                                                                    double2 *hw_out) {
+  constexpr int RSD = LC + LC / 16, NT = fft_threads<LC>();
+  __shared__ double2 s[2 * RSD];
+  __shared__ double s_red[2 * NT / 64][2];
+  const int ch = blockIdx.y, blk = blockIdx.x, tid = threadIdx.x;
+  const int half = tid / NT;
+  const int A = blk, B = blk == 0 ? F.R / 2 : F.R - blk;
   const double2 *Yc = Y + (size_t)ch * F.M;
-  for (int idx = tid; idx < 2 * LC; idx += kRowThreads) {
-    const int half = idx / LC, b = idx % LC;
-    s[half * LP + b] = Yc[(size_t)(half ? B : A) * F.C + b];
-  }
-  __syncthreads();
-  fft_lds<LC, NT>(s + (tid / NT) * LP, tid % NT, F.twC);
-  const double2 *sA = s, *sB = s + LP;
+  fft_first_global<LC, NT>(Yc + (size_t)(half ? B : A) * F.C, s + half * RSD, tid % NT, F.twC);
+  const double2 *sA = s, *sB = s + RSD;
   const double2 *Dp = F.Dp[ch];
   const double *wp = F.wp[ch];
   double2 *hw = hw_out ? hw_out + (size_t)ch * (F.M + 1) : nullptr;
@@ -158,8 +253,10 @@ __global__ __launch_bounds__(kRowThreads) void k_fft_rows(FftArgs F, const doubl
     hh += x.x * x.x + x.y * x.y;
     if (hw) hw[k] = x;
   };
-  for (int k2 = tid; k2 < LC; k2 += kRowThreads) {
-    if (A == 0) {
+  constexpr int KP = (LC + 2 * NT - 1) / (2 * NT);  // k2 values per thread
+  if (blk == 0) {
+    const int H = F.R / 2;
+    for (int k2 = tid; k2 < LC; k2 += 2 * NT) {
       if (k2 == 0) {
         // H_0 = E_0 + O_0 and H_M = E_0 - O_0, both real
         const double2 z0 = sA[0];
@@ -172,13 +269,18 @@ __global__ __launch_bounds__(kRowThreads) void k_fft_rows(FftArgs F, const doubl
           hw[F.M] = xm;
         }
       } else {
-        emit((long long)F.R * k2, sA[k2], sA[LC - k2], k2);
+        emit((long long)F.R * k2, sA[pad(k2)], sA[pad(LC - k2)], k2);
       }
-    } else if (A == B) {
-      emit(A + (long long)F.R * k2, sA[k2], sA[LC - 1 - k2], A * LC + k2);
-    } else {
-      emit(A + (long long)F.R * k2, sA[k2], sB[LC - 1 - k2], A * LC + k2);
-      emit(B + (long long)F.R * k2, sB[k2], sA[LC - 1 - k2], B * LC + k2);
+      emit(H + (long long)F.R * k2, sB[pad(k2)], sB[pad(LC - 1 - k2)], H * LC + k2);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < KP; q++) {
+      const int k2 = tid + q * 2 * NT;
+      if (LC % (2 * NT) == 0 || k2 < LC) {
+        emit(A + (long long)F.R * k2, sA[pad(k2)], sB[pad(LC - 1 - k2)], A * LC + k2);
+        emit(B + (long long)F.R * k2, sB[pad(k2)], sA[pad(LC - 1 - k2)], B * LC + k2);
+      }
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -186,17 +288,17 @@ __global__ __launch_bounds__(kRowThreads) void k_fft_rows(FftArgs F, const doubl
     hh += __shfl_xor(hh, o, 64);
   }
   if ((tid & 63) == 0) {
-    s_red[0][tid >> 6] = dh;
-    s_red[1][tid >> 6] = hh;
+    s_red[tid >> 6][0] = dh;
+    s_red[tid >> 6][1] = hh;
   }
   __syncthreads();
   if (tid == 0) {
     double a = 0.0, b = 0.0;
-    for (int w = 0; w < kRowThreads / 64; w++) {
-      a += s_red[0][w];
-      b += s_red[1][w];
+    for (int w = 0; w < 2 * NT / 64; w++) {
+      a += s_red[w][0];
+      b += s_red[w][1];
     }
-    const int q = ch * (F.R / 2 + 1) + p;
+    const int q = ch * (F.R / 2) + blk;
     partials[2 * q] = a;
     partials[2 * q + 1] = b;
   }
@@ -205,12 +307,13 @@ __global__ __launch_bounds__(kRowThreads) void k_fft_rows(FftArgs F, const doubl
 // ---------------------------------------------------------------- launch
 template <int LR>
 static void launch_cols(const FftArgs &F, const double *h, double2 *Y, hipStream_t s) {
-  hipLaunchKernelGGL(k_fft_cols<LR>, dim3(F.C / col_cols<LR>(), 2), dim3(col_cols<LR>() * kColNT), 0, s, F, h, Y);
+  hipLaunchKernelGGL(k_fft_cols<LR>, dim3(F.C / col_cols<LR>(), 2), dim3(col_cols<LR>() * fft_threads<LR>()), 0, s, F,
+                     h, Y);
 }
 
 template <int LC>
 static void launch_rows(const FftArgs &F, const double2 *Y, double *partials, double2 *hw, hipStream_t s) {
-  hipLaunchKernelGGL(k_fft_rows<LC>, dim3(F.R / 2 + 1, 2), dim3(kRowThreads), 0, s, F, Y, partials, hw);
+  hipLaunchKernelGGL(k_fft_rows<LC>, dim3(F.R / 2, 2), dim3(2 * fft_threads<LC>()), 0, s, F, Y, partials, hw);
 }
 
 bool fft_supported(int N, int *R, int *C) {
@@ -223,7 +326,7 @@ bool fft_supported(int N, int *R, int *C) {
   return true;
 }
 
-int fft_partials(const FftArgs &F) { return 2 * (F.R / 2 + 1); }
+int fft_partials(const FftArgs &F) { return F.R; }
 
 void launch_fft_pass1(const FftArgs &F, const double *h, double2 *Y, hipStream_t s) {
   switch (F.R) {

@@ -30,7 +30,7 @@ namespace aak {
 
 // ------------------------------------------------------------------ rates
 constexpr int kRatesThreads = 256;
-constexpr int kStageCap = 320;  // knot intervals staged in LDS per block
+constexpr int kStageCap = 128;  // knot intervals staged in LDS per block (else direct evaluation)
 
 __device__ inline int knot_search(const double *x, int lo, int hi, double t) {
   // largest k in [lo, hi-1] with x[k] <= t (hi is exclusive upper knot)
@@ -130,9 +130,11 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   }
   const double e = y[0], v = y[1], M = y[2], S = y[3];
   B.e[i] = e;
-  B.v[i] = v;
-  B.M[i] = M;
-  B.S[i] = S;
+  if (P.store_vms) {
+    B.v[i] = v;
+    B.M[i] = M;
+    B.S[i] = S;
+  }
   const double Y = P.coslam;
   const double T = PN::dtdm(v, e, Y, S);
   const double R = PN::drdm(v, e, Y, S);
@@ -274,12 +276,23 @@ static_assert(kFoldPerThread == 16, "fold_lds_index assumes runs of 16");
 constexpr int kFoldLds = kFoldBlock + kFoldBlock / kFoldPerThread;
 __device__ inline int fold_lds_index(int l) { return l + (l >> 4); }
 
+// Every load is issued before the first use: steps past the end read the last
+// step's rates (in bounds) and stage 0.
 __device__ inline void stage_increments(const FoldArgs &A, int p, int b, double *sd) {
-#pragma unroll 4
+  const double *rate = A.rate[p];
+  double r[kFoldPerThread], rm[kFoldPerThread];
+#pragma unroll
+  for (int q = 0; q < kFoldPerThread; q++) {
+    const int s = min(b * kFoldBlock + (int)threadIdx.x + q * kFoldThreads, A.nsteps - 1);
+    const int i = A.i0 + A.dir * s;
+    r[q] = rate[i];
+    rm[q] = rate[s == 0 ? i : i - A.dir];
+  }
+#pragma unroll
   for (int q = 0; q < kFoldPerThread; q++) {
     const int l = threadIdx.x + q * kFoldThreads;
-    const int s = b * kFoldBlock + l;
-    sd[fold_lds_index(l)] = (s < A.nsteps) ? fold_incr(A.rate[p], A.i0, A.dir, s, A.dt) : 0.0;
+    const double d = (1.5 * r[q] - .5 * rm[q]) * A.dt;
+    sd[fold_lds_index(l)] = (b * kFoldBlock + l < A.nsteps) ? (A.dir > 0 ? d : -d) : 0.0;
   }
   __syncthreads();
 }
@@ -602,7 +615,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
 #pragma unroll
     for (int j = 0; j < kFoldPerThread; j++) sd[fold_lds_index(tid * kFoldPerThread + j)] = outv[j];
     __syncthreads();
-#pragma unroll 4
+#pragma unroll
     for (int q = 0; q < kFoldPerThread; q++) {
       const int l = tid + q * kFoldThreads;
       const int s = b * kFoldBlock + l;
@@ -616,8 +629,9 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
 // single steps near binade boundaries.  The item and tie lists of all blocks
 // are gathered into LDS; consecutive runs on one binade grid without ties
 // form a segment whose total is a segmented int64 scan, so the one serial
-// lane only visits segment heads (binade changes, ties, single steps); the
-// run starts inside a segment are filled in parallel and re-checked against
+// lane only visits segment heads (binade changes, ties, single steps), each
+// packed beforehand into one walk record; the run starts inside a segment
+// are then filled in parallel, one item per lane, and re-checked against
 // their binade.  Lists too long for LDS take a serial walk over global
 // memory with the same arithmetic.
 constexpr int kChainThreads = 256;
@@ -654,41 +668,52 @@ __device__ inline bool chain_item(const FoldArgs &A, int p, int b, ItemRef I, Ti
   return true;
 }
 
-// exclusive scan of v[0..n) in place (n <= kChainMaxBlocks), returns the total
-__device__ inline int block_exclusive_scan(int *v, int n, int *s_tmp) {
+static_assert(kChainThreads == kFoldThreads, "the chain uses the fold's block scans");
+
+// One serial step of the chain's walk: a segment head (run or single step).
+struct ChainHead {
+  union {
+    long long R;  // kind 0: total of the segment's runs (ties rounded down)
+    double d;     // kind 1: the step's increment
+  };
+  int out;        // kind 1: output index of the step
+  short k, tie_first, nt;
+  signed char kind, sgn;
+};
+
+// Exclusive prefix of the per-thread chunk sums of v[0..n) (chunks of
+// ceil(n / kChainThreads) consecutive entries), written in place; returns the
+// total.  One wave-level scan and one block barrier pair.
+__device__ inline int chain_excl_scan(int *v, int n, int *sh) {
   const int tid = threadIdx.x;
   const int per = (n + kChainThreads - 1) / kChainThreads;
   const int lo = min(n, tid * per), hi = min(n, lo + per);
   int sum = 0;
   for (int j = lo; j < hi; j++) sum += v[j];
-  s_tmp[tid] = sum;
-  __syncthreads();
-  for (int o = 1; o < kChainThreads; o <<= 1) {
-    const int add = (tid >= o) ? s_tmp[tid - o] : 0;
-    __syncthreads();
-    s_tmp[tid] += add;
-    __syncthreads();
-  }
-  int run = s_tmp[tid] - sum;
+  int total;
+  int run = block_excl_sum(sum, sh, &total);
   for (int j = lo; j < hi; j++) {
     const int c = v[j];
     v[j] = run;
     run += c;
   }
-  const int total = s_tmp[kChainThreads - 1];
   __syncthreads();
   return total;
 }
 
 __global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWork W) {
   __shared__ int s_io[kChainMaxBlocks + 1], s_to[kChainMaxBlocks + 1];
-  __shared__ int s_tmp[kChainThreads];
-  __shared__ long long s_ltmp[kChainThreads];
-  __shared__ int s_ftmp[kChainThreads];
+  __shared__ int s_tmp[kChainThreads / 64];
+  __shared__ long long s_ltmp[2 * (kChainThreads / 64)];
+  __shared__ int s_ftmp[kChainThreads / 64];
   __shared__ FoldItem s_items[kChainMaxItems];
   __shared__ FoldTie s_ties[kChainMaxTies];
   __shared__ long long s_pref[kChainMaxItems];  // exclusive R prefix inside the segment
-  __shared__ int s_head[kChainMaxItems + 1];     // head flags, then the compacted head list
+  __shared__ int s_head[kChainMaxItems];        // head flags
+  __shared__ int s_seg[kChainMaxItems];         // item -> block (gather), then item -> segment
+  __shared__ int s_hlist[kChainMaxItems];       // segment -> head item
+  __shared__ long long s_mh[kChainMaxItems];    // segment -> exact start of the head run
+  __shared__ ChainHead s_hinfo[kChainMaxItems];
   __shared__ int s_nheads, s_ok;
   const int p = blockIdx.x;
   const int tid = threadIdx.x;
@@ -705,8 +730,8 @@ __global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWo
       s_to[b] = gnt[b];
     }
     __syncthreads();
-    TI = block_exclusive_scan(s_io, nblk, s_tmp);
-    TT = block_exclusive_scan(s_to, nblk, s_tmp);
+    TI = chain_excl_scan(s_io, nblk, s_tmp);
+    TT = chain_excl_scan(s_to, nblk, s_tmp);
     fits = TI <= kChainMaxItems && TT <= kChainMaxTies;
   }
   if (!fits) {
@@ -732,17 +757,21 @@ __global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWo
     s_to[nblk] = TT;
   }
   __syncthreads();
-  // gather: items get their block and a block-global tie offset
+  // block of every item (LDS only), and the ties of every block
   for (int b = tid; b < nblk; b += kChainThreads) {
-    const int io = s_io[b], ni = s_io[b + 1] - io, to = s_to[b];
-    for (int j = 0; j < ni; j++) {
-      FoldItem it = gitems[(size_t)b * kFoldMaxItems + j];
-      it.blk = b;
-      it.tie_first += to;
-      s_items[io + j] = it;
-    }
-    const int nt = s_to[b + 1] - to;
+    const int io = s_io[b], ni = s_io[b + 1] - io;
+    for (int j = 0; j < ni; j++) s_seg[io + j] = b;
+    const int to = s_to[b], nt = s_to[b + 1] - to;
     for (int j = 0; j < nt; j++) s_ties[to + j] = gties[(size_t)b * kFoldMaxTies + j];
+  }
+  __syncthreads();
+  // gather: independent loads, items get their block and a block-global tie offset
+  for (int i = tid; i < TI; i += kChainThreads) {
+    const int b = s_seg[i];
+    FoldItem it = gitems[(size_t)b * kFoldMaxItems + (i - s_io[b])];
+    it.blk = b;
+    it.tie_first += s_to[b];
+    s_items[i] = it;
   }
   __syncthreads();
   // tie count of item i: up to the next item's first tie, or the block's end
@@ -763,109 +792,124 @@ __global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWo
     s_head[i] = h;
   }
   __syncthreads();
-  // segmented exclusive scan of R over items (per-thread chunks, then lanes)
+  const int per = (TI + kChainThreads - 1) / kChainThreads;
+  const int lo = min(TI, tid * per), hi = min(TI, lo + per);
   {
-    const int per = (TI + kChainThreads - 1) / kChainThreads;
-    const int lo = min(TI, tid * per), hi = min(TI, lo + per);
+    // segmented exclusive scan of R over items: chunk summaries, then lanes
     long long tail = 0;
-    int has = 0;
+    int has = 0, cnt = 0;
     for (int j = lo; j < hi; j++) {
       if (s_head[j]) {
         has = 1;
         tail = 0;
+        cnt++;
       }
       tail += s_items[j].kind == 0 ? s_items[j].R : 0;
     }
-    s_ltmp[tid] = tail;
-    s_ftmp[tid] = has;
+    const long long incl = block_seg_incl(tail, has, s_ltmp, s_ftmp);
+    long long carry = __shfl_up(incl, 1, 64);
+    if ((tid & 63) == 63) s_ltmp[kChainThreads / 64 + (tid >> 6)] = incl;
     __syncthreads();
-    for (int off = 1; off < kChainThreads; off <<= 1) {
-      long long v = 0;
-      int f = 0;
-      if (tid >= off) {
-        v = s_ltmp[tid - off];
-        f = s_ftmp[tid - off];
-      }
-      __syncthreads();
-      if (tid >= off && !s_ftmp[tid]) {
-        s_ltmp[tid] += v;
-        s_ftmp[tid] |= f;
-      }
-      __syncthreads();
-    }
-    long long run = (tid > 0) ? s_ltmp[tid - 1] : 0;
+    if ((tid & 63) == 0) carry = (tid > 0) ? s_ltmp[kChainThreads / 64 + (tid >> 6) - 1] : 0;
+    int nheads;
+    int pos = block_excl_sum(cnt, s_tmp, &nheads);
+    long long run = carry;
     for (int j = lo; j < hi; j++) {
-      if (s_head[j]) run = 0;
+      if (s_head[j]) {
+        run = 0;
+        s_hlist[pos++] = j;
+      }
       s_pref[j] = run;
+      s_seg[j] = pos - 1;
       run += s_items[j].kind == 0 ? s_items[j].R : 0;
     }
+    if (tid == 0) s_nheads = nheads;
   }
   __syncthreads();
-  // compact the head list (flags -> indices)
-  {
-    const int per = (TI + kChainThreads - 1) / kChainThreads;
-    const int lo = min(TI, tid * per), hi = min(TI, lo + per);
-    int cnt = 0;
-    for (int j = lo; j < hi; j++) cnt += s_head[j];
-    s_tmp[tid] = cnt;
-    __syncthreads();
-    for (int o = 1; o < kChainThreads; o <<= 1) {
-      const int add = (tid >= o) ? s_tmp[tid - o] : 0;
-      __syncthreads();
-      s_tmp[tid] += add;
-      __syncthreads();
+  const int nh = s_nheads;
+  // one walk record per segment
+  for (int q = tid; q < nh; q += kChainThreads) {
+    const int h = s_hlist[q];
+    const int last = ((q + 1 < nh) ? s_hlist[q + 1] : TI) - 1;
+    const FoldItem &I = s_items[h];
+    ChainHead H;
+    H.kind = (signed char)I.kind;
+    H.sgn = (signed char)I.sgn;
+    H.k = (short)I.k;
+    H.tie_first = (short)I.tie_first;
+    H.nt = (short)nties_of(h);
+    if (I.kind == 1) {
+      H.d = I.d;
+      H.out = out_index(A, I.blk * kFoldBlock + I.first);
+    } else {
+      H.R = (last > h) ? s_pref[last] + s_items[last].R : I.R;
+      H.out = 0;
     }
-    int pos = s_tmp[tid] - cnt;
-    int flags[8];
-    const int nloc = hi - lo;
-    for (int j = 0; j < nloc && j < 8; j++) flags[j] = s_head[lo + j];
-    __syncthreads();
-    for (int j = 0; j < nloc; j++) {
-      const int f = (j < 8) ? flags[j] : 0;
-      if (f) s_head[pos++] = lo + j;
-    }
-    if (tid == kChainThreads - 1) s_nheads = s_tmp[kChainThreads - 1];
+    s_hinfo[q] = H;
   }
   __syncthreads();
-  // serial walk over the segment heads
+  // serial walk over the segment heads; the next record is read ahead
   if (tid == 0) {
-    const int nh = s_nheads;
     double x = A.x0[p];
     A.out[p][A.i0] = x;
     int ok = 1;
-    for (int q = 0; q < nh && ok; q++) {
-      const int h = s_head[q];
-      const int last = ((q + 1 < nh) ? s_head[q + 1] : TI) - 1;
-      FoldItem &I = s_items[h];
-      // the segment's later runs add their totals to the head's end value
-      // (s_pref is the exclusive prefix from the head, so it includes R_h)
-      const long long extra = (last > h) ? s_pref[last] + s_items[last].R - I.R : 0;
-      ok = chain_item<FoldItem &>(A, p, I.blk, I, s_ties + I.tie_first, nties_of(h), x, extra) ? 1 : 0;
+    ChainHead Hn = s_hinfo[0];
+    for (int q = 0; q < nh; q++) {
+      const ChainHead H = Hn;
+      if (q + 1 < nh) Hn = s_hinfo[q + 1];
+      if (H.kind == 1) {
+        x = x + H.d;
+        A.out[p][H.out] = x;
+        s_mh[q] = 0;
+        continue;
+      }
+      int kx;
+      frexp(x, &kx);
+      if (kx != H.k || (x > 0.0 ? 1 : -1) != H.sgn) {
+        ok = 0;
+        break;
+      }
+      const long long m = (long long)ldexp(x, 53 - H.k);
+      s_mh[q] = m;
+      long long ups = 0;
+      for (int t = 0; t < H.nt; t++) {
+        FoldTie &T = s_ties[H.tie_first + t];
+        const int up = (int)((m + T.P + ups) & 1LL);
+        T.up = up;
+        ups += up;
+      }
+      const long long mend = m + H.R + ups;
+      const long long am = mend < 0 ? -mend : mend;
+      if (am < (1LL << 52) || am >= (1LL << 53)) {
+        ok = 0;
+        break;
+      }
+      x = ldexp((double)mend, H.k - 53);
     }
     s_ok = ok;
     if (!ok) W.status[p] = 1;
   }
   __syncthreads();
   if (!s_ok) return;
-  // run starts inside segments, re-checked against their binade
-  const int nh = s_nheads;
+  // exact start of every run, re-checked against its binade
   int bad = 0;
-  for (int q = tid; q < nh; q += kChainThreads) {
-    const int h = s_head[q];
-    const int last = ((q + 1 < nh) ? s_head[q + 1] : TI) - 1;
-    const long long mh = s_items[h].m_start;
-    for (int i = h + 1; i <= last; i++) {
-      const long long m = mh + s_pref[i];
+  for (int i = tid; i < TI; i += kChainThreads) {
+    const int q = s_seg[i];
+    long long m = s_mh[q];
+    if (i > s_hlist[q]) {
+      m += s_pref[i];
       const long long am = m < 0 ? -m : m;
       if (am < (1LL << 52) || am >= (1LL << 53)) bad = 1;
-      s_items[i].m_start = m;
     }
+    s_items[i].m_start = m;
   }
   if (bad) W.status[p] = 1;
   __syncthreads();
+  for (int i = tid; i < TI; i += kChainThreads) {
+    const int b = s_items[i].blk;
+    gitems[(size_t)b * kFoldMaxItems + (i - s_io[b])].m_start = s_items[i].m_start;
+  }
   for (int b = tid; b < nblk; b += kChainThreads) {
-    const int io = s_io[b], ni = s_io[b + 1] - io;
-    for (int j = 0; j < ni; j++) gitems[(size_t)b * kFoldMaxItems + j].m_start = s_items[io + j].m_start;
     const int to = s_to[b], nt = s_to[b + 1] - to;
     for (int j = 0; j < nt; j++) gties[(size_t)b * kFoldMaxTies + j].up = s_ties[to + j].up;
   }

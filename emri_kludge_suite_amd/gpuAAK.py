@@ -136,17 +136,22 @@ class GPUAAK:
         _lib.check(_lib.lib().aak_set_profiling(self._h, int(bool(on))), "set_profiling")
 
     def timings(self):
-        """Stage times (ms) of the last gpu_gen_AAK / Likelihood."""
+        """Stage times (ms) of the last gpu_gen_AAK / Likelihood.
+
+        fft / dot: for power-of-two lengths the column pass of the fused FFT,
+        then the row pass with the weighting and both sums; otherwise the
+        rocFFT transform, then the weighted reduction.
+        """
         buf = np.zeros(6)
         _lib.lib().aak_get_timings(self._h, _lib.dptr(buf), 6)
-        keys = ("host", "rates", "fold", "modesum", "rfft", "reduce")
+        keys = ("host", "rates", "fold", "modesum", "fft", "dot")
         return dict(zip(keys, (float(x) for x in buf)))
 
     def batch_timings(self):
         """Stage time sums (ms) over the last LikelihoodBatch run with profiling on."""
         buf = np.zeros(5)
         _lib.check(_lib.lib().aak_get_batch_timings(self._h, _lib.dptr(buf), 5), "batch_timings")
-        keys = ("rates", "fold", "modesum", "rfft", "reduce")
+        keys = ("rates", "fold", "modesum", "fft", "dot")
         return dict(zip(keys, (float(x) for x in buf)))
 
     def inner_product_stage(self, hI, hII):

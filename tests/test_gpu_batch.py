@@ -109,7 +109,7 @@ def test_batch_profiled_stage_times():
     g.LikelihoodBatch(P)
     tm = g.batch_timings()
     print(tm, flush=True)
-    assert tm["modesum"] > 0 and tm["rates"] > 0 and tm["rfft"] > 0
+    assert tm["modesum"] > 0 and tm["rates"] > 0 and tm["fft"] > 0 and tm["dot"] > 0
 
 
 @pytest.mark.timeout(400)
