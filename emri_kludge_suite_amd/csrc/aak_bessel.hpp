@@ -239,12 +239,18 @@ AAK_HD void taylor_eval(const double *coef, int p, double d, double *abg) {
   abg[2] = ga;
 }
 
-// Taylor order for harmonics up to n_max and |delta| <= del (-1: none <= kTaylorMax).
-AAK_HD int taylor_order(int n_max, double del) {
+// Remainder bounds of the expansion: FP64 sums, and FP32 sums (whose
+// coefficients and arithmetic carry relative errors near 6e-8 anyway).
+constexpr double kTaylorTolFp64 = 1e-17;
+constexpr double kTaylorTolFp32 = 1e-9;
+
+// Taylor order for harmonics up to n_max and |delta| <= del with remainder
+// bound at most tol (-1: none <= kTaylorMax).
+AAK_HD int taylor_order(int n_max, double del, double tol = kTaylorTolFp64) {
   const double nm = (double)n_max, z = nm * del;
   double term = 7.0 * nm * z;  // 7 n (n delta)^{p+1} / (p+1)! at p = 0
   for (int q = 0; q <= kTaylorMax; q++) {
-    if (term <= 1e-17) return q;
+    if (term <= tol) return q;
     term = term * z / (q + 2);
   }
   return -1;

@@ -138,7 +138,7 @@ struct Lane {
   hipEvent_t ev_upload[2] = {nullptr, nullptr}, ev_rates[2] = {nullptr, nullptr};
   bool rates_recorded[2] = {false, false};
   double *d_dense = nullptr;  // e, v, M, S, rate[3], phase[3], amp: 11 x (N+1), then nu0
-  double *d_h = nullptr;      // [2][N+2]
+  double *d_h = nullptr;      // hI | hII, N+2 samples each at channel stride h_stride(N)
   double *d_t = nullptr;      // [N+2]
   int *d_flags = nullptr;     // [4]: non-positive Phidot count, fold fallbacks
   int *d_guard = nullptr;
@@ -226,6 +226,7 @@ struct aak_handle_s {
     B.amp = ln.d_dense + 10 * L;
     B.nu0 = ln.d_dense + 11 * L;
     B.h = ln.d_h;
+    B.h2 = ln.d_h + h_stride(N);
     B.t = ln.d_t;
     B.flags = ln.d_flags;
     B.guard_blk = ln.d_guard;
@@ -248,7 +249,7 @@ static int alloc_lane(aak_handle h, Lane &L) {
     HIPCHK(hipEventCreateWithFlags(&L.ev_rates[k], hipEventDisableTiming));
   }
   HIPCHK(hipMalloc(&L.d_dense, sizeof(double) * (11 * ((size_t)N + 1) + 1)));
-  HIPCHK(hipMalloc(&L.d_h, sizeof(double) * 2 * ((size_t)N + 2)));
+  HIPCHK(hipMalloc(&L.d_h, sizeof(double) * 2 * h_stride(N)));
   HIPCHK(hipMalloc(&L.d_t, sizeof(double) * ((size_t)N + 2)));
   HIPCHK(hipMalloc(&L.d_flags, sizeof(int) * 4));
   HIPCHK(hipMalloc(&L.d_guard, sizeof(int) * ((size_t)N / 4096 + 4)));
@@ -297,7 +298,7 @@ static int alloc_lane(aak_handle h, Lane &L) {
   rocfft_plan_description desc = nullptr;
   if (rocfft_plan_description_create(&desc) != rocfft_status_success) return fail(-101, "rocfft description");
   const size_t in_stride = 1, out_stride = 1;
-  const size_t in_dist = (size_t)N + 2, out_dist = (size_t)nb;
+  const size_t in_dist = h_stride(N), out_dist = (size_t)nb;
   if (rocfft_plan_description_set_data_layout(desc, rocfft_array_type_real, rocfft_array_type_hermitian_interleaved,
                                               nullptr, nullptr, 1, &in_stride, in_dist, 1, &out_stride,
                                               out_dist) != rocfft_status_success) {
@@ -640,6 +641,7 @@ int aak_create(double T_fit, int init_length, int length, double init_dt, double
       h->fft.M = length / 2;
       h->fft.R = R;
       h->fft.C = C;
+      h->fft.hs = (int)h_stride(length);
     }
   }
   const int rc = alloc_handle(h);
@@ -969,7 +971,7 @@ int aak_get_waveform(aak_handle h, double *t, double *hI, double *hII) {
     HIPCHK(hipMemcpyAsync(hII, L.d_hf + h->nbins, sizeof(double2) * h->nbins, hipMemcpyDeviceToHost, L.stream));
   } else {
     HIPCHK(hipMemcpyAsync(hI, L.d_h, sizeof(double) * n, hipMemcpyDeviceToHost, L.stream));
-    HIPCHK(hipMemcpyAsync(hII, L.d_h + n, sizeof(double) * n, hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(hipMemcpyAsync(hII, L.d_h + h_stride(h->N), sizeof(double) * n, hipMemcpyDeviceToHost, L.stream));
   }
   HIPCHK(hipStreamSynchronize(L.stream));
   return 0;
@@ -1135,7 +1137,7 @@ int aak_test_inner_product(aak_handle h, const double *hI, const double *hII, do
                            int *fused) {
   if (!h) return fail(-1, "null handle");
   if (!h->have_data) return fail(-2, "inner product before input_data");
-  const size_t n = (size_t)h->N, ld = n + 2;
+  const size_t n = (size_t)h->N, ld = h_stride(h->N);
   Lane &L = h->lane[0];
   HIPCHK(hipMemcpyAsync(L.d_h, hI, sizeof(double) * n, hipMemcpyHostToDevice, L.stream));
   HIPCHK(hipMemcpyAsync(L.d_h + ld, hII, sizeof(double) * n, hipMemcpyHostToDevice, L.stream));

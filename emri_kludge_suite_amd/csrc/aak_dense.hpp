@@ -8,6 +8,10 @@
 
 namespace aak {
 
+// Channel stride of the time-domain buffer: N+2 samples rounded up to 128
+// bytes, so that both channels start on a cache line.
+AAK_HD size_t h_stride(int N) { return ((size_t)N + 2 + 15) & ~(size_t)15; }
+
 // Device knot table: y of e, v, M, S (4K) | their spline coefficients c (4K) |
 // head knot times (KnotGrid::steps).
 constexpr int kTableArrays = 8;
@@ -150,7 +154,7 @@ struct DenseBuffers {
   double *phase[3];                // gim, Phi, alp [N+1]
   double *amp;                     // [N+1] mode-sum amplitude (Omega_phi M Msun)^(2/3) zeta
   double *nu0;                     // [1] radial frequency at sample 0 from the PN maps
-  double *h;                       // [2][N+2] time domain hI | hII
+  double *h, *h2;                  // time domain hI, hII: N+2 samples each, h2 = h + h_stride(N)
   double *t;                       // [N+2]
   int *flags;                      // [4]: [0] non-positive Phidot count, [1] fold replays, [2] per-sample mode-sum blocks
   int *guard_blk;                  // [guard blocks]: last positive-Phidot index per block

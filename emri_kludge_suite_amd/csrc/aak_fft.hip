@@ -193,7 +193,7 @@ __global__ __launch_bounds__(col_cols<LR>() * fft_threads<LR>()) void k_fft_cols
   const int ch = blockIdx.y, b0 = blockIdx.x * CC, tid = threadIdx.x;
   constexpr int PT = (LR + NT - 1) / NT;  // elements per thread in the staging loops
   constexpr bool kFull = LR % NT == 0;
-  const double2 *z = reinterpret_cast<const double2 *>(h + (size_t)ch * (F.N + 2));
+  const double2 *z = reinterpret_cast<const double2 *>(h + (size_t)ch * F.hs);
   {
     // every load in flight before the first LDS store
     double2 v[PT];

@@ -12,6 +12,7 @@ namespace aak {
 // k1 * C + k2 for bin k = k1 + R k2 (k < M), position M for the Nyquist bin.
 struct FftArgs {
   int N, M, R, C;
+  int hs;  // channel stride of the time-domain input (doubles, even)
   const double2 *twR, *twC, *twMlo, *twMhi, *twNlo, *twNhi;
   const double2 *Dp[2];
   const double *wp[2];
@@ -19,9 +20,9 @@ struct FftArgs {
 
 // Lengths handled (powers of two, 2^8 <= N <= 2^23) and their R, C.
 bool fft_supported(int N, int *R, int *C);
-// Partial-sum pairs written by launch_fft_dot.
+// Partial-sum pairs written by launch_fft_pass2.
 int fft_partials(const FftArgs &F);
-// Both channels of h ([2][N+2]).  Pass 1 writes the 2M-complex scratch Y;
+// Both channels of h (channel stride hs).  Pass 1 writes the 2M-complex scratch Y;
 // pass 2 writes fft_partials(F) pairs (sum of <h w, d w>, sum of |h w|^2) to
 // partials and, if hw_out is not null, the weighted spectra h~ w ([2][M+1],
 // natural order).

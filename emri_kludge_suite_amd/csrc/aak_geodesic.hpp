@@ -321,17 +321,34 @@ struct CircularOrbit {
       Q = qfunc(r, a, E, 0.0, D);
       Lovercosi = sqrt(Q);
     } else {
+      // Bisection on E (E = (hi + lo) / 2 until |cos iota(E) - ci| < 1e-9),
+      // two levels per round: the midpoint and both of its possible
+      // successors are evaluated side by side (independent dependency
+      // chains), then the two decisions are applied in order, so every
+      // midpoint, comparison and exit is the one-level loop's.
       double lo = Kerr::e_pro(r, a);
       double hi = (r >= Kerr::isco_ret(a)) ? Kerr::e_ret(r, a) : 1.;
-      for (int it = 0;; it++) {
-        E = 0.5 * (hi + lo);
-        Lz = lzfunc(r, a, E, D);
-        Q = qfunc(r, a, E, Lz, D);
-        const double cg = Lz / sqrt(Lz * Lz + Q);
-        if (cg > ci) lo = E;
-        else hi = E;
-        if (fabs(cg - ci) < 1.e-9) break;
+      auto cosi_of = [&](double e) {
+        const double lz = lzfunc(r, a, e, D);
+        const double q = qfunc(r, a, e, lz, D);
+        return lz / sqrt(lz * lz + q);
+      };
+      for (int it = 0;; it += 2) {
+        const double E0 = 0.5 * (hi + lo);
+        const double Ea = 0.5 * (hi + E0), Eb = 0.5 * (E0 + lo);
+        const double c0 = cosi_of(E0), ca = cosi_of(Ea), cb = cosi_of(Eb);
+        E = E0;
+        if (c0 > ci) lo = E0;
+        else hi = E0;
+        if (fabs(c0 - ci) < 1.e-9) break;
         if (it > 4000) return -1;
+        const bool up = c0 > ci;  // lo moved: the next midpoint is Ea
+        E = up ? Ea : Eb;
+        const double c1 = up ? ca : cb;
+        if (c1 > ci) lo = E;
+        else hi = E;
+        if (fabs(c1 - ci) < 1.e-9) break;
+        if (it + 1 > 4000) return -1;
       }
     }
     Lz = lzfunc(r, a, E, D);

@@ -1179,7 +1179,8 @@ __device__ inline double harmonic_phase(const DenseParams &P, const SampleGeom &
 // Inside one block of samples e varies by a few 1e-5: the amplitude factors
 // alpha_n, beta_n, gamma_n are expanded in delta = e - e_ref once per block
 // (k_modesum_setup) and evaluated per sample by Horner (k_modesum).  The order
-// p is the smallest with remainder bound 7 n (n delta)^{p+1}/(p+1)! <= 1e-17.
+// p is the smallest with remainder bound 7 n (n delta)^{p+1}/(p+1)! <= 1e-17
+// (1e-9 for the FP32 harmonic loop of the mixed form).
 // Blocks where no order up to kTaylorMax qualifies, or where the Bessel series
 // is not used (e > kSeriesMaxE, more than kTaylorModes harmonics), are summed
 // per sample by k_modesum_direct.
@@ -1246,7 +1247,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, 
     p = 0;
   } else if (P.bessel_series && P.nmodes <= kTaylorModes && hi <= kSeriesMaxE) {
     eref = 0.5 * (lo + hi);
-    p = taylor_order(P.nmodes, 0.5 * (hi - lo));
+    p = taylor_order(P.nmodes, 0.5 * (hi - lo), MIXED ? kTaylorTolFp32 : kTaylorTolFp64);
   }
   if (lane == 0) {
     B.ms_p[blk] = p;
@@ -1295,35 +1296,25 @@ __device__ inline void phase_seed(const DenseParams &P, const SampleGeom &G, Pha
   sincos(S.dB, &S.qi, &S.qr);
 }
 
-// a_0 + sum_{j=1..PO} a_j d^j with the powers dp[j] = d^j shared by all
-// harmonics; the a_j come from SGPRs, so every term is one FMA with a scalar
-// operand (the highest power first).
-template <int PO, typename T, typename CP>
-__device__ inline T poly_pow(CP c, const T *dp) {
-  if (PO == 0) return c[0];
-  T x = c[PO] * dp[PO];
-#pragma unroll
-  for (int j = PO - 1; j >= 1; j--) x = fma(c[j], dp[j], x);
-  return x + c[0];
-}
-
 // FP64 harmonic loop at Taylor order PO.  The response factors are the same
 // for every harmonic, so only the three sums
 //   Sa = sum cos(s_n) alpha_n, Sg = sum cos(s_n) gamma_n, Sb = sum sin(s_n) beta_n
-// are accumulated.  The rotation gives exp(i theta), theta = n Phi + Doppler_n;
-// the reference's argument is its rounded s_n = rn(rn(n Phi) + Doppler_n), which
-// differs from theta by a few ulp of s_n (~1e-9 rad late in a year-long
-// inspiral).  E = theta - s_n = fma(n, Phi, -s_n) + Doppler_n, accurate to the
-// rounding of Doppler_n (~1e-14 rad), and exp(i s_n) = exp(i theta) (1 - i E)
-// to second order in E.
+// are needed; with alpha_n = sum_j a_nj d^j (and likewise for beta, gamma)
+// they are accumulated per power, Sa_j = sum_n cos(s_n) a_nj, one FMA with a
+// scalar operand per coefficient, and combined with d^j after the loop.  The
+// rotation gives exp(i theta), theta = n Phi + Doppler_n; the reference's
+// argument is its rounded s_n = rn(rn(n Phi) + Doppler_n), which differs from
+// theta by a few ulp of s_n (~1e-9 rad late in a year-long inspiral).
+// E = theta - s_n = fma(n, Phi, -s_n) + Doppler_n, accurate to the rounding of
+// Doppler_n (~1e-14 rad), and exp(i s_n) = exp(i theta) (1 - i E) to second
+// order in E.
 template <int PO>
 __device__ inline void harmonics_fp64(int nmodes, cdptr cf, double Phi, double d, const PhaseSeed &S, double *sum) {
   constexpr int NC = 3 * (PO + 1);
-  double dp[PO + 1];
-  dp[0] = 1.0;
+  double qr = S.qr, qi = S.qi;
+  double Sa[PO + 1], Sb[PO + 1], Sg[PO + 1];
 #pragma unroll
-  for (int j = 1; j <= PO; j++) dp[j] = dp[j - 1] * d;
-  double qr = S.qr, qi = S.qi, Sa = 0.0, Sb = 0.0, Sg = 0.0;
+  for (int j = 0; j <= PO; j++) Sa[j] = Sb[j] = Sg[j] = 0.0;
   const double ur = S.ur, ui = S.ui;
 #pragma unroll 2
   for (int nn = 1; nn <= nmodes; nn++, cf += NC) {
@@ -1335,50 +1326,62 @@ __device__ inline void harmonics_fp64(int nmodes, cdptr cf, double Phi, double d
     const double s_ = dn * Phi + D;  // not contracted: rn(rn(n Phi) + D)
     const double E = fma(dn, Phi, -s_) + D;
     const double cnp = fma(E, qi, qr), snp = fma(-E, qr, qi);
-    const double al = poly_pow<PO>(cf, dp);
-    const double be = poly_pow<PO>(cf + (PO + 1), dp);
-    const double ga = poly_pow<PO>(cf + 2 * (PO + 1), dp);
-    Sa = fma(cnp, al, Sa);
-    Sg = fma(cnp, ga, Sg);
-    Sb = fma(snp, be, Sb);
+#pragma unroll
+    for (int j = 0; j <= PO; j++) {
+      Sa[j] = fma(cnp, cf[j], Sa[j]);
+      Sb[j] = fma(snp, cf[PO + 1 + j], Sb[j]);
+      Sg[j] = fma(cnp, cf[2 * (PO + 1) + j], Sg[j]);
+    }
   }
-  sum[0] = Sa;
-  sum[1] = Sb;
-  sum[2] = Sg;
+  double a = Sa[PO], b = Sb[PO], g = Sg[PO];
+#pragma unroll
+  for (int j = PO - 1; j >= 0; j--) {
+    a = fma(a, d, Sa[j]);
+    b = fma(b, d, Sb[j]);
+    g = fma(g, d, Sg[j]);
+  }
+  sum[0] = a;
+  sum[1] = b;
+  sum[2] = g;
 }
 
-// FP32 harmonic loop (mixed precision): rotation, amplitude polynomials and
-// the three sums in FP32 from FP64-reduced phase factors; no argument
-// correction.
+// FP32 harmonic loop (mixed precision): rotation and the per-power sums in
+// FP32 from FP64-reduced phase factors; no argument correction.
 template <int PO>
 __device__ inline void harmonics_fp32(int nmodes, cfptr cf, float d, const PhaseSeed &S, double *sum) {
   constexpr int NC = 3 * (PO + 1);
-  float dp[PO + 1];
-  dp[0] = 1.f;
+  float qr = (float)S.qr, qi = (float)S.qi;
+  float Sa[PO + 1], Sb[PO + 1], Sg[PO + 1];
 #pragma unroll
-  for (int j = 1; j <= PO; j++) dp[j] = dp[j - 1] * d;
-  float qr = (float)S.qr, qi = (float)S.qi, Sa = 0.f, Sb = 0.f, Sg = 0.f;
+  for (int j = 0; j <= PO; j++) Sa[j] = Sb[j] = Sg[j] = 0.f;
   const float ur = (float)S.ur, ui = (float)S.ui;
 #pragma unroll 2
   for (int nn = 1; nn <= nmodes; nn++, cf += NC) {
     const float tr = fmaf(qr, ur, -(qi * ui));
     qi = fmaf(qr, ui, qi * ur);
     qr = tr;
-    const float al = poly_pow<PO>(cf, dp);
-    const float be = poly_pow<PO>(cf + (PO + 1), dp);
-    const float ga = poly_pow<PO>(cf + 2 * (PO + 1), dp);
-    Sa = fmaf(qr, al, Sa);
-    Sg = fmaf(qr, ga, Sg);
-    Sb = fmaf(qi, be, Sb);
+#pragma unroll
+    for (int j = 0; j <= PO; j++) {
+      Sa[j] = fmaf(qr, cf[j], Sa[j]);
+      Sb[j] = fmaf(qi, cf[PO + 1 + j], Sb[j]);
+      Sg[j] = fmaf(qr, cf[2 * (PO + 1) + j], Sg[j]);
+    }
   }
-  sum[0] = Sa;
-  sum[1] = Sb;
-  sum[2] = Sg;
+  float a = Sa[PO], b = Sb[PO], g = Sg[PO];
+#pragma unroll
+  for (int j = PO - 1; j >= 0; j--) {
+    a = fmaf(a, d, Sa[j]);
+    b = fmaf(b, d, Sb[j]);
+    g = fmaf(g, d, Sg[j]);
+  }
+  sum[0] = a;
+  sum[1] = b;
+  sum[2] = g;
 }
 
 // Samples past the signal (or the two padding slots) get time and zeros.
 __device__ inline bool modesum_tail(const DenseParams &P, const DenseBuffers &B, int i, int n_end) {
-  double *hI = B.h, *hII = B.h + (P.N + 2);
+  double *hI = B.h, *hII = B.h2;
   if (i >= P.N) {
     B.t[i] = 0.0;
     hI[i] = 0.0;
@@ -1446,7 +1449,7 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
   const double accII = G.W[3] * sum[0] + G.W[4] * sum[1] + G.W[5] * sum[2];
   const double sc = (i <= P.ip) ? G.Amp : G.Amp / taper(P, i, G.ti);
   B.h[i] = sc * accI;
-  B.h[P.N + 2 + i] = sc * accII;
+  B.h2[i] = sc * accII;
 }
 
 // Per-sample mode sum for the blocks listed by k_modesum_setup (B.ms_p < 0):
@@ -1506,7 +1509,7 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum_direct(DenseParams P, De
       B.t[i] = G.ti;
       const double sc = (i <= P.ip) ? G.Amp : G.Amp / taper(P, i, G.ti);
       B.h[i] = sc * accI;
-      B.h[P.N + 2 + i] = sc * accII;
+      B.h2[i] = sc * accII;
     }
   }
 }
