@@ -185,6 +185,7 @@ struct aak_handle_s {
   PinnedBuf single_pin;
   // stage timing on the compute stream: rates, guard+folds, mode sum, rFFT, reduction
   int profiling = 0;
+  int force_fold_replay = 0;  // test hook: every fold takes the serial replay
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   hipEvent_t *evs = ev;               // events the marks go to (per template in a batch)
   std::vector<hipEvent_t> batch_ev;   // 6 per template of the last profiled batch
@@ -471,7 +472,9 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
     const int dir = pass == 0 ? 1 : -1;
     const int nsteps = pass == 0 ? nfwd : P.i0;
     if (nsteps <= 0) continue;
-    if (fold_exact(P, B, L.fw, L.fold_nblk_cap, dir, nsteps, L.stream)) return fail(-7, "fold block capacity");
+    FoldWork fw = L.fw;
+    fw.force_replay = h->force_fold_replay;
+    if (fold_exact(P, B, fw, L.fold_nblk_cap, dir, nsteps, L.stream)) return fail(-7, "fold block capacity");
   }
   launch_fold_fill(P, B, L.stream);
   h->mark(L, 2);
@@ -1148,6 +1151,12 @@ int aak_test_inner_product(aak_handle h, const double *hI, const double *hII, do
   HIPCHK(hipStreamSynchronize(L.stream));
   h->state = 0;  // the time-domain buffers no longer hold a generated waveform
   if (fused) *fused = h->fused_fft ? 1 : 0;
+  return 0;
+}
+
+int aak_test_force_fold_replay(aak_handle h, int on) {
+  if (!h) return fail(-1, "null handle");
+  h->force_fold_replay = on ? 1 : 0;
   return 0;
 }
 

@@ -144,6 +144,7 @@ struct FoldWork {
   FoldTie *ties;                        // [3][nblk][kFoldMaxTies]
   int *status;                          // [3]: 0 ok, 1 overflow/fallback
   int *fallbacks;                       // serial replays since the last counter reset
+  int force_replay;                     // 1: every phase takes the serial replay (test hook)
 };
 
 // Device buffers of one waveform slot.

@@ -14,8 +14,9 @@
 //                  block the amplitudes are Taylor-expanded in e
 //                  (k_modesum_setup); blocks where that does not apply are
 //                  summed per sample (k_modesum_direct)
-//   k_weighted_dot PSD-weighted inner products of the rFFT output
-//                  (AAK_manager.cu:352-446 semantics)
+//   k_weighted_dot PSD-weighted inner products of a rocFFT spectrum (lengths
+//                  the fused FFT of aak_fft.hip does not take), k_dot_final
+//                  the last reduction of either form (AAK_manager.cu:352-446)
 //
 // Compiled with -ffp-contract=off: every expression that feeds the phases
 // must round exactly like the reference CPU build.
@@ -274,6 +275,7 @@ __device__ inline int out_index(const FoldArgs &A, int s) { return A.i0 + A.dir 
 // both access patterns free of bank conflicts.
 static_assert(kFoldPerThread == 16, "fold_lds_index assumes runs of 16");
 constexpr int kFoldLds = kFoldBlock + kFoldBlock / kFoldPerThread;
+static_assert(kFoldLds >= 4096, "the serial replay stages 4096 steps in the staging array");
 __device__ inline int fold_lds_index(int l) { return l + (l >> 4); }
 
 // Every load is issued before the first use: steps past the end read the last
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(1024) void k_fold_plan(FoldArgs A, FoldWork W) {
     int ex = 0;
     frexp(shm[0] > 0.0 ? shm[0] : 1.0, &ex);
     W.E[p] = ((double)A.nsteps + 2.0 * kFoldBlock) * ldexp(1.0, ex - 52);
-    W.status[p] = 0;
+    W.status[p] = W.force_replay ? 1 : 0;
   }
 }
 
@@ -455,6 +457,30 @@ __device__ inline int binade_key(double a, double E) {
   return (klo + 4096) * 2 + (a > 0.0 ? 1 : 0);
 }
 
+// Fallback: plain serial replay of one phase's whole fold, by one workgroup
+// of kFoldThreads lanes with buf (4096 doubles of LDS) as staging; used when
+// the item bookkeeping overflowed or a chain check failed (correct for any
+// input).
+__device__ void fold_serial_replay(const FoldArgs &A, const FoldWork &W, int p, double *buf) {
+  if (threadIdx.x == 0) atomicAdd(W.fallbacks, 1);
+  double x = A.x0[p];
+  if (threadIdx.x == 0) A.out[p][A.i0] = x;
+  for (int base = 0; base < A.nsteps; base += 4096) {
+    const int n = min(4096, A.nsteps - base);
+    for (int j = threadIdx.x; j < n; j += kFoldThreads) buf[j] = fold_incr(A.rate[p], A.i0, A.dir, base + j, A.dt);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int j = 0; j < n; j++) {
+        x = x + buf[j];
+        buf[j] = x;
+      }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += kFoldThreads) A.out[p][out_index(A, base + j)] = buf[j];
+    __syncthreads();
+  }
+}
+
 // Classify steps, build the block's item list (PLAN) or write the exact
 // fold values of run steps (WRITE, after the chain resolved the run starts).
 template <bool WRITE>
@@ -465,7 +491,12 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWor
   __shared__ int shf[kFoldThreads];
   __shared__ double sd[kFoldLds];
   const int p = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
-  if (W.status[p]) return;
+  if (W.status[p]) {
+    // the write pass of a phase the chain could not resolve: its first
+    // workgroup replays the fold serially
+    if (WRITE && b == 0) fold_serial_replay(A, W, p, sd);
+    return;
+  }
   const int base = b * kFoldBlock + tid * kFoldPerThread;
   const double E = W.E[p];
   stage_increments(A, p, b, sd);
@@ -915,31 +946,6 @@ __global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWo
   }
 }
 
-// Fallback: plain serial replay of the whole fold (used when the item
-// bookkeeping overflows; correct for any input).
-__global__ __launch_bounds__(256) void k_fold_serial(FoldArgs A, FoldWork W) {
-  __shared__ double buf[4096];
-  const int p = blockIdx.x;
-  if (!W.status[p]) return;
-  if (threadIdx.x == 0) atomicAdd(W.fallbacks, 1);
-  double x = A.x0[p];
-  if (threadIdx.x == 0) A.out[p][A.i0] = x;
-  for (int base = 0; base < A.nsteps; base += 4096) {
-    const int n = min(4096, A.nsteps - base);
-    for (int j = threadIdx.x; j < n; j += blockDim.x) buf[j] = fold_incr(A.rate[p], A.i0, A.dir, base + j, A.dt);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int j = 0; j < n; j++) {
-        x = x + buf[j];
-        buf[j] = x;
-      }
-    }
-    __syncthreads();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) A.out[p][out_index(A, base + j)] = buf[j];
-    __syncthreads();
-  }
-}
-
 int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, int nblk_cap, int dir, int nsteps,
                hipStream_t s) {
   if (nsteps <= 0) return 0;
@@ -960,7 +966,6 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
   hipLaunchKernelGGL(k_fold_local<false>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_chain, dim3(3), dim3(kChainThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_local<true>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
-  hipLaunchKernelGGL(k_fold_serial, dim3(3), dim3(256), 0, s, A, W);
   return 0;
 }
 

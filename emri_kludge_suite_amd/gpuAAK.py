@@ -173,6 +173,10 @@ class GPUAAK:
         spec = spec[0::2] + 1j * spec[1::2]
         return (float(like[0]), float(like[1])), spec.reshape(2, n // 2 + 1), bool(fused[0])
 
+    def force_fold_replay(self, on=True):
+        """Route every later phase fold through the serial replay (test hook)."""
+        _lib.check(_lib.lib().aak_test_force_fold_replay(self._h, int(bool(on))), "force_fold_replay")
+
     def dense_arrays(self):
         """Device dense arrays of the last waveform (test hook)."""
         L = self.length + 1

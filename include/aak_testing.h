@@ -54,6 +54,11 @@ struct aak_handle_s;
 int aak_test_inner_product(struct aak_handle_s *h, const double *hI, const double *hII, double *like, double *spec,
                            int *fused);
 
+/* on != 0: every phase fold of later waveforms on this handle skips the
+ * parallel resolution and takes the serial replay (the fallback used when the
+ * fold's bookkeeping overflows), so that path can be checked. */
+int aak_test_force_fold_replay(struct aak_handle_s *h, int on);
+
 /* Copies the dense arrays of the last aak_gpu_gen_AAK to the host:
  * out[10*(length+1)] = e | v | M | S | gimdot | Phidot | alpdot | gim | Phi | alp. */
 int aak_test_get_dense(struct aak_handle_s *h, double *out);

@@ -139,6 +139,27 @@ def test_likelihood_parity():
     assert np.max(np.abs(spec - hI)) <= 1e-9 * np.max(np.abs(hI))
 
 
+def test_serial_fold_replay_is_bit_identical():
+    # the fallback the parallel fold takes when its bookkeeping overflows,
+    # forced for every phase of both directions (backint)
+    from emri_kludge_suite_amd import GPUAAK
+    pars = [np.pi / 6, 0.5, 8.0, 0.1, 1e6, 100.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
+    N, dt = 2 ** 18, 10.0
+    print("case serial-replay N=%d dt=%g backint=1" % (N, dt), flush=True)
+    ref = ol.o_aak_wave(pars, length=N, dt=dt, backint=True, want_dense=True)
+    g = GPUAAK(1.0, N // 10, N, 100.0, dt, True, True)
+    g.force_fold_replay(True)
+    g.gpu_gen_AAK(*pars)
+    info = g.info()
+    dense = g.dense_arrays()
+    print(info, flush=True)
+    assert info["fold_fallbacks"] == (6 if info["i0"] > 0 else 3)
+    for name in ("gim", "Phi", "alp"):
+        assert np.array_equal(dense[name], ref["dense_" + name]), name
+    t, hI, hII = g.GetWaveform()
+    assert _rel_l2(hI[:N], ref["hI"]) <= 1e-10 and _rel_l2(hII[:N], ref["hII"]) <= 1e-10
+
+
 def test_high_eccentricity_parity():
     # e0 = 0.7 lies above the series range: the mode sum takes the Miller path
     pars = [0.5, 0.6, 12.0, 0.7, 1e6, 10.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
