@@ -7,9 +7,10 @@ e0 ~ U[1e-3,0.5], p0 ~ U[7,15], iota0 = 0.5, angles uniform on the sphere),
 noise-weighted inner products against a seeded injected signal under the
 analytic LISA instrument + confusion PSD (1 yr) on the rFFT grid.
 
-One step = one batch of templates (default 32) through the whole path: host
+One step = one batch of templates (default 256, the "hundreds of waveforms per
+launch" of the batched form) through the whole path: host
 inspiral + knots (thread pool), device spline/phasing/exact folds/mode sum,
-rFFT and the fused weighted reduction, pipelined across templates.  The data
+the fused FFT + PSD-weighted reduction, pipelined across templates.  The data
 (weighted injection) is resident in HBM before timing starts.
 
 The headline value is the mixed-precision form (FP32 harmonic loop on FP64
@@ -133,9 +134,9 @@ def read_profile_value(name, key):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5, help="timed batches")
-    ap.add_argument("--warmup", type=int, default=8, help="untimed batches (also brings the GPU clocks up)")
-    ap.add_argument("--batch", type=int, default=32, help="templates per step")
+    ap.add_argument("--steps", type=int, default=10, help="timed batches")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed batches (also brings the GPU clocks up)")
+    ap.add_argument("--batch", type=int, default=256, help="templates per step")
     ap.add_argument("--lanes", type=int, default=2, help="concurrent streams of the batched path")
     ap.add_argument("--single", type=int, default=8, help="one-waveform-per-call likelihoods timed beside")
     ap.add_argument("--mismatch", type=int, default=4, help="templates whose mixed/FP64 mismatch is measured")

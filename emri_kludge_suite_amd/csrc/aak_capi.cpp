@@ -203,8 +203,9 @@ struct aak_handle_s {
   std::condition_variable cv_work, cv_ready;
   bool stop = false;
   const double *b_params = nullptr;
-  long long b_n = 0, b_next = 0;
+  long long b_n = 0, b_next = 0, b_end = 0;  // b_order[b_next, b_end) not yet taken
   std::vector<long long> b_order;  // templates in decreasing estimated host cost
+  int feeders = 1;                 // workers that take the cheapest remaining template
   std::atomic<size_t> pin_hint{0};  // largest knot table (doubles) planned by the pool
   size_t knot_hint = 0;             // largest evolution array length planned by the pool (under mu)
   std::vector<size_t> worker_cap;   // evolution array length of each worker (under mu)
@@ -530,11 +531,13 @@ static void worker_main(aak_handle h, int id) {
     size_t grow_to = 0;
     {
       std::unique_lock<std::mutex> lk(h->mu);
-      auto has_work = [&] { return h->b_params && h->b_next < h->b_n && !h->free_slots.empty(); };
+      auto has_work = [&] { return h->b_params && h->b_next < h->b_end && !h->free_slots.empty(); };
       h->cv_work.wait(lk, [&] { return h->stop || has_work() || plan.knots.eA.size() < h->knot_hint; });
       if (h->stop) return;
       if (has_work()) {
-        i = h->b_order[(size_t)h->b_next++];
+        // feeders keep the GPU supplied with short plans from the start;
+        // the others work down from the longest, which bound the batch
+        i = (id < h->feeders) ? h->b_order[(size_t)--h->b_end] : h->b_order[(size_t)h->b_next++];
         params = h->b_params;
         sidx = h->free_slots.back();
         h->free_slots.pop_back();
@@ -601,6 +604,7 @@ static int start_pool(aak_handle h) {
     h->ring.push_back(std::move(slot));
   }
   h->worker_cap.assign((size_t)T, 0);
+  h->feeders = T >= 8 ? T / 4 : 1;
   for (int t = 0; t < T; t++) h->workers.emplace_back(worker_main, h, t);
   return 0;
 }
@@ -848,9 +852,13 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
     h->batch_ev.resize((size_t)6 * n, nullptr);
     for (size_t k = old; k < h->batch_ev.size(); k++) HIPCHK(hipEventCreate(&h->batch_ev[k]));
   }
-  // Longest first: the host stage costs a few ms plus O(K) with K, the knot
-  // count, ~ 99 max(1, t_rr / T_fit) where t_rr = length dt mu / (M^2 M_sun)
-  // is the observation time in radiation-reaction units (AAKpy.cc:24-26).
+  // Order by estimated host cost: the host stage costs a few ms plus O(K)
+  // with K, the knot count, ~ 99 max(1, t_rr / T_fit) where
+  // t_rr = length dt mu / (M^2 M_sun) is the observation time in
+  // radiation-reaction units (AAKpy.cc:24-26).  A quarter of the workers take
+  // templates from the cheap end (a steady supply of plans from the first
+  // millisecond), the rest from the expensive end (the longest plans, which
+  // bound the batch, start at once).
   std::vector<std::pair<double, long long>> cost((size_t)n);
   for (long long i = 0; i < n; i++) {
     const double M = params[14 * i + 4], mu = params[14 * i + 5];
@@ -862,8 +870,6 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
                    [](const std::pair<double, long long> &a, const std::pair<double, long long> &b) {
                      return a.first > b.first;
                    });
-  // the two cheapest go first so the GPU starts after one short host stage
-  if (n > 2) std::rotate(cost.begin(), cost.end() - 2, cost.end());
   {
     std::lock_guard<std::mutex> lk(h->mu);
     h->b_order.resize((size_t)n);
@@ -874,6 +880,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
     h->b_params = params;
     h->b_n = n;
     h->b_next = 0;
+    h->b_end = n;
   }
   h->cv_work.notify_all();
   int err_rc = 0;
