@@ -183,16 +183,23 @@ def main():
         g.set_lanes(LANES)
         return np.array([tm[x] for x in ("rates", "fold", "modesum", "fft", "dot")])
 
+    host_plan = {}
+
     def run_batched(mode):
         g.set_precision(mode)
         g.set_lanes(LANES)
         for _ in range(args.warmup):
             g.LikelihoodBatch(P)
         t0 = time.perf_counter()
+        host = np.zeros(2)
         for _ in range(args.steps):
             like = g.LikelihoodBatch(P)
+            hs, hm = g.batch_host_ms()
+            host += (hs, hm)
         elapsed = time.perf_counter() - t0
         assert np.all(np.isfinite(like)), "bench: a template failed"
+        host_plan[mode] = {"ms_per_template": host[0] / (args.steps * len(P)), "longest_ms": host[1] / args.steps,
+                           "pool_threads": int(os.environ.get("AAK_HOST_THREADS", "0")) or None}
         # stage times: on one lane (no overlap with another template's kernels)
         # and as they run concurrently on the bench's lanes
         return elapsed, profiled_pass(1), profiled_pass(LANES), like
@@ -266,6 +273,8 @@ def main():
                                "(batched), mixed precision" % len(P),
                    "N": N_SAMPLES, "dt": DT, "n_max": N_MAX, "batch": len(P), "lanes": LANES,
                    "psd": "LISA_Noise(dur=1)"},
+        "host_stage": {"mixed": host_plan.get("mixed"), "fp64": host_plan.get("fp64"),
+                       "note": "host-stage time per template on the pool during the timed batches"},
         "accuracy": {"max_mismatch_vs_fp64": max(mism), "templates_checked": len(mism),
                      "max_rel_diff_hh_vs_fp64": like_rel},
         "stages_ms_per_template": dict(zip(stages, [round(x / per_pass, 4) for x in st_mx])),

@@ -118,6 +118,7 @@ struct RingSlot {
   bool copy_pending = false;
   long long tpl = -1;           // template index held
   int rc = 0;
+  double plan_ms = 0.0;         // host stage time of the template held
   ~RingSlot() {
     pin.release();
     if (copied) (void)hipEventDestroy(copied);
@@ -191,6 +192,7 @@ struct aak_handle_s {
   std::vector<hipEvent_t> batch_ev;   // 6 per template of the last profiled batch
   double stage_ms[5] = {0, 0, 0, 0, 0};
   double batch_ms[5] = {0, 0, 0, 0, 0};  // stage sums over the last profiled batch
+  double batch_host_sum = 0.0, batch_host_max = 0.0;  // host stage ms over the last batch
   int batch_profiled = 0;
   double host_ms = 0.0;
   // batch machinery
@@ -564,7 +566,9 @@ static void worker_main(aak_handle h, int id) {
     std::memcpy(&sp, params + 14 * (size_t)i, sizeof(SourceParams));
     PinnedSink sink(&slot->pin, &h->pin_hint);
     std::string err;
+    const auto t_plan = std::chrono::steady_clock::now();
     const int rc = plan_waveform(h->cfg, sp, &plan, &err, &sink);
+    slot->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
     slot->P = plan.P;
     slot->tab_n = plan.tab_n;
     bool hint_grew = false;
@@ -885,6 +889,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   h->cv_work.notify_all();
   int err_rc = 0;
   std::string err_msg;
+  double host_sum = 0.0, host_max = 0.0;
   for (long long done = 0; done < n; done++) {
     int sidx;
     {
@@ -896,6 +901,8 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
     RingSlot &slot = *h->ring[(size_t)sidx];
     const long long i = slot.tpl;
     st[i] = slot.rc < 0 ? slot.rc : 0;
+    host_sum += slot.plan_ms;
+    host_max = std::max(host_max, slot.plan_ms);
     if (st[i] == 0 && err_rc == 0) {
       if (prof) h->evs = h->batch_ev.data() + 6 * i;
       // templates alternate over the lanes; each lane alternates its two table slots
@@ -950,6 +957,8 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
       return true;
     });
   }
+  h->batch_host_sum = host_sum;
+  h->batch_host_max = host_max;
   if (prof) {
     for (int k = 0; k < 5; k++) h->batch_ms[k] = 0.0;
     float ms = 0.f;
@@ -1009,6 +1018,13 @@ int aak_get_batch_timings(aak_handle h, double *ms, int n) {
   const int m = n < 5 ? n : 5;
   for (int i = 0; i < m; i++) ms[i] = h->batch_ms[i];
   return m;
+}
+
+int aak_get_batch_host_ms(aak_handle h, double *sum_ms, double *max_ms) {
+  if (!h) return fail(-1, "null handle");
+  if (sum_ms) *sum_ms = h->batch_host_sum;
+  if (max_ms) *max_ms = h->batch_host_max;
+  return 0;
 }
 
 int aak_get_info(aak_handle h, int *info, int n) {

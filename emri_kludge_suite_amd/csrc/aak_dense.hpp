@@ -159,10 +159,10 @@ struct DenseBuffers {
   double *t;                       // [N+2]
   int *flags;                      // [4]: [0] non-positive Phidot count, [1] fold replays, [2] per-sample mode-sum blocks
   int *guard_blk;                  // [guard blocks]: last positive-Phidot index per block
-  double *ms_coef;                 // per mode-sum block nmodes x 3 x (kTaylorMax+1) doubles; order p uses [nmodes][3][p+1] (float in mixed mode)
-  int *ms_p;                       // [mode-sum blocks] Taylor order (-1: per-sample path)
-  int *ms_list;                    // [mode-sum blocks] blocks of the per-sample path (count in flags[2])
-  double *ms_eref;                 // [mode-sum blocks] expansion point
+  double *ms_coef;                 // per expansion block nmodes x 3 x (kTaylorMax+1) doubles; order p uses [nmodes][3][p+1] (float in mixed mode)
+  int *ms_p;                       // [expansion blocks] Taylor order (-1: per-sample path)
+  int *ms_list;                    // [expansion blocks] blocks of the per-sample path (count in flags[2])
+  double *ms_eref;                 // [expansion blocks] expansion point
   const double *bessel_c;          // series coefficients c_m(k)
   const double *bessel_lgf;        // log(k!)
 };

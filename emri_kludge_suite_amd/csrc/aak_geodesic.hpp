@@ -154,9 +154,27 @@ struct Geodesic {
   int stable;  // 1 stable, 0 marginal, -1 unstable, 2 unclassified
 
   // Newton on (E, Lz) with a 2x2 Crout factorisation using implicit row
-  // scaling (the scheme of the reference's mnewt/ludcmp/lubksb).
+  // scaling (the scheme of the reference's mnewt/ludcmp/lubksb), at most 200
+  // iterations.  An iteration depends on the state x alone, so once x equals
+  // bit for bit the state of P iterations earlier (near-circular orbits often
+  // never meet the tolerance and settle into such a cycle of period <= ~40),
+  // every remaining iteration repeats that cycle: the state the 200th
+  // iteration would leave is read from the recorded cycle.
   AAK_HD int newton(double *x) {
+    constexpr int kRing = 64;
+    double ring[kRing][2];  // state at the start of iterations k-63..k
     for (int k = 1; k <= 200; k++) {
+      for (int P = 1; P <= kRing - 1 && P < k; P++) {
+        const double *o = ring[(k - P) % kRing];
+        if (bits_equal(o[0], x[0]) && bits_equal(o[1], x[1])) {
+          const int j = k - P + (201 - k) % P;  // state at the start of "iteration 201"
+          x[0] = ring[j % kRing][0];
+          x[1] = ring[j % kRing][1];
+          return 0;
+        }
+      }
+      ring[k % kRing][0] = x[0];
+      ring[k % kRing][1] = x[1];
       const double e = x[0], lz = x[1];
       const double ti2 = tani * tani, ci2 = cosiota * cosiota, q = lz * lz * ti2;
       const double f0 = radial_potential(peri, a, e, lz, q);

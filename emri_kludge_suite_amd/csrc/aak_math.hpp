@@ -30,6 +30,9 @@ AAK_HD double dmax(double a, double b) { return a > b ? a : b; }
 AAK_HD bool is_nan(double x) { return x != x; }
 AAK_HD bool is_finite(double x) { return (x - x) == 0.0; }
 AAK_HD double dmin(double a, double b) { return a < b ? a : b; }
+AAK_HD bool bits_equal(double a, double b) {
+  return __builtin_bit_cast(unsigned long long, a) == __builtin_bit_cast(unsigned long long, b);
+}
 
 // ---------------------------------------------------------------- Kerr
 // Globals.h:43-179

@@ -154,6 +154,12 @@ class GPUAAK:
         keys = ("rates", "fold", "modesum", "fft", "dot")
         return dict(zip(keys, (float(x) for x in buf)))
 
+    def batch_host_ms(self):
+        """Host-stage time (ms) of the last LikelihoodBatch: (sum over templates, longest)."""
+        a, b = np.zeros(1), np.zeros(1)
+        _lib.check(_lib.lib().aak_get_batch_host_ms(self._h, _lib.dptr(a), _lib.dptr(b)), "batch_host_ms")
+        return float(a[0]), float(b[0])
+
     def inner_product_stage(self, hI, hII):
         """Runs the inner-product stage alone on two real series (test hook).
 

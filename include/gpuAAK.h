@@ -115,8 +115,13 @@ int aak_set_profiling(aak_handle h, int on);
 int aak_get_timings(aak_handle h, double *ms, int n);
 
 /* Stage time sums (ms) over the templates of the last batch run with
- * profiling on: rates+spline, guard+folds, mode sum, rFFT, reduction. */
+ * profiling on: rates+spline, guard+folds, mode sum, FFT pass (or rFFT),
+ * second FFT pass with the weighted sums (or the reduction). */
 int aak_get_batch_timings(aak_handle h, double *ms, int n);
+
+/* Host-stage time of the last batch's templates (ms, on the host pool):
+ * their sum and the longest one.  No counterpart in the reference. */
+int aak_get_batch_host_ms(aak_handle h, double *sum_ms, double *max_ms);
 
 /* Message of the last failed call on this thread. */
 const char *aak_last_error(void);
