@@ -362,7 +362,7 @@ static int alloc_handle(aak_handle h) {
     // W^e = W^(1024 (e >> 10)) W^(e & 1023), in extended precision
     const int nhi = F.M / 1024 + 2;
     const size_t o_R = 0, o_C = o_R + F.R, o_Mlo = o_C + F.C, o_Mhi = o_Mlo + 1024, o_Nlo = o_Mhi + nhi,
-                 o_Nhi = o_Nlo + 1024, total = o_Nhi + nhi;
+                 o_Nhi = o_Nlo + 1024, o_Nr = o_Nhi + nhi, total = o_Nr + F.C;
     std::vector<double2> tw(total);
     auto root = [](long long e, long long L) {
       const long double pi = 3.141592653589793238462643383279502884L;
@@ -379,6 +379,7 @@ static int alloc_handle(aak_handle h) {
       tw[o_Mhi + e] = root(1024LL * e, F.M);
       tw[o_Nhi + e] = root(1024LL * e, F.N);
     }
+    for (int e = 0; e < F.C; e++) tw[o_Nr + e] = root((long long)F.R * e, F.N);
     HIPCHK(hipMalloc(&h->d_fft_tw, sizeof(double2) * total));
     HIPCHK(hipMemcpy(h->d_fft_tw, tw.data(), sizeof(double2) * total, hipMemcpyHostToDevice));
     F.twR = h->d_fft_tw + o_R;
@@ -387,6 +388,7 @@ static int alloc_handle(aak_handle h) {
     F.twMhi = h->d_fft_tw + o_Mhi;
     F.twNlo = h->d_fft_tw + o_Nlo;
     F.twNhi = h->d_fft_tw + o_Nhi;
+    F.twNr = h->d_fft_tw + o_Nr;
     F.Dp[0] = h->d_data;
     F.Dp[1] = h->d_data + nb;
     F.wp[0] = h->d_w;

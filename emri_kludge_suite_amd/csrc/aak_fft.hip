@@ -174,9 +174,11 @@ __device__ inline double2 tw2(const double2 *hi, const double2 *lo, long long e)
 }
 
 // ---------------------------------------------------------------- pass 1
-// columns per workgroup: 8 (one 128-byte line per row) while 8 columns fit LDS
+// columns per workgroup: 8 (one 128-byte line per row) up to 512-point
+// columns, 4 beyond (two workgroups per CU at 1024 points); the workgroups
+// of the two halves of a line are placed on one XCD (col_group)
 template <int LR>
-constexpr int col_cols() { return LR <= 1024 ? 8 : 4; }
+constexpr int col_cols() { return LR <= 512 ? 8 : 4; }
 // LDS stride of a column: padded length, plus one so the columns of a row
 // start on different banks
 template <int LR>
@@ -190,7 +192,20 @@ __global__ __launch_bounds__(col_cols<LR>() * fft_threads<LR>()) void k_fft_cols
                                                                                   double2 *Y) {
   constexpr int CC = col_cols<LR>(), CS = col_stride<LR>(), NT = fft_threads<LR>();
   __shared__ double2 s[CC * CS];
-  const int ch = blockIdx.y, b0 = blockIdx.x * CC, tid = threadIdx.x;
+  int ch = blockIdx.y, grp = blockIdx.x;
+  if (CC < 8) {
+    // workgroups are dealt round-robin to the 8 XCDs in launch order; pair
+    // groups 2q and 2q+1 (one 128-byte line per row between them) on one XCD,
+    // launched 8 apart, so each line is fetched into one L2 once
+    const int L = blockIdx.y * gridDim.x + blockIdx.x, total = gridDim.x * gridDim.y;
+    if ((total & 15) == 0) {
+      const int xcd = L & 7, r = L >> 3;
+      const int g = ((((r >> 1) << 3) | xcd) << 1) | (r & 1);
+      ch = g / gridDim.x;
+      grp = g % gridDim.x;
+    }
+  }
+  const int b0 = grp * CC, tid = threadIdx.x;
   constexpr int PT = (LR + NT - 1) / NT;  // elements per thread in the staging loops
   constexpr bool kFull = LR % NT == 0;
   const double2 *z = reinterpret_cast<const double2 *>(h + (size_t)ch * F.hs);
@@ -233,53 +248,67 @@ __global__ __launch_bounds__(2 * fft_threads<LC>()) void k_fft_rows(FftArgs F, c
   const int half = tid / NT;
   const int A = blk, B = blk == 0 ? F.R / 2 : F.R - blk;
   const double2 *Yc = Y + (size_t)ch * F.M;
-  fft_first_global<LC, NT>(Yc + (size_t)(half ? B : A) * F.C, s + half * RSD, tid % NT, F.twC);
-  const double2 *sA = s, *sB = s + RSD;
   const double2 *Dp = F.Dp[ch];
   const double *wp = F.wp[ch];
+  // this thread's weighted data and weights, fetched before the FFT so their
+  // latency hides behind it; k2 = tid + q 2NT in rows A and B
+  constexpr int KP = (LC + 2 * NT - 1) / (2 * NT);
+  constexpr bool kFull = LC % (2 * NT) == 0;
+  double2 dA[KP], dB[KP];
+  double wA[KP], wB[KP];
+#pragma unroll
+  for (int q = 0; q < KP; q++) {
+    const int k2 = min(tid + q * 2 * NT, LC - 1);
+    dA[q] = Dp[A * LC + k2];
+    wA[q] = wp[A * LC + k2];
+    dB[q] = Dp[B * LC + k2];
+    wB[q] = wp[B * LC + k2];
+  }
+  fft_first_global<LC, NT>(Yc + (size_t)(half ? B : A) * F.C, s + half * RSD, tid % NT, F.twC);
+  const double2 *sA = s, *sB = s + RSD;
   double2 *hw = hw_out ? hw_out + (size_t)ch * (F.M + 1) : nullptr;
   double dh = 0.0, hh = 0.0;
-  // H_k from Z_k and conj(Z_(M-k)); pidx: position of k in the stored data order
-  auto emit = [&](long long k, double2 zk, double2 zm, int pidx) {
+  // W_N^(row + R k2) = W_N^row W_N^(R k2)
+  const double2 WrA = tw2(F.twNhi, F.twNlo, A), WrB = tw2(F.twNhi, F.twNlo, B);
+  // H_k from Z_k and conj(Z_(M-k)), k = row + R k2, weighted and summed
+  auto emit = [&](long long k, double2 W, double2 zk, double2 zm, double2 d, double w) {
     const double2 zmc = make_double2(zm.x, -zm.y);
     const double2 E = make_double2(0.5 * (zk.x + zmc.x), 0.5 * (zk.y + zmc.y));
     const double2 dz = make_double2(zk.x - zmc.x, zk.y - zmc.y);
     const double2 O = make_double2(0.5 * dz.y, -0.5 * dz.x);
-    const double2 WO = cmul(tw2(F.twNhi, F.twNlo, k), O);
-    const double w = wp[pidx];
+    const double2 WO = cmul(W, O);
     const double2 x = make_double2((E.x + WO.x) * w, (E.y + WO.y) * w);
-    const double2 d = Dp[pidx];
     dh += x.x * d.x + x.y * d.y;
     hh += x.x * x.x + x.y * x.y;
     if (hw) hw[k] = x;
   };
-  constexpr int KP = (LC + 2 * NT - 1) / (2 * NT);  // k2 values per thread
-  if (blk == 0) {
-    const int H = F.R / 2;
-    for (int k2 = tid; k2 < LC; k2 += 2 * NT) {
-      if (k2 == 0) {
-        // H_0 = E_0 + O_0 and H_M = E_0 - O_0, both real
-        const double2 z0 = sA[0];
-        const double w0 = wp[0], wm = wp[F.M];
-        const double2 x0 = make_double2((z0.x + z0.y) * w0, 0.0), xm = make_double2((z0.x - z0.y) * wm, 0.0);
-        dh += x0.x * Dp[0].x + x0.y * Dp[0].y + xm.x * Dp[F.M].x + xm.y * Dp[F.M].y;
-        hh += x0.x * x0.x + xm.x * xm.x;
-        if (hw) {
-          hw[0] = x0;
-          hw[F.M] = xm;
-        }
-      } else {
-        emit((long long)F.R * k2, sA[pad(k2)], sA[pad(LC - k2)], k2);
-      }
-      emit(H + (long long)F.R * k2, sB[pad(k2)], sB[pad(LC - 1 - k2)], H * LC + k2);
-    }
-  } else {
 #pragma unroll
-    for (int q = 0; q < KP; q++) {
-      const int k2 = tid + q * 2 * NT;
-      if (LC % (2 * NT) == 0 || k2 < LC) {
-        emit(A + (long long)F.R * k2, sA[pad(k2)], sB[pad(LC - 1 - k2)], A * LC + k2);
-        emit(B + (long long)F.R * k2, sB[pad(k2)], sA[pad(LC - 1 - k2)], B * LC + k2);
+  for (int q = 0; q < KP; q++) {
+    const int k2 = tid + q * 2 * NT;
+    if (kFull || k2 < LC) {
+      const double2 t = F.twNr[k2];
+      const long long kA = A + (long long)F.R * k2, kB = B + (long long)F.R * k2;
+      if (blk == 0) {
+        // row 0 (Z_(M-k) in row 0 itself) and row R/2 (its own mirror)
+        if (k2 == 0) {
+          // H_0 = E_0 + O_0 and H_M = E_0 - O_0, both real
+          const double2 z0 = sA[0];
+          const double wm = wp[F.M];
+          const double2 dm = Dp[F.M];
+          const double2 x0 = make_double2((z0.x + z0.y) * wA[q], 0.0), xm = make_double2((z0.x - z0.y) * wm, 0.0);
+          dh += x0.x * dA[q].x + x0.y * dA[q].y + xm.x * dm.x + xm.y * dm.y;
+          hh += x0.x * x0.x + xm.x * xm.x;
+          if (hw) {
+            hw[0] = x0;
+            hw[F.M] = xm;
+          }
+        } else {
+          emit(kA, t, sA[pad(k2)], sA[pad(LC - k2)], dA[q], wA[q]);
+        }
+        emit(kB, cmul(WrB, t), sB[pad(k2)], sB[pad(LC - 1 - k2)], dB[q], wB[q]);
+      } else {
+        emit(kA, cmul(WrA, t), sA[pad(k2)], sB[pad(LC - 1 - k2)], dA[q], wA[q]);
+        emit(kB, cmul(WrB, t), sB[pad(k2)], sA[pad(LC - 1 - k2)], dB[q], wB[q]);
       }
     }
   }

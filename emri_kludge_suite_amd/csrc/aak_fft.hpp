@@ -7,13 +7,14 @@ namespace aak {
 
 // N = 2M real samples per channel, M = R C (powers of two).  Tables: W_R^e,
 // W_C^e (e < R, C), W_M^e = twMhi[e >> 10] twMlo[e & 1023] for e < M and
-// W_N^k = twNhi[k >> 10] twNlo[k & 1023] for k <= M.  Dp, wp: weighted data and
+// W_N^k = twNhi[k >> 10] twNlo[k & 1023] for k <= M, twNr[k2] = W_N^(R k2) for
+// k2 < C.  Dp, wp: weighted data and
 // weights of each channel in the order of the second pass, position
 // k1 * C + k2 for bin k = k1 + R k2 (k < M), position M for the Nyquist bin.
 struct FftArgs {
   int N, M, R, C;
   int hs;  // channel stride of the time-domain input (doubles, even)
-  const double2 *twR, *twC, *twMlo, *twMhi, *twNlo, *twNhi;
+  const double2 *twR, *twC, *twMlo, *twMhi, *twNlo, *twNhi, *twNr;
   const double2 *Dp[2];
   const double *wp[2];
 };
