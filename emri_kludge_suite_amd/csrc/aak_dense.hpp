@@ -72,6 +72,14 @@ AAK_HD void spline_interval(double x0, double x1, double y0, double y1, double c
   *d = (c1 - c0) / (3.0 * h);
 }
 
+// Single-precision copies of the response constants, for the mixed form's
+// per-sample geometry.
+struct GeomF32 {
+  float cosqL[3], phiLup[3], phiLdown[3], betadown[3], rc[3], rs[3], nxL2[6];
+  float cosqS, sinqS, cos_phS, sin_phS, cos_phK, sin_phK, cosqK, sinqK, Sdotn, coslam, halfsqrt3;
+  float psi_c0, psi_c1, psi_c2, psi_c3, nxS2, ldc_c2psi, ldc_s2psi;
+};
+
 // Per-waveform scalars of the dense stage, filled on the host.
 struct DenseParams {
   int N;           // samples (vlength)
@@ -107,6 +115,7 @@ struct DenseParams {
   KnotGrid grid;
   int mixed;                      // 1: FP32 harmonic loop (mixed precision)
   int store_vms;                  // 1: keep v, M, S per sample (read by no later stage)
+  GeomF32 gf;                     // response constants in FP32 (mixed form)
   int bessel_series;              // 1: series + recurrence, 0: Miller from above
   unsigned char bterms[kMaxModes + 1];  // series length per harmonic ([0]: orders 0..4)
 };

@@ -667,6 +667,36 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   P.bessel_series = (nmodes <= kMaxModes && e_bound <= kSeriesMaxE) ? 1 : 0;
   P.mixed = cfg.mixed ? 1 : 0;
   P.store_vms = 1;
+  {
+    GeomF32 &F = P.gf;
+    for (int k = 0; k < 3; k++) {
+      F.cosqL[k] = (float)P.g_cosqL[k];
+      F.phiLup[k] = (float)P.g_phiLup[k];
+      F.phiLdown[k] = (float)P.g_phiLdown[k];
+      F.betadown[k] = (float)P.g_betadown[k];
+      F.rc[k] = (float)P.g_rc[k];
+      F.rs[k] = (float)P.g_rs[k];
+    }
+    for (int k = 0; k < 6; k++) F.nxL2[k] = (float)P.g_nxL2[k];
+    F.cosqS = (float)P.cosqS;
+    F.sinqS = (float)P.sinqS;
+    F.cos_phS = (float)P.cos_phS;
+    F.sin_phS = (float)P.sin_phS;
+    F.cos_phK = (float)P.cos_phK;
+    F.sin_phK = (float)P.sin_phK;
+    F.cosqK = (float)P.cosqK;
+    F.sinqK = (float)P.sinqK;
+    F.Sdotn = (float)P.Sdotn;
+    F.coslam = (float)P.coslam;
+    F.halfsqrt3 = (float)P.halfsqrt3;
+    F.psi_c0 = (float)P.psi_c0;
+    F.psi_c1 = (float)P.psi_c1;
+    F.psi_c2 = (float)P.psi_c2;
+    F.psi_c3 = (float)P.psi_c3;
+    F.nxS2 = (float)P.nxS2;
+    F.ldc_c2psi = (float)P.ldc_c2psi;
+    F.ldc_s2psi = (float)P.ldc_s2psi;
+  }
   if (P.bessel_series) bessel_series_terms(e_bound, nmodes, P.bterms);
   return 0;
 }

@@ -1163,6 +1163,92 @@ __device__ inline void sample_geometry(const DenseParams &P, const DenseBuffers 
   G.W[5] = pc * XII;
 }
 
+// Angle reduced to [-pi, pi] in FP64, returned in FP32 (the reduction error,
+// ~k 2.4e-16 rad, is far below FP32 resolution).
+__device__ inline float reduce_2pi_f(double x) {
+  const double k = rint(x * (1.0 / (2.0 * kPi)));
+  return (float)fma(-k, 6.283185307179586, x);
+}
+
+// The mixed form's per-sample geometry: the same response as sample_geometry
+// in FP32, after the large angles (gamma, alpha) are reduced mod 2 pi in FP64.
+// The phase, its rates, e, t and the amplitude are read as they are.
+__device__ inline void sample_geometry_mixed(const DenseParams &P, const DenseBuffers &B, int i, SampleGeom &G) {
+#pragma clang fp contract(fast)
+  const GeomF32 &F = P.gf;
+  const double ti = sample_time(P, i);
+  G.ti = ti;
+  G.e = B.e[i];
+  G.Phi = B.phase[1][i];
+  G.gimdot = B.rate[0][i];
+  G.nu = (i == 0) ? B.nu0[0] : B.rate[1][i] * (0.5 / kPi);
+  G.Amp = B.amp[i];
+  const float gim = reduce_2pi_f(B.phase[0][i]), alp = reduce_2pi_f(B.phase[2][i]);
+  float sa, ca;
+  sincosf(alp, &sa, &ca);
+  auto aff = [&](const float *c) { return c[0] + c[1] * sa + c[2] * ca; };
+  const float cosqL = aff(F.cosqL);
+  const float sinqL = sqrtf(fmaxf(1.f - cosqL * cosqL, 0.f));
+  const float phiLup = aff(F.phiLup), phiLdown = aff(F.phiLdown);
+  const float rL2 = phiLup * phiLup + phiLdown * phiLdown;
+  const float irL = rL2 > 0.f ? rsqrtf(rL2) : 0.f;
+  const float cphiL = rL2 > 0.f ? phiLdown * irL : 1.f, sphiL = phiLup * irL;
+  const float Ldotn = cosqL * F.cosqS + sinqL * F.sinqS * (cphiL * F.cos_phS + sphiL * F.sin_phS);
+  const float Ldotn2 = Ldotn * Ldotn;
+  const float beta = atan2f(-F.Sdotn + F.coslam * Ldotn, aff(F.betadown));
+  float sin2gam, cos2gam;
+  sincosf(2.f * (gim + beta), &sin2gam, &cos2gam);
+  float cosorbphs = 0.f, FplusI, FcrosI, FplusII, FcrosII, h3;
+  if (P.mich) {
+    float so, co;
+    sincosf((float)(2. * kPi * ti / kYear), &so, &co);
+    cosorbphs = co * F.cos_phS + so * F.sin_phS;
+    const float sinorbphs = so * F.cos_phS - co * F.sin_phS;
+    const float cosorbK = co * F.cos_phK + so * F.sin_phK;
+    const float cosq = .5f * F.cosqS - F.halfsqrt3 * F.sinqS * cosorbphs;
+    const float Yw = F.halfsqrt3 * F.cosqS + .5f * F.sinqS * cosorbphs, Xw = F.sinqS * sinorbphs;
+    const float rw = Xw * Xw + Yw * Yw;
+    const float irw = rw > 0.f ? 1.f / rw : 0.f;
+    const float c2w = rw > 0.f ? (Xw * Xw - Yw * Yw) * irw : 1.f, s2w = 2.f * Xw * Yw * irw;
+    const float c2o = co * co - so * so, s2o = 2.f * so * co;
+    const float cos2phi = c2o * c2w - s2o * s2w, sin2phi = s2o * c2w + c2o * s2w;
+    const float psiup = .5f * F.cosqK - F.halfsqrt3 * F.sinqK * cosorbK - cosq * F.psi_c0;
+    const float psidown = F.psi_c1 - F.halfsqrt3 * co * F.psi_c2 - F.halfsqrt3 * so * F.psi_c3;
+    const float rp = psiup * psiup + psidown * psidown;
+    const float irp = rp > 0.f ? 1.f / rp : 0.f;
+    const float cos2psi = rp > 0.f ? (psidown * psidown - psiup * psiup) * irp : 1.f;
+    const float sin2psi = 2.f * psidown * psiup * irp;
+    const float cosq1 = .5f * (1.f + cosq * cosq);
+    FplusI = cosq1 * cos2phi * cos2psi - cosq * sin2phi * sin2psi;
+    FcrosI = cosq1 * cos2phi * sin2psi + cosq * sin2phi * cos2psi;
+    FplusII = cosq1 * sin2phi * cos2psi + cosq * cos2phi * sin2psi;
+    FcrosII = cosq1 * sin2phi * sin2psi - cosq * cos2phi * cos2psi;
+    h3 = F.halfsqrt3;
+  } else {
+    FplusI = F.ldc_c2psi;
+    FcrosI = -F.ldc_s2psi;
+    FplusII = F.ldc_s2psi;
+    FcrosII = F.ldc_c2psi;
+    h3 = 1.f;
+  }
+  G.cosorbphs = cosorbphs;
+  const float cn = aff(F.rc), sn = aff(F.rs);
+  const float *q2 = F.nxL2;
+  const float nxL2 = q2[0] + q2[1] * (sa * sa) + q2[2] * (ca * ca) + q2[3] * sa + q2[4] * ca + q2[5] * (sa * ca);
+  const float iN2 = 1.f / (nxL2 * F.nxS2);
+  const float rot0 = 2.f * (cn * cn) * iN2 - 1.f, rot1 = cn * sn * iN2;
+  const float XI = h3 * (FplusI * rot0 - FcrosI * rot1), YI = h3 * (FplusI * rot1 + FcrosI * rot0);
+  const float XII = h3 * (FplusII * rot0 - FcrosII * rot1), YII = h3 * (FplusII * rot1 + FcrosII * rot0);
+  const float pa = -(1.f + Ldotn2) * cos2gam, pb = (1.f + Ldotn2) * sin2gam, pc = 1.f - Ldotn2;
+  const float qa = 2.f * Ldotn * sin2gam, qb = 2.f * Ldotn * cos2gam;
+  G.W[0] = pa * XI + qa * YI;
+  G.W[1] = pb * XI + qb * YI;
+  G.W[2] = pc * XI;
+  G.W[3] = pa * XII + qa * YII;
+  G.W[4] = pb * XII + qb * YII;
+  G.W[5] = pc * XII;
+}
+
 // Planck taper between plunge and zero time (AAK.cc:535-547)
 __device__ inline double taper(const DenseParams &P, int i, double ti) {
   if (i <= P.ip) return 1.0;
@@ -1426,7 +1512,8 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
   const int n_end = min(P.N, P.ip + P.ib);  // samples [0, n_end) carry the signal
   if (modesum_tail(P, B, i, n_end)) return;
   SampleGeom G;
-  sample_geometry(P, B, i, G);
+  if (MIXED) sample_geometry_mixed(P, B, i, G);
+  else sample_geometry(P, B, i, G);
   B.t[i] = G.ti;
   PhaseSeed S;
   phase_seed(P, G, S);
