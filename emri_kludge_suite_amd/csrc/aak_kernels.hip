@@ -365,7 +365,7 @@ __device__ inline long long block_seg_incl(long long v, int f, long long *shv, i
 
 // approximate block sums and prefix ranges
 __global__ __launch_bounds__(kFoldThreads) void k_fold_sums(FoldArgs A, FoldWork W) {
-  __shared__ double sh[kFoldThreads];
+  __shared__ double sh[2 * (kFoldThreads / 64)];
   __shared__ double sd[kFoldLds];
   const int p = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
   const int base = b * kFoldBlock + tid * kFoldPerThread;
@@ -484,11 +484,11 @@ __device__ void fold_serial_replay(const FoldArgs &A, const FoldWork &W, int p, 
 // Classify steps, build the block's item list (PLAN) or write the exact
 // fold values of run steps (WRITE, after the chain resolved the run starts).
 template <bool WRITE>
-__global__ __launch_bounds__(kFoldThreads) void k_fold_local(FoldArgs A, FoldWork W) {
-  __shared__ double shd[kFoldThreads];
-  __shared__ long long shl[kFoldThreads];
+__global__ __launch_bounds__(kFoldThreads, WRITE ? 3 : 4) void k_fold_local(FoldArgs A, FoldWork W) {
+  __shared__ double shd[kFoldThreads / 64];
+  __shared__ long long shl[2 * (kFoldThreads / 64)];
   __shared__ int shi[kFoldThreads];
-  __shared__ int shf[kFoldThreads];
+  __shared__ int shf[kFoldThreads / 64];
   __shared__ double sd[kFoldLds];
   const int p = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
   if (W.status[p]) {
