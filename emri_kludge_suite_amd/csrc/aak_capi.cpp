@@ -161,6 +161,9 @@ struct Lane {
 };
 
 constexpr int kMaxLanes = 4;
+// short plans kept ready or in progress for the GPU by the batch pool (about
+// 2 ms of device work at N = 2^21); other picks take the longest remaining
+constexpr int kFeedDepth = 6;
 
 struct aak_handle_s {
   WaveConfig cfg;
@@ -207,7 +210,7 @@ struct aak_handle_s {
   const double *b_params = nullptr;
   long long b_n = 0, b_next = 0, b_end = 0;  // b_order[b_next, b_end) not yet taken
   std::vector<long long> b_order;  // templates in decreasing estimated host cost
-  int feeders = 1;                 // workers that take the cheapest remaining template
+  int cheap_inflight = 0;          // plans in progress taken from the cheap end (under mu)
   std::atomic<size_t> pin_hint{0};  // largest knot table (doubles) planned by the pool
   size_t knot_hint = 0;             // largest evolution array length planned by the pool (under mu)
   std::vector<size_t> worker_cap;   // evolution array length of each worker (under mu)
@@ -533,15 +536,18 @@ static void worker_main(aak_handle h, int id) {
     RingSlot *slot;
     int sidx;
     size_t grow_to = 0;
+    bool took_cheap = false;
     {
       std::unique_lock<std::mutex> lk(h->mu);
       auto has_work = [&] { return h->b_params && h->b_next < h->b_end && !h->free_slots.empty(); };
       h->cv_work.wait(lk, [&] { return h->stop || has_work() || plan.knots.eA.size() < h->knot_hint; });
       if (h->stop) return;
       if (has_work()) {
-        // feeders keep the GPU supplied with short plans from the start;
-        // the others work down from the longest, which bound the batch
-        i = (id < h->feeders) ? h->b_order[(size_t)--h->b_end] : h->b_order[(size_t)h->b_next++];
+        // keep about kFeedDepth short plans ready or in progress for the GPU;
+        // otherwise work down from the longest, which bound the batch
+        took_cheap = (int)h->ready_slots.size() + h->cheap_inflight < kFeedDepth;
+        if (took_cheap) h->cheap_inflight++;
+        i = took_cheap ? h->b_order[(size_t)--h->b_end] : h->b_order[(size_t)h->b_next++];
         params = h->b_params;
         sidx = h->free_slots.back();
         h->free_slots.pop_back();
@@ -584,6 +590,7 @@ static void worker_main(aak_handle h, int id) {
       slot->rc = rc;
       slot->tpl = i;
       h->ready_slots.push_back(sidx);
+      if (took_cheap) h->cheap_inflight--;
     }
     h->cv_ready.notify_all();
     if (hint_grew) h->cv_work.notify_all();  // idle workers grow to the new size
@@ -610,7 +617,6 @@ static int start_pool(aak_handle h) {
     h->ring.push_back(std::move(slot));
   }
   h->worker_cap.assign((size_t)T, 0);
-  h->feeders = T >= 8 ? T / 4 : 1;
   for (int t = 0; t < T; t++) h->workers.emplace_back(worker_main, h, t);
   return 0;
 }
@@ -861,10 +867,11 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   // Order by estimated host cost: the host stage costs a few ms plus O(K)
   // with K, the knot count, ~ 99 max(1, t_rr / T_fit) where
   // t_rr = length dt mu / (M^2 M_sun) is the observation time in
-  // radiation-reaction units (AAKpy.cc:24-26).  A quarter of the workers take
-  // templates from the cheap end (a steady supply of plans from the first
-  // millisecond), the rest from the expensive end (the longest plans, which
-  // bound the batch, start at once).
+  // radiation-reaction units (AAKpy.cc:24-26).  Workers take templates from
+  // the cheap end while fewer than kFeedDepth plans are ready or in progress
+  // from there (a steady supply for the GPU from the first millisecond), and
+  // from the expensive end otherwise (the longest plans, which bound the
+  // batch, start early).
   std::vector<std::pair<double, long long>> cost((size_t)n);
   for (long long i = 0; i < n; i++) {
     const double M = params[14 * i + 4], mu = params[14 * i + 5];
@@ -887,6 +894,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
     h->b_n = n;
     h->b_next = 0;
     h->b_end = n;
+    h->cheap_inflight = 0;
   }
   h->cv_work.notify_all();
   int err_rc = 0;
