@@ -256,9 +256,17 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
     return g.stable == -1 || g.E > 1;
   };
   int bad = 0;
+  // p_j = (1 - e^2) x^(-2/3) with x = Omega_phi M_phys M_sun (pn_p).  Where
+  // (1 - e^2)^3 >= LSO_max^3 (1 + 1e-9) x^2, p_j >= LSO_max for certain (the
+  // margin dwarfs every rounding), so neither test below can pass and the pow
+  // is skipped; elsewhere p_j is formed exactly as pn_p forms it.
+  const double L3 = LSO_max * LSO_max * LSO_max * (1.0 + 1e-9);
   for (int j = j0; j <= j_max_temp; j++) {
     if (j > j_max) break;
-    const double pj = pn_p(vf[j], ef[j], coslam, Sf[j], Mf[j], M_phys);
+    const double x = PN::omega_phi(vf[j], ef[j], coslam, Sf[j], Mf[j]) * M_phys * kMsunSec;
+    const double ome = 1. - ef[j] * ef[j];
+    if (x > 0. && ome > 0. && ome * ome * ome >= L3 * (x * x)) continue;
+    const double pj = ome * pow(x, -2. / 3.);
     if (pj < LSO_min && j_max == j_max_temp) {
       j_min = std::max(j - j_RR, j0);
       j_max = std::min(j + 1, j_max_temp);
