@@ -278,23 +278,35 @@ constexpr int kFoldLds = kFoldBlock + kFoldBlock / kFoldPerThread;
 static_assert(kFoldLds >= 4096, "the serial replay stages 4096 steps in the staging array");
 __device__ inline int fold_lds_index(int l) { return l + (l >> 4); }
 
-// Every load is issued before the first use: steps past the end read the last
-// step's rates (in bounds) and stage 0.
+// Every load is issued before the first use.  An interior block (all its
+// steps in range, none of them step 0) reads at fixed offsets from one base
+// pointer per lane; other blocks clamp: steps past the end read the last
+// step's rates (in bounds) and stage 0.  DIR is the fold direction (A.dir).
+template <int DIR>
 __device__ inline void stage_increments(const FoldArgs &A, int p, int b, double *sd) {
   const double *rate = A.rate[p];
   double r[kFoldPerThread], rm[kFoldPerThread];
+  if (b > 0 && (b + 1) * kFoldBlock <= A.nsteps) {
+    const double *rp = rate + (A.i0 + DIR * (b * kFoldBlock + (int)threadIdx.x));
 #pragma unroll
-  for (int q = 0; q < kFoldPerThread; q++) {
-    const int s = min(b * kFoldBlock + (int)threadIdx.x + q * kFoldThreads, A.nsteps - 1);
-    const int i = A.i0 + A.dir * s;
-    r[q] = rate[i];
-    rm[q] = rate[s == 0 ? i : i - A.dir];
+    for (int q = 0; q < kFoldPerThread; q++) {
+      r[q] = rp[DIR * q * kFoldThreads];
+      rm[q] = rp[DIR * q * kFoldThreads - DIR];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < kFoldPerThread; q++) {
+      const int s = min(b * kFoldBlock + (int)threadIdx.x + q * kFoldThreads, A.nsteps - 1);
+      const int i = A.i0 + DIR * s;
+      r[q] = rate[i];
+      rm[q] = rate[s == 0 ? i : i - DIR];
+    }
   }
 #pragma unroll
   for (int q = 0; q < kFoldPerThread; q++) {
     const int l = threadIdx.x + q * kFoldThreads;
     const double d = (1.5 * r[q] - .5 * rm[q]) * A.dt;
-    sd[fold_lds_index(l)] = (b * kFoldBlock + l < A.nsteps) ? (A.dir > 0 ? d : -d) : 0.0;
+    sd[fold_lds_index(l)] = (b * kFoldBlock + l < A.nsteps) ? (DIR > 0 ? d : -d) : 0.0;
   }
   __syncthreads();
 }
@@ -364,12 +376,13 @@ __device__ inline long long block_seg_incl(long long v, int f, long long *shv, i
 }
 
 // approximate block sums and prefix ranges
+template <int DIR>
 __global__ __launch_bounds__(kFoldThreads) void k_fold_sums(FoldArgs A, FoldWork W) {
   __shared__ double sh[2 * (kFoldThreads / 64)];
   __shared__ double sd[kFoldLds];
   const int p = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
   const int base = b * kFoldBlock + tid * kFoldPerThread;
-  stage_increments(A, p, b, sd);
+  stage_increments<DIR>(A, p, b, sd);
   double run = 0.0, mn = 0.0, mx = 0.0;
   for (int j = 0; j < kFoldPerThread; j++) {
     const int s = base + j;
@@ -483,7 +496,7 @@ __device__ void fold_serial_replay(const FoldArgs &A, const FoldWork &W, int p, 
 
 // Classify steps, build the block's item list (PLAN) or write the exact
 // fold values of run steps (WRITE, after the chain resolved the run starts).
-template <bool WRITE>
+template <bool WRITE, int DIR>
 __global__ __launch_bounds__(kFoldThreads, WRITE ? 3 : 4) void k_fold_local(FoldArgs A, FoldWork W) {
   __shared__ double shd[kFoldThreads / 64];
   __shared__ long long shl[2 * (kFoldThreads / 64)];
@@ -499,7 +512,7 @@ __global__ __launch_bounds__(kFoldThreads, WRITE ? 3 : 4) void k_fold_local(Fold
   }
   const int base = b * kFoldBlock + tid * kFoldPerThread;
   const double E = W.E[p];
-  stage_increments(A, p, b, sd);
+  stage_increments<DIR>(A, p, b, sd);
   double d[kFoldPerThread];
   double tsum = 0.0;
 #pragma unroll
@@ -646,12 +659,13 @@ __global__ __launch_bounds__(kFoldThreads, WRITE ? 3 : 4) void k_fold_local(Fold
 #pragma unroll
     for (int j = 0; j < kFoldPerThread; j++) sd[fold_lds_index(tid * kFoldPerThread + j)] = outv[j];
     __syncthreads();
+    // step s goes to A.i0 + DIR (s + 1)
+    double *op = A.out[p] + (A.i0 + DIR * (b * kFoldBlock + tid + 1));
 #pragma unroll
     for (int q = 0; q < kFoldPerThread; q++) {
-      const int l = tid + q * kFoldThreads;
-      const int s = b * kFoldBlock + l;
-      const double v = sd[fold_lds_index(l)];
-      if (s < A.nsteps && !__builtin_isnan(v)) A.out[p][out_index(A, s)] = v;
+      const int s = b * kFoldBlock + tid + q * kFoldThreads;
+      const double v = sd[fold_lds_index(tid + q * kFoldThreads)];
+      if (s < A.nsteps && !__builtin_isnan(v)) op[DIR * q * kFoldThreads] = v;
     }
   }
 }
@@ -961,11 +975,14 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
   A.dt = P.dt;
   A.nblk = (nsteps + kFoldBlock - 1) / kFoldBlock;
   if (A.nblk > nblk_cap) return -1;
-  hipLaunchKernelGGL(k_fold_sums, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  if (dir > 0) hipLaunchKernelGGL(k_fold_sums<1>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  else hipLaunchKernelGGL(k_fold_sums<-1>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_plan, dim3(3), dim3(1024), 0, s, A, W);
-  hipLaunchKernelGGL(k_fold_local<false>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  if (dir > 0) hipLaunchKernelGGL((k_fold_local<false, 1>), dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  else hipLaunchKernelGGL((k_fold_local<false, -1>), dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_chain, dim3(3), dim3(kChainThreads), 0, s, A, W);
-  hipLaunchKernelGGL(k_fold_local<true>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  if (dir > 0) hipLaunchKernelGGL((k_fold_local<true, 1>), dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  else hipLaunchKernelGGL((k_fold_local<true, -1>), dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   return 0;
 }
 
