@@ -162,8 +162,11 @@ struct Lane {
 
 constexpr int kMaxLanes = 4;
 // short plans kept ready or in progress for the GPU by the batch pool (about
-// 2 ms of device work at N = 2^21); other picks take the longest remaining
-constexpr int kFeedDepth = 6;
+// 3 ms of device work at N = 2^21); other picks take the longest remaining.
+// Measured over 256-template batches of the production prior on 16 host
+// threads: 6 -> 3.26e3, 8 -> 3.42e3, 10 -> 3.41e3 (FP64 +3%), 12 -> 3.09e3,
+// 16 -> 2.63e3 likelihoods/s (too few threads left for the longest plans).
+constexpr int kFeedDepth = 10;
 
 struct aak_handle_s {
   WaveConfig cfg;
