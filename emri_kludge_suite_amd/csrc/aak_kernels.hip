@@ -1399,6 +1399,7 @@ struct PhaseSeed {
   double ur, ui, qr, qi, dA, dB;
 };
 
+template <bool MIXED>
 __device__ inline void phase_seed(const DenseParams &P, const SampleGeom &G, PhaseSeed &S) {
   const double kq = rint(G.Phi * (1.0 / (2.0 * kPi)));
   double phr = fma(-kq, 6.283185307179586, G.Phi);
@@ -1412,7 +1413,16 @@ __device__ inline void phase_seed(const DenseParams &P, const SampleGeom &G, Pha
     S.dB = 2. * G.gimdot * proj;
   }
   sincos(phr + S.dA, &S.ui, &S.ur);
-  sincos(S.dB, &S.qi, &S.qr);
+  if (MIXED) {
+    // the seed of the FP32 loop: dB is a small angle (the Doppler shift of
+    // the periapsis precession), so FP32 keeps its rotation to ~6e-8
+    float qi, qr;
+    sincosf((float)S.dB, &qi, &qr);
+    S.qi = qi;
+    S.qr = qr;
+  } else {
+    sincos(S.dB, &S.qi, &S.qr);
+  }
 }
 
 // FP64 harmonic loop at Taylor order PO.  The response factors are the same
@@ -1533,7 +1543,7 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
   else sample_geometry(P, B, i, G);
   B.t[i] = G.ti;
   PhaseSeed S;
-  phase_seed(P, G, S);
+  phase_seed<MIXED>(P, G, S);
   const double d = G.e - B.ms_eref[eb];
   const size_t base = (size_t)eb * P.nmodes * kCoefStride;
   double sum[3];
