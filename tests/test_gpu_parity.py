@@ -160,6 +160,37 @@ def test_serial_fold_replay_is_bit_identical():
     assert _rel_l2(hI[:N], ref["hI"]) <= 1e-10 and _rel_l2(hII[:N], ref["hII"]) <= 1e-10
 
 
+def test_likelihood_parity_non_power_of_two_length():
+    # lengths the fused FFT does not take run rocFFT and the separate
+    # weighted reduction; same bar
+    from emri_kludge_suite_amd import GPUAAK
+    from emri_kludge_suite_amd.LISAnoise import LISA_Noise, generate_noise_frequencies
+    N, dt = 3 * 2 ** 15, 10.0
+    print("case likelihood-rocfft N=%d" % N, flush=True)
+    pars = _config1(4)
+    ref = ol.o_aak_wave(pars, length=N, dt=dt)
+    freqs = generate_noise_frequencies(N * dt, 1 / dt)
+    freqs[0] = freqs[1] / 10.0
+    w = 1.0 / np.sqrt(LISA_Noise(freqs, dur=1)) * np.sqrt(dt / N)
+    pars2 = list(pars)
+    pars2[6] += 0.2
+    ref2 = ol.o_aak_wave(pars2, length=N, dt=dt)
+    dI = np.fft.rfft(ref2["hI"]) * w
+    dII = np.fft.rfft(ref2["hII"]) * w
+    g = GPUAAK(1.0, N // 10, N, 100.0, dt, True, False)
+    g.input_data(dI, dII, w, w)
+    like = g.WaveformThroughLikelihood(*pars)
+    batch = g.LikelihoodBatch(np.array([pars, pars2]))
+    hI = np.fft.rfft(ref["hI"]) * w
+    hII = np.fft.rfft(ref["hII"]) * w
+    dh = 4 * np.sum(np.real(np.conj(hI) * dI) + np.real(np.conj(hII) * dII))
+    hh = 4 * np.sum(np.abs(hI) ** 2 + np.abs(hII) ** 2)
+    print("likelihood", like, dh, hh, flush=True)
+    assert abs(like[0] - dh) <= 1e-9 * abs(hh)
+    assert abs(like[1] - hh) <= 1e-9 * abs(hh)
+    assert np.array_equal(batch[0], like)
+
+
 def test_high_eccentricity_parity():
     # e0 = 0.7 lies above the series range: the mode sum takes the Miller path
     pars = [0.5, 0.6, 12.0, 0.7, 1e6, 10.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
