@@ -920,7 +920,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
       if (prof) h->evs = h->batch_ev.data() + 6 * i;
       // templates alternate over the lanes; each lane alternates its two table slots
       Lane &L = h->lane[done % nl];
-      slot.P.store_vms = 0;  // likelihoods need e, the rates, phases and amplitude only
+      slot.P.full_dense = 0;  // likelihoods read e, the rates, phases and amplitude up to the plunge
       rc = enqueue_waveform(h, L, slot.P, slot.pin.p, slot.tab_n, (int)((done / nl) & 1), slot.copied);
       if (!rc) {
         slot.copy_pending = true;

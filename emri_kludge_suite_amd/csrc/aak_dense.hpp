@@ -114,7 +114,8 @@ struct DenseParams {
   double t_plunge, t_zero;
   KnotGrid grid;
   int mixed;                      // 1: FP32 harmonic loop (mixed precision)
-  int store_vms;                  // 1: keep v, M, S per sample (read by no later stage)
+  int full_dense;                 // 1: every dense array at every sample (single waveforms: phase
+                                  // output, dense arrays); 0: only what a likelihood reads
   GeomF32 gf;                     // response constants in FP32 (mixed form)
   int bessel_series;              // 1: series + recurrence, 0: Miller from above
   unsigned char bterms[kMaxModes + 1];  // series length per harmonic ([0]: orders 0..4)

@@ -674,7 +674,7 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   const double e_bound = e_hi * 1.01 + 0.01;
   P.bessel_series = (nmodes <= kMaxModes && e_bound <= kSeriesMaxE) ? 1 : 0;
   P.mixed = cfg.mixed ? 1 : 0;
-  P.store_vms = 1;
+  P.full_dense = 1;
   {
     GeomF32 &F = P.gf;
     for (int k = 0; k < 3; k++) {

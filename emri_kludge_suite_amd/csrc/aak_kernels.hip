@@ -79,6 +79,11 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   const int tid = threadIdx.x;
   const int i_lo = blockIdx.x * kRatesThreads;
   if (i_lo > P.N) return;
+  // likelihoods read the rates up to the last folded step and e and the
+  // amplitude below the end of the signal (<= last_fold + 1); past that a
+  // source that plunged needs nothing (single waveforms keep every sample
+  // for the phase output and the dense arrays)
+  if (!P.full_dense && i_lo > P.last_fold + 1) return;
   const int i_hi = min(i_lo + kRatesThreads - 1, P.N);
   const KnotGrid &g = P.grid;
   const int K = g.K;
@@ -131,7 +136,7 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   }
   const double e = y[0], v = y[1], M = y[2], S = y[3];
   B.e[i] = e;
-  if (P.store_vms) {
+  if (P.full_dense) {
     B.v[i] = v;
     B.M[i] = M;
     B.S[i] = S;
