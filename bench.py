@@ -199,7 +199,7 @@ def main():
         elapsed = time.perf_counter() - t0
         assert np.all(np.isfinite(like)), "bench: a template failed"
         host_plan[mode] = {"ms_per_template": host[0] / (args.steps * len(P)), "longest_ms": host[1] / args.steps,
-                           "pool_threads": int(os.environ.get("AAK_HOST_THREADS", "0")) or None}
+                           "pool_threads": int(os.environ.get("AAK_HOST_THREADS", "0")) or min(os.cpu_count() or 4, 16)}
         # stage times: on one lane (no overlap with another template's kernels)
         # and as they run concurrently on the bench's lanes
         return elapsed, profiled_pass(1), profiled_pass(LANES), like
