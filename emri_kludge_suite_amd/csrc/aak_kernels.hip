@@ -1192,8 +1192,22 @@ __device__ inline float reduce_2pi_f(double x) {
   return (float)fma(-k, 6.283185307179586, x);
 }
 
+// Hardware sine/cosine (v_sin_f32, v_cos_f32; absolute error ~1e-6 for
+// |x| <= 4 pi) and reciprocal (v_rcp_f32, 1 ulp): the response factors they
+// feed set amplitudes, where such errors are far below the FP32 harmonic
+// sum's own.
+__device__ inline void sincos_hw(float x, float *s, float *c) {
+  *s = __sinf(x);
+  *c = __cosf(x);
+}
+__device__ inline float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // The mixed form's per-sample geometry: the same response as sample_geometry
-// in FP32, after the large angles (gamma, alpha) are reduced mod 2 pi in FP64.
+// in FP32, after the large angles (gamma, alpha, the orbital phase) are
+// reduced mod 2 pi in FP64.  2 beta enters only through cos and sin of
+// 2 (gamma + beta), formed from the double-angle identities of
+// beta = atan2(y, x) (cos 2 beta = (x^2 - y^2) / r^2, sin 2 beta = 2 x y / r^2;
+// 1 and 0 at x = y = 0, where atan2 gives 0 or pi).
 // The phase, its rates, e, t and the amplitude are read as they are.
 __device__ inline void sample_geometry_mixed(const DenseParams &P, const DenseBuffers &B, int i, SampleGeom &G) {
 #pragma clang fp contract(fast)
@@ -1207,7 +1221,7 @@ __device__ inline void sample_geometry_mixed(const DenseParams &P, const DenseBu
   G.Amp = B.amp[i];
   const float gim = reduce_2pi_f(B.phase[0][i]), alp = reduce_2pi_f(B.phase[2][i]);
   float sa, ca;
-  sincosf(alp, &sa, &ca);
+  sincos_hw(alp, &sa, &ca);
   auto aff = [&](const float *c) { return c[0] + c[1] * sa + c[2] * ca; };
   const float cosqL = aff(F.cosqL);
   const float sinqL = sqrtf(fmaxf(1.f - cosqL * cosqL, 0.f));
@@ -1217,27 +1231,31 @@ __device__ inline void sample_geometry_mixed(const DenseParams &P, const DenseBu
   const float cphiL = rL2 > 0.f ? phiLdown * irL : 1.f, sphiL = phiLup * irL;
   const float Ldotn = cosqL * F.cosqS + sinqL * F.sinqS * (cphiL * F.cos_phS + sphiL * F.sin_phS);
   const float Ldotn2 = Ldotn * Ldotn;
-  const float beta = atan2f(-F.Sdotn + F.coslam * Ldotn, aff(F.betadown));
-  float sin2gam, cos2gam;
-  sincosf(2.f * (gim + beta), &sin2gam, &cos2gam);
+  const float by = -F.Sdotn + F.coslam * Ldotn, bx = aff(F.betadown);
+  const float rb2 = bx * bx + by * by;
+  const float irb2 = rb2 > 0.f ? rcp_hw(rb2) : 0.f;
+  const float c2b = rb2 > 0.f ? (bx * bx - by * by) * irb2 : 1.f, s2b = 2.f * bx * by * irb2;
+  float s2g, c2g;
+  sincos_hw(2.f * gim, &s2g, &c2g);
+  const float sin2gam = s2g * c2b + c2g * s2b, cos2gam = c2g * c2b - s2g * s2b;
   float cosorbphs = 0.f, FplusI, FcrosI, FplusII, FcrosII, h3;
   if (P.mich) {
     float so, co;
-    sincosf((float)(2. * kPi * ti / kYear), &so, &co);
+    sincos_hw(reduce_2pi_f(2. * kPi * ti / kYear), &so, &co);
     cosorbphs = co * F.cos_phS + so * F.sin_phS;
     const float sinorbphs = so * F.cos_phS - co * F.sin_phS;
     const float cosorbK = co * F.cos_phK + so * F.sin_phK;
     const float cosq = .5f * F.cosqS - F.halfsqrt3 * F.sinqS * cosorbphs;
     const float Yw = F.halfsqrt3 * F.cosqS + .5f * F.sinqS * cosorbphs, Xw = F.sinqS * sinorbphs;
     const float rw = Xw * Xw + Yw * Yw;
-    const float irw = rw > 0.f ? 1.f / rw : 0.f;
+    const float irw = rw > 0.f ? rcp_hw(rw) : 0.f;
     const float c2w = rw > 0.f ? (Xw * Xw - Yw * Yw) * irw : 1.f, s2w = 2.f * Xw * Yw * irw;
     const float c2o = co * co - so * so, s2o = 2.f * so * co;
     const float cos2phi = c2o * c2w - s2o * s2w, sin2phi = s2o * c2w + c2o * s2w;
     const float psiup = .5f * F.cosqK - F.halfsqrt3 * F.sinqK * cosorbK - cosq * F.psi_c0;
     const float psidown = F.psi_c1 - F.halfsqrt3 * co * F.psi_c2 - F.halfsqrt3 * so * F.psi_c3;
     const float rp = psiup * psiup + psidown * psidown;
-    const float irp = rp > 0.f ? 1.f / rp : 0.f;
+    const float irp = rp > 0.f ? rcp_hw(rp) : 0.f;
     const float cos2psi = rp > 0.f ? (psidown * psidown - psiup * psiup) * irp : 1.f;
     const float sin2psi = 2.f * psidown * psiup * irp;
     const float cosq1 = .5f * (1.f + cosq * cosq);
@@ -1257,7 +1275,7 @@ __device__ inline void sample_geometry_mixed(const DenseParams &P, const DenseBu
   const float cn = aff(F.rc), sn = aff(F.rs);
   const float *q2 = F.nxL2;
   const float nxL2 = q2[0] + q2[1] * (sa * sa) + q2[2] * (ca * ca) + q2[3] * sa + q2[4] * ca + q2[5] * (sa * ca);
-  const float iN2 = 1.f / (nxL2 * F.nxS2);
+  const float iN2 = rcp_hw(nxL2 * F.nxS2);
   const float rot0 = 2.f * (cn * cn) * iN2 - 1.f, rot1 = cn * sn * iN2;
   const float XI = h3 * (FplusI * rot0 - FcrosI * rot1), YI = h3 * (FplusI * rot1 + FcrosI * rot0);
   const float XII = h3 * (FplusII * rot0 - FcrosII * rot1), YII = h3 * (FplusII * rot1 + FcrosII * rot0);
@@ -1417,15 +1435,24 @@ __device__ inline void phase_seed(const DenseParams &P, const SampleGeom &G, Pha
     S.dA = 2. * kPi * G.nu * proj;
     S.dB = 2. * G.gimdot * proj;
   }
-  sincos(phr + S.dA, &S.ui, &S.ur);
   if (MIXED) {
-    // the seed of the FP32 loop: dB is a small angle (the Doppler shift of
-    // the periapsis precession), so FP32 keeps its rotation to ~6e-8
+    // the seeds of the FP32 loop.  u: sincosf of the FP32 angle, corrected to
+    // first order by the angle's FP32 remainder (|remainder| <= 1.2e-7, so
+    // the dropped second-order term is below 1e-14).  dB is a small angle (the
+    // Doppler shift of the periapsis precession), so FP32 keeps its rotation
+    // to ~6e-8.
+    const double th = phr + S.dA;
+    const float thf = (float)th, tl = (float)(th - (double)thf);
+    float su, cu;
+    sincosf(thf, &su, &cu);
+    S.ui = fmaf(cu, tl, su);
+    S.ur = fmaf(-su, tl, cu);
     float qi, qr;
     sincosf((float)S.dB, &qi, &qr);
     S.qi = qi;
     S.qr = qr;
   } else {
+    sincos(phr + S.dA, &S.ui, &S.ur);
     sincos(S.dB, &S.qi, &S.qr);
   }
 }
