@@ -149,7 +149,7 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+        print("bench.py: measures one GPU; --gpus %d is not supported "
               "(waveform batches shard across GPUs without communication)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (emri_kludge_suite_amd/)
  * includes, links or calls this code.  Only tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg load liboracle.so, as the checker.
+ * and bench.py's cpu_baseline leg load liboracle.so, to check results or
+ * time the CPU baseline.
  *
  * The functions restate, in plain C, the behaviour of the reference CPU AAK
  * (AAKwrapper.wave -> AAKwave -> GenWave -> waveform -> PNevolution) and of

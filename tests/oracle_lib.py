@@ -2,7 +2,8 @@
 reference's own trajectory subset (oracle/_ref/libaakref.so).
 
 Test infrastructure: used by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker, never by the product.
+cpu_baseline leg of bench.py, to check results or time the CPU baseline;
+never by the product.
 """
 import ctypes as C
 import os

@@ -1,6 +1,6 @@
 """GPU parity: the HIP path against the CPU oracle on identical inputs.
 
-Tolerances (north star): h_I/h_II relative L2 <= 1e-10 in plain FP64; inner
+Tolerances: h_I/h_II relative L2 <= 1e-10 in plain FP64; inner
 product relative error <= 1e-9.  The host stage is bit-identical to the oracle
 (checked on the CPU in test_host_parity.py), so the dense arrays are compared
 bit for bit as well: the spline values and the phase rates must be identical,
