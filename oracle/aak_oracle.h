@@ -13,6 +13,12 @@
  * published method is restated instead (natural cubic spline, Powell hybrid,
  * least squares, Miller recurrence for J_n); those endpoints are pinned by
  * scipy/numpy in tests/, not by the reference itself (GSL is absent here).
+ *
+ * Parity unpinned (needs the GSL-linked reference, which cannot be built
+ * here): the exact endpoint of GSL's hybrid solver in the frequency map (the
+ * restatement stops on the same residual test), the bit patterns of GSL's
+ * spline, least squares and Bessel routines, and hence the end-to-end output
+ * of the reference's AAKwrapper.wave.
  */
 #ifndef AAK_ORACLE_H
 #define AAK_ORACLE_H
