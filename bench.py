@@ -224,6 +224,11 @@ def main():
     single_elapsed = time.perf_counter() - t1
     g.set_profiling(False)
     value_single = args.single / single_elapsed
+    # waveforms alone (host stage + dense stage + mode sum into HBM), FP64, one per call
+    t1 = time.perf_counter()
+    for k in range(args.single):
+        g.gpu_gen_AAK(*templates[k % len(templates)])
+    wave_elapsed = time.perf_counter() - t1
 
     # accuracy of the mixed form: noise-weighted mismatch against FP64 waveforms
     mism = []
@@ -286,6 +291,8 @@ def main():
                              "ms_per_likelihood": 1e3 * single_elapsed / args.single,
                              "stages_ms": dict(zip(("host",) + stages,
                                                    [round(x / args.single, 4) for x in single_stage]))},
+        "single_waveform_fp64": {"value": args.single / wave_elapsed, "unit": "waveforms/s",
+                                 "ms_per_waveform": 1e3 * wave_elapsed / args.single},
         "speedup": {"mixed_batched_over_single_fp64": value / value_single,
                     "fp64_batched_over_single_fp64": value_64 / value_single},
         "roofline": {"bound": "mfma", "achieved": ach_mx, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -57,13 +57,20 @@ def test_batch_equals_single_calls():
     like, st = g.LikelihoodBatch(P, return_status=True)
     print("status", st.tolist(), flush=True)
     assert st[4] < 0 and np.all(np.isnan(like[4]))
+    plunged = 0
     for i in range(len(P)):
         if st[i] < 0:
             with pytest.raises(RuntimeError):
                 g.WaveformThroughLikelihood(*P[i])
             continue
         single = g.WaveformThroughLikelihood(*P[i])
+        inf = g.info()
+        # batches compute rates only up to the last folded sample; single
+        # calls compute them everywhere
+        plunged += inf["i_plunge"] + inf["i_buffer"] < N2 - 1
         assert np.array_equal(like[i], single), (i, like[i], single)
+    print("plunging templates", plunged, flush=True)
+    assert plunged >= 1
 
 
 @pytest.mark.timeout(300)
