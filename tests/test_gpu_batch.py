@@ -213,3 +213,32 @@ def test_mixed_batch_equals_mixed_single_calls():
         assert np.array_equal(like[i], g.WaveformThroughLikelihood(*P[i]))
         # the likelihood terms agree with the FP64 path to well within the mismatch budget
         assert abs(like[i][1] - ref[i][1]) <= 1e-5 * abs(ref[i][1])
+
+
+@pytest.mark.timeout(300)
+def test_batch_edge_cases():
+    # no data yet, an empty batch, a batch whose every host stage fails, and
+    # a one-template batch on the same handle afterwards
+    print("case batch-edges N=2^18 dt=10 n_max=10 seeds 5/6", flush=True)
+    from emri_kludge_suite_amd import GPUAAK
+    N, dt = 2 ** 18, 10.0
+    g0 = GPUAAK(1.0, N // 10, N, 100.0, dt, True, False)
+    with pytest.raises(RuntimeError):
+        g0.LikelihoodBatch(_prior(np.random.default_rng(5), 1))
+    g = _handle_with_data(N, dt, 10)
+    assert g.LikelihoodBatch(np.zeros((0, 14))).shape == (0, 2)
+    P = _prior(np.random.default_rng(6), 3)
+    bad = P.copy()
+    bad[:, 2] = 3.0  # p0 inside the LSO
+    like, st = g.LikelihoodBatch(bad, return_status=True)
+    assert np.all(st < 0) and np.all(np.isnan(like))
+    rng = np.random.default_rng(7)
+    while True:
+        tpl = _prior(rng, 1)
+        try:
+            single = g.WaveformThroughLikelihood(*tpl[0])
+            break
+        except RuntimeError:
+            continue
+    one = g.LikelihoodBatch(tpl)
+    assert np.array_equal(one[0], single)
