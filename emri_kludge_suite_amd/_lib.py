@@ -59,6 +59,7 @@ def lib():
         "aak_get_batch_host_ms": (_i, [C.c_void_p, _pd, _pd]),
         "aak_set_profiling": (_i, [C.c_void_p, _i]),
         "aak_get_timings": (_i, [C.c_void_p, _pd, _i]),
+        "aak_get_pool_stats": (_i, [C.c_void_p, _pd, _i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -24,12 +24,14 @@
 // GPU works on earlier ones.
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
+#include <sched.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -66,6 +68,47 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 std::once_flag g_fft_once;
+
+// The host stage grows its evolution arrays with the knot count (2 j0
+// doubles each, j0 up to 4e8): an allocation failure becomes a status code
+// instead of an exception crossing the C boundary or ending a pool thread.
+int plan_safe(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *plan, std::string *err,
+              TableSink *sink = nullptr) noexcept {
+  try {
+    return plan_waveform(cfg, sp, plan, err, sink);
+  } catch (const std::bad_alloc &) {
+    *err = "host memory exhausted by the knot arrays";
+  } catch (...) {
+    *err = "host stage failed";
+  }
+  return -9;
+}
+
+// Idle pool workers pre-grow their evolution arrays to the largest plan seen,
+// so that later batches do not allocate, but never beyond this many entries
+// (7 arrays x 2^21 doubles = 117 MB per worker); a longer plan grows the
+// arrays of the worker that runs it only.
+constexpr size_t kIdleGrowCap = size_t(1) << 21;
+
+// Threads the process may run on: its CPU affinity, bounded by a cgroup v2
+// CPU quota when one is set.
+int cpu_share() {
+  int n = 0;
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof set, &set) == 0) n = CPU_COUNT(&set);
+  if (n <= 0) n = (int)std::thread::hardware_concurrency();
+  if (FILE *f = std::fopen("/sys/fs/cgroup/cpu.max", "r")) {
+    char q[32] = {0};
+    long long period = 0;
+    if (std::fscanf(f, "%31s %lld", q, &period) == 2 && std::strcmp(q, "max") != 0 && period > 0) {
+      const long long quota = std::atoll(q);
+      const int lim = (int)((quota + period - 1) / period);
+      if (lim > 0 && lim < n) n = lim;
+    }
+    std::fclose(f);
+  }
+  return n > 0 ? n : 4;
+}
 
 // Pinned host buffer that grows on demand.
 struct PinnedBuf {
@@ -199,6 +242,7 @@ struct aak_handle_s {
   double stage_ms[5] = {0, 0, 0, 0, 0};
   double batch_ms[5] = {0, 0, 0, 0, 0};  // stage sums over the last profiled batch
   double batch_host_sum = 0.0, batch_host_max = 0.0;  // host stage ms over the last batch
+  double batch_wall_ms = 0.0;                         // wall time of the last batch call
   int batch_profiled = 0;
   double host_ms = 0.0;
   // batch machinery
@@ -501,10 +545,10 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
 
 static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_spectrum) {
   if (h->fused_fft) {
-    launch_fft_pass1(h->fft, L.d_h, L.d_fy, L.stream);
+    launch_fft_pass1(h->fft, L.d_h, L.d_h + h_stride(h->N), L.d_fy, L.stream);
     h->mark(L, 4);
     launch_fft_pass2(h->fft, L.d_fy, L.d_partials, keep_spectrum ? L.d_hf : nullptr, L.stream);
-    launch_dot_final(L.d_partials, fft_partials(h->fft), out_dev, L.stream);
+    launch_dot_final(L.d_partials, fft_partials(h->fft), out_dev, L.d_flags, L.stream);
     h->mark(L, 5);
     HIPCHK(hipGetLastError());
     return 0;
@@ -514,7 +558,7 @@ static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_
   if (rocfft_execute(L.plan, in, outb, L.exec) != rocfft_status_success) return fail(-101, "rocfft execute");
   h->mark(L, 4);
   launch_weighted_dot(L.d_hf, h->d_data, h->d_w, h->nbins, keep_spectrum ? L.d_hf : nullptr, L.d_partials, out_dev,
-                      L.stream);
+                      L.d_flags, L.stream);
   h->mark(L, 5);
   HIPCHK(hipGetLastError());
   return 0;
@@ -533,6 +577,7 @@ static void worker_main(aak_handle h, int id) {
   // the evolution arrays of the worker's plans, reused across templates and
   // batches; the slot keeps the parameters and the knot table
   WaveformPlan plan;
+  size_t grow_failed = SIZE_MAX;  // idle growth to this size failed: not retried
   for (;;) {
     long long i;
     const double *params;
@@ -543,7 +588,9 @@ static void worker_main(aak_handle h, int id) {
     {
       std::unique_lock<std::mutex> lk(h->mu);
       auto has_work = [&] { return h->b_params && h->b_next < h->b_end && !h->free_slots.empty(); };
-      h->cv_work.wait(lk, [&] { return h->stop || has_work() || plan.knots.eA.size() < h->knot_hint; });
+      auto idle_target = [&] { return std::min(h->knot_hint, kIdleGrowCap); };
+      auto wants_growth = [&] { return plan.knots.eA.size() < idle_target() && idle_target() < grow_failed; };
+      h->cv_work.wait(lk, [&] { return h->stop || has_work() || wants_growth(); });
       if (h->stop) return;
       if (has_work()) {
         // keep about kFeedDepth short plans ready or in progress for the GPU;
@@ -556,15 +603,25 @@ static void worker_main(aak_handle h, int id) {
         h->free_slots.pop_back();
         slot = h->ring[(size_t)sidx].get();
       } else {
-        grow_to = h->knot_hint;
+        grow_to = idle_target();
       }
     }
     if (grow_to) {
-      // idle: grow to the largest evolution planned so far, outside any template
-      reserve_knots(&plan.knots, grow_to);
+      // idle: grow to the largest evolution planned so far (up to
+      // kIdleGrowCap), outside any template; on failure the worker's next plan
+      // grows what it needs, or reports the failure
+      bool grown = true;
+      try {
+        reserve_knots(&plan.knots, grow_to);
+      } catch (const std::bad_alloc &) {
+        grown = false;
+        grow_failed = grow_to;
+      }
       {
         std::lock_guard<std::mutex> lk(h->mu);
-        h->worker_cap[(size_t)id] = plan.knots.eA.size();
+        // a failed growth reports the target as reached, so the end of a batch
+        // does not wait for it
+        h->worker_cap[(size_t)id] = grown ? plan.knots.eA.size() : grow_to;
       }
       h->cv_ready.notify_all();
       continue;
@@ -578,7 +635,7 @@ static void worker_main(aak_handle h, int id) {
     PinnedSink sink(&slot->pin, &h->pin_hint);
     std::string err;
     const auto t_plan = std::chrono::steady_clock::now();
-    const int rc = plan_waveform(h->cfg, sp, &plan, &err, &sink);
+    const int rc = plan_safe(h->cfg, sp, &plan, &err, &sink);
     slot->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
     slot->P = plan.P;
     slot->tab_n = plan.tab_n;
@@ -606,11 +663,7 @@ static int start_pool(aak_handle h) {
   if (T <= 0) {
     const char *env = std::getenv("AAK_HOST_THREADS");
     T = env ? std::atoi(env) : 0;
-    if (T <= 0) {
-      const unsigned hc = std::thread::hardware_concurrency();
-      T = hc ? (int)hc : 4;
-      if (T > 16) T = 16;
-    }
+    if (T <= 0) T = cpu_share();
   }
   const int R = T + 4;
   h->ring.clear();
@@ -663,7 +716,6 @@ int aak_create(double T_fit, int init_length, int length, double init_dt, double
       h->fft.M = length / 2;
       h->fft.R = R;
       h->fft.C = C;
-      h->fft.hs = (int)h_stride(length);
     }
   }
   const int rc = alloc_handle(h);
@@ -761,7 +813,7 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   // the staging buffer is free: every call ends with the compute stream idle,
   // and the compute stream waited for the upload
   PinnedSink sink(&h->single_pin);
-  int rc = plan_waveform(h->cfg, sp, &plan, &err, &sink);
+  int rc = plan_safe(h->cfg, sp, &plan, &err, &sink);
   h->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (rc < 0) {
     h->state = 0;
@@ -774,6 +826,10 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   HIPCHK(hipMemcpyAsync(flags, L.d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, L.stream));
   HIPCHK(hipStreamSynchronize(L.stream));
   read_stage_times(h, 0, 3);
+  if (flags[3]) {
+    h->state = 0;
+    return fail(-3, "non-positive radial frequency at t0");
+  }
   h->state = 1;
   const DenseParams &P = plan.P;
   int *I = h->info;
@@ -799,7 +855,7 @@ int aak_gen_phases(aak_handle h, const double *params, double *t, double *phase_
   WaveformPlan &plan = h->single;
   std::string err;
   PinnedSink sink(&h->single_pin);
-  int rc = plan_waveform(h->cfg, sp, &plan, &err, &sink);
+  int rc = plan_safe(h->cfg, sp, &plan, &err, &sink);
   if (rc < 0) {
     h->state = 0;
     return fail(rc, err);
@@ -844,6 +900,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   if (!h->have_data) return fail(-2, "likelihood batch before input_data");
   if (n < 0 || (n > 0 && (!params || !like_out))) return fail(-2, "bad batch arguments");
   if (n == 0) return 0;
+  const auto t_call = std::chrono::steady_clock::now();
   int rc = start_pool(h);
   if (rc) return rc;
   const int nl = h->nlanes;
@@ -964,14 +1021,16 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
       if (slot->pin.cap < hint && slot->pin.reserve(hint)) return fail(-100, "pinned host allocation failed");
     // and every worker's evolution arrays at the largest planned so far
     std::unique_lock<std::mutex> lk(h->mu);
+    const size_t target = std::min(h->knot_hint, kIdleGrowCap);
     h->cv_ready.wait(lk, [&] {
       for (size_t c : h->worker_cap)
-        if (c < h->knot_hint) return false;
+        if (c < target) return false;
       return true;
     });
   }
   h->batch_host_sum = host_sum;
   h->batch_host_max = host_max;
+  h->batch_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call).count();
   if (prof) {
     for (int k = 0; k < 5; k++) h->batch_ms[k] = 0.0;
     float ms = 0.f;
@@ -985,6 +1044,9 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   }
   const double qnan = std::nan("");
   for (long long i = 0; i < n; i++) {
+    // a NaN result of a planned template: rejected on the device (non-positive
+    // radial rate at t = 0, as the single call reports it)
+    if (st[i] == 0 && std::isnan(h->like_host.p[2 * i + 1])) st[i] = -3;
     like_out[2 * i] = st[i] ? qnan : h->like_host.p[2 * i];
     like_out[2 * i + 1] = st[i] ? qnan : h->like_host.p[2 * i + 1];
   }
@@ -1040,6 +1102,16 @@ int aak_get_batch_host_ms(aak_handle h, double *sum_ms, double *max_ms) {
   return 0;
 }
 
+int aak_get_pool_stats(aak_handle h, double *out, int n) {
+  if (!h) return fail(-1, "null handle");
+  const double threads = (double)h->workers.size();
+  const double busy = (threads > 0 && h->batch_wall_ms > 0) ? h->batch_host_sum / (threads * h->batch_wall_ms) : 0.0;
+  const double v[5] = {threads, h->batch_host_sum, h->batch_host_max, h->batch_wall_ms, busy};
+  const int m = n < 5 ? n : 5;
+  for (int i = 0; i < m; i++) out[i] = v[i];
+  return m;
+}
+
 int aak_get_info(aak_handle h, int *info, int n) {
   if (!h) return fail(-1, "null handle");
   const int m = n < 10 ? n : 10;
@@ -1060,7 +1132,7 @@ static int test_plan(const double *params, double T_fit, int length, double dt, 
   SourceParams sp;
   std::memcpy(&sp, params, sizeof(double) * 14);
   std::string err;
-  const int rc = plan_waveform(cfg, sp, plan, &err);
+  const int rc = plan_safe(cfg, sp, plan, &err);
   return rc < 0 ? fail(rc, err) : 0;
 }
 

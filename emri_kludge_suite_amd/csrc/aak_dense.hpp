@@ -167,7 +167,8 @@ struct DenseBuffers {
   double *nu0;                     // [1] radial frequency at sample 0 from the PN maps
   double *h, *h2;                  // time domain hI, hII: N+2 samples each, h2 = h + h_stride(N)
   double *t;                       // [N+2]
-  int *flags;                      // [4]: [0] non-positive Phidot count, [1] fold replays, [2] per-sample mode-sum blocks
+  int *flags;                      // [4]: [0] non-positive Phidot count, [1] fold replays, [2] per-sample mode-sum
+                                   // blocks, [3] non-positive Phidot at t = 0 (waveform rejected)
   int *guard_blk;                  // [guard blocks]: last positive-Phidot index per block
   double *ms_coef;                 // per expansion block nmodes x 3 x (kTaylorMax+1) doubles; order p uses [nmodes][3][p+1] (float in mixed mode)
   int *ms_p;                       // [expansion blocks] Taylor order (-1: per-sample path)
@@ -188,9 +189,11 @@ int modesum_blocks(const DenseParams &P);  // mode-sum blocks (256 samples each)
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 void launch_clear_flags(const DenseBuffers &B, hipStream_t s);
 void launch_phase_out(const DenseParams &P, const DenseBuffers &B, double *out, hipStream_t s);
+// flags (optional): the waveform's counters; flags[3] != 0 (rejected on the
+// device) makes both outputs NaN.
 void launch_weighted_dot(const double2 *hf, const double2 *data, const double *w, int nbins, double2 *hw_out,
-                         double *partials, double *out, hipStream_t s);
+                         double *partials, double *out, const int *flags, hipStream_t s);
 // 4 x the sums of nparts (dh, hh) pairs into out[0..1].
-void launch_dot_final(const double *partials, int nparts, double *out, hipStream_t s);
+void launch_dot_final(const double *partials, int nparts, double *out, const int *flags, hipStream_t s);
 
 }  // namespace aak

@@ -188,8 +188,8 @@ template <int L>
 constexpr int fft_threads() { return L / 16 > 64 ? L / 16 : 64; }
 
 template <int LR>
-__global__ __launch_bounds__(col_cols<LR>() * fft_threads<LR>()) void k_fft_cols(FftArgs F, const double *h,
-                                                                                  double2 *Y) {
+__global__ __launch_bounds__(col_cols<LR>() * fft_threads<LR>()) void k_fft_cols(FftArgs F, const double *h0,
+                                                                                  const double *h1, double2 *Y) {
   constexpr int CC = col_cols<LR>(), CS = col_stride<LR>(), NT = fft_threads<LR>();
   __shared__ double2 s[CC * CS];
   int ch = blockIdx.y, grp = blockIdx.x;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(col_cols<LR>() * fft_threads<LR>()) void k_fft_cols
   const int b0 = grp * CC, tid = threadIdx.x;
   constexpr int PT = (LR + NT - 1) / NT;  // elements per thread in the staging loops
   constexpr bool kFull = LR % NT == 0;
-  const double2 *z = reinterpret_cast<const double2 *>(h + (size_t)ch * F.hs);
+  const double2 *z = reinterpret_cast<const double2 *>(ch ? h1 : h0);
   {
     // every load in flight before the first LDS store
     double2 v[PT];
@@ -335,9 +335,9 @@ __global__ __launch_bounds__(2 * fft_threads<LC>()) void k_fft_rows(FftArgs F, c
 
 // ---------------------------------------------------------------- launch
 template <int LR>
-static void launch_cols(const FftArgs &F, const double *h, double2 *Y, hipStream_t s) {
+static void launch_cols(const FftArgs &F, const double *h0, const double *h1, double2 *Y, hipStream_t s) {
   hipLaunchKernelGGL(k_fft_cols<LR>, dim3(F.C / col_cols<LR>(), 2), dim3(col_cols<LR>() * fft_threads<LR>()), 0, s, F,
-                     h, Y);
+                     h0, h1, Y);
 }
 
 template <int LC>
@@ -357,16 +357,16 @@ bool fft_supported(int N, int *R, int *C) {
 
 int fft_partials(const FftArgs &F) { return F.R; }
 
-void launch_fft_pass1(const FftArgs &F, const double *h, double2 *Y, hipStream_t s) {
+void launch_fft_pass1(const FftArgs &F, const double *h0, const double *h1, double2 *Y, hipStream_t s) {
   switch (F.R) {
-    case 16: launch_cols<16>(F, h, Y, s); break;
-    case 32: launch_cols<32>(F, h, Y, s); break;
-    case 64: launch_cols<64>(F, h, Y, s); break;
-    case 128: launch_cols<128>(F, h, Y, s); break;
-    case 256: launch_cols<256>(F, h, Y, s); break;
-    case 512: launch_cols<512>(F, h, Y, s); break;
-    case 1024: launch_cols<1024>(F, h, Y, s); break;
-    default: launch_cols<2048>(F, h, Y, s); break;
+    case 16: launch_cols<16>(F, h0, h1, Y, s); break;
+    case 32: launch_cols<32>(F, h0, h1, Y, s); break;
+    case 64: launch_cols<64>(F, h0, h1, Y, s); break;
+    case 128: launch_cols<128>(F, h0, h1, Y, s); break;
+    case 256: launch_cols<256>(F, h0, h1, Y, s); break;
+    case 512: launch_cols<512>(F, h0, h1, Y, s); break;
+    case 1024: launch_cols<1024>(F, h0, h1, Y, s); break;
+    default: launch_cols<2048>(F, h0, h1, Y, s); break;
   }
 }
 

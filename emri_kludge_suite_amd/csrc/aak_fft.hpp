@@ -13,7 +13,6 @@ namespace aak {
 // k1 * C + k2 for bin k = k1 + R k2 (k < M), position M for the Nyquist bin.
 struct FftArgs {
   int N, M, R, C;
-  int hs;  // channel stride of the time-domain input (doubles, even)
   const double2 *twR, *twC, *twMlo, *twMhi, *twNlo, *twNhi, *twNr;
   const double2 *Dp[2];
   const double *wp[2];
@@ -23,11 +22,11 @@ struct FftArgs {
 bool fft_supported(int N, int *R, int *C);
 // Partial-sum pairs written by launch_fft_pass2.
 int fft_partials(const FftArgs &F);
-// Both channels of h (channel stride hs).  Pass 1 writes the 2M-complex scratch Y;
+// Channels h0 and h1 (16-byte aligned).  Pass 1 writes the 2M-complex scratch Y;
 // pass 2 writes fft_partials(F) pairs (sum of <h w, d w>, sum of |h w|^2) to
 // partials and, if hw_out is not null, the weighted spectra h~ w ([2][M+1],
 // natural order).
-void launch_fft_pass1(const FftArgs &F, const double *h, double2 *Y, hipStream_t s);
+void launch_fft_pass1(const FftArgs &F, const double *h0, const double *h1, double2 *Y, hipStream_t s);
 void launch_fft_pass2(const FftArgs &F, const double2 *Y, double *partials, double2 *hw_out, hipStream_t s);
 
 }  // namespace aak

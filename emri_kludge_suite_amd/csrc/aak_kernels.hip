@@ -157,7 +157,11 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   const double cr = cbrt(Om * P.Mphys * kMsunSec);
   B.amp[i] = (cr * cr) * P.zeta;
   if (i == 0) B.nu0[0] = R / T / (2. * kPi * M);
-  if (i > P.i0 && i <= P.guard_last && !(phidot > 0.)) atomicAdd(&B.flags[0], 1);
+  // the reference's test is Phidot <= 0 (AAK.cc:254): a NaN rate is kept as it is
+  if (i > P.i0 && i <= P.guard_last && phidot <= 0.) atomicAdd(&B.flags[0], 1);
+  // a non-positive rate at t = 0 has no earlier value to take: the waveform is
+  // rejected (flags[3]; the likelihood of such a template reads NaN)
+  if (i == P.i0 && phidot <= 0.) B.flags[3] = 1;
 }
 
 void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
@@ -166,16 +170,17 @@ void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
 }
 
 // Non-positive radial rates take the previous (already corrected) value
-// (AAK.cc:254): sample i in (i0, guard_last] with Phidot <= 0 (or NaN) gets the
-// rate of the last sample j in (i0, i] with Phidot > 0, or of i0 if there is
-// none.  That source index is a running maximum, computed as a two-pass
+// (AAK.cc:254): sample i in (i0, guard_last] with Phidot <= 0 gets the rate of
+// the last sample j in (i0, i] whose rate is not <= 0 (a NaN rate counts as
+// such a sample and is kept, as the reference's test leaves it), or of i0 if
+// there is none.  That source index is a running maximum, computed as a two-pass
 // parallel scan; both passes return at once when k_rates counted no such
 // sample, so they are always enqueued and need no host round trip.
 constexpr int kGuardThreads = 256;
 constexpr int kGuardPer = 16;
 constexpr int kGuardBlock = kGuardThreads * kGuardPer;
 
-__device__ inline bool phidot_ok(double r) { return r > 0.; }
+__device__ inline bool phidot_ok(double r) { return !(r <= 0.); }
 
 __global__ __launch_bounds__(kGuardThreads) void k_guard_last(DenseParams P, DenseBuffers B) {
   if (B.flags[0] == 0) return;
@@ -1031,6 +1036,9 @@ __global__ void k_fold_fill(DenseParams P, DenseBuffers B) {
 }
 
 void launch_fold_fill(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
+  // only the dense arrays and the phase output read phases past the last
+  // folded step; the mode sum of a likelihood stops at the end of the taper
+  if (!P.full_dense) return;
   const int n = P.N - (P.last_fold + 2) + 1;
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fold_fill, dim3((n + 255) / 256), dim3(256), 0, s, P, B);
@@ -1726,7 +1734,10 @@ __global__ __launch_bounds__(kDotThreads) void k_weighted_dot(const double2 *hf,
   }
 }
 
-__global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *partials, int nparts, double *out) {
+// out = 4 x (sum of the partial pairs); NaN when the waveform was rejected
+// on the device (flags[3], a non-positive radial rate at t = 0)
+__global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *partials, int nparts, double *out,
+                                                           const int *flags) {
   __shared__ double s_dh[kDotThreads], s_hh[kDotThreads];
   double dh = 0.0, hh = 0.0;
   for (int q = threadIdx.x; q < nparts; q += blockDim.x) {
@@ -1744,21 +1755,22 @@ __global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *partial
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = 4.0 * s_dh[0];
-    out[1] = 4.0 * s_hh[0];
+    const bool bad = flags && flags[3] != 0;
+    out[0] = bad ? __builtin_nan("") : 4.0 * s_dh[0];
+    out[1] = bad ? __builtin_nan("") : 4.0 * s_hh[0];
   }
 }
 
-void launch_dot_final(const double *partials, int nparts, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nparts, out);
+void launch_dot_final(const double *partials, int nparts, double *out, const int *flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nparts, out, flags);
 }
 
 void launch_weighted_dot(const double2 *hf, const double2 *data, const double *w, int nbins, double2 *hw_out,
-                         double *partials, double *out, hipStream_t s) {
+                         double *partials, double *out, const int *flags, hipStream_t s) {
   const int need = (2 * nbins + kDotThreads - 1) / kDotThreads;
   const int nb = need < kDotBlocks ? need : kDotBlocks;
   hipLaunchKernelGGL(k_weighted_dot, dim3(nb), dim3(kDotThreads), 0, s, hf, data, w, nbins, hw_out, partials);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nb, out);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nb, out, flags);
 }
 
 }  // namespace aak

@@ -1,8 +1,8 @@
 // aak_math.hpp -- constants, Kerr closed forms, Carlson elliptic integrals and
 // the post-Newtonian maps used by the AAK waveform.  Every function is
 // __host__ __device__: the host build reproduces the reference CPU AAK bit
-// for bit (same operation order, no contraction), the device build is used
-// by the batched inspiral kernels.
+// for bit (same operation order, no contraction); the device kernels use the
+// post-Newtonian maps (dense phase rates, amplitude), the host the rest.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -160,6 +160,14 @@ class GPUAAK:
         _lib.check(_lib.lib().aak_get_batch_host_ms(self._h, _lib.dptr(a), _lib.dptr(b)), "batch_host_ms")
         return float(a[0]), float(b[0])
 
+    def pool_stats(self):
+        """Host pool of the last LikelihoodBatch: threads, host ms summed over
+        the templates, longest template's host ms, wall ms of the call, busy
+        fraction."""
+        buf = np.zeros(5)
+        _lib.check(_lib.lib().aak_get_pool_stats(self._h, _lib.dptr(buf), 5), "pool_stats")
+        return dict(zip(("threads", "host_ms_sum", "host_ms_max", "wall_ms", "busy"), (float(x) for x in buf)))
+
     def inner_product_stage(self, hI, hII):
         """Runs the inner-product stage alone on two real series (test hook).
 

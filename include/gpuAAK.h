@@ -76,9 +76,16 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
  * evolution, e > 0.66) keep the FP64 per-harmonic evaluation. */
 int aak_set_precision(aak_handle h, int mode);
 
-/* Host threads of the batch pool (0 = AAK_HOST_THREADS or the hardware
- * concurrency, at most 16); no counterpart in the reference. */
+/* Host threads of the batch pool (0 = AAK_HOST_THREADS, else the CPUs the
+ * process may run on: its affinity mask, bounded by a cgroup CPU quota); no
+ * counterpart in the reference. */
 int aak_set_host_threads(aak_handle h, int n);
+
+/* Host pool of the last batch; no counterpart in the reference.
+ * out[0..4] = pool threads, host-stage ms summed over the templates, longest
+ * template's host ms, wall ms of the batch call, busy fraction
+ * (sum / (threads x wall)).  Returns the number written. */
+int aak_get_pool_stats(aak_handle h, double *out, int n);
 
 /* Concurrent streams ("lanes", 1..4) over which aak_likelihood_batch spreads
  * its templates, each with its own device buffers (default AAK_LANES or 2);
