@@ -14,11 +14,32 @@
  * least squares, Miller recurrence for J_n); those endpoints are pinned by
  * scipy/numpy in tests/, not by the reference itself (GSL is absent here).
  *
- * Parity unpinned (needs the GSL-linked reference, which cannot be built
- * here): the exact endpoint of GSL's hybrid solver in the frequency map (the
- * restatement stops on the same residual test), the bit patterns of GSL's
- * spline, least squares and Bessel routines, and hence the end-to-end output
- * of the reference's AAKwrapper.wave.
+ * What pins each part (tests/):
+ *   trajectory, geodesics, Carlson/Legendre integrals, quartic roots --
+ *     bit for bit against the reference's own GSL-free sources compiled here
+ *     (test_oracle_ref.py) and against closed forms (test_oracle_math.py);
+ *   waveform assembly (orientation, polarisation frames, LISA pattern,
+ *     Doppler, taper) -- a basis-free rebuild: detector tensors of the LISA
+ *     cartwheel on the TT projection of the Keplerian quadrupole
+ *     (test_oracle_response.py);
+ *   dv/dt -- at e = 0 for every spin (equatorial), against the test-mass
+ *     Kerr flux through 3PN and the exact circular-orbit energy; at leading
+ *     order in v for every e (test_oracle_pn.py, test_oracle_math.py);
+ *   AK evolution, fits, plunge search, buffer and backward knots -- second-
+ *     order convergence to an ODE solution, exact quadratic fits, a knot-by-
+ *     knot replay of the plunge rule, closed forms (test_oracle_pn.py);
+ *   spline, least squares, J_n, frequency map -- scipy/numpy.
+ *
+ * Parity unpinned:
+ *   - the eccentricity dependence of dv/dt and all of de/dt beyond leading
+ *     order in v (the 3PN O(e^6) series of Sago & Fujita 2015 as transcribed
+ *     in KSParMap.cc:86-144; the reference holds a single copy, and no
+ *     published eccentric expansion is available here to compare with);
+ *   - needs the GSL-linked reference, which cannot be built here: the exact
+ *     endpoint of GSL's hybrid solver in the frequency map (the restatement
+ *     stops on the same residual test), the bit patterns of GSL's spline,
+ *     least squares and Bessel routines, and hence the end-to-end output of
+ *     the reference's AAKwrapper.wave.
  */
 #ifndef AAK_ORACLE_H
 #define AAK_ORACLE_H

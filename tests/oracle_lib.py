@@ -188,3 +188,37 @@ def o_aak_phase(pars, T_fit=1.0, length=2 ** 14, dt=10.0, backint=False):
     if rc != 0:
         raise RuntimeError("oracle AAK phase failed (%d): %s" % (rc, oracle().o_last_error().decode()))
     return dict(zip(PHASE_NAMES, out.reshape(8, length)))
+
+
+def o_knots(pars, T_fit=1.0, length=2 ** 14, dt=10.0, backint=False):
+    """Sparse stage + PNevolution knots of the oracle (AAKpy.cc:16-44,
+    AAK.cc:27-237): dict with the knot arrays, the sparse trajectory, the
+    12-entry par vector and info = (i0, j0, j_min, j_max, j_max_temp,
+    j_start, j_end, plunged, points)."""
+    if isinstance(pars, dict):
+        pars = [pars[k] for k in PARAM_ORDER]
+    iota, s, p, e, M, mu, gamma, psi, alph, qS, phS, qK, phK, D = [float(x) for x in pars]
+    sp = [np.zeros(100) for _ in range(5)]
+    Phi0 = C.c_double(0)
+    flag = C.c_int(0)
+    steps = oracle().o_sparse_stage(T_fit, length, dt, iota, s, p, e, M, mu, psi, *[_ptr(a) for a in sp],
+                                    C.byref(Phi0), C.byref(flag))
+    if steps < 3:
+        raise RuntimeError("oracle sparse stage failed (%d)" % steps)
+    e_traj, v_map, M_map, s_map, dt_map = sp
+    msun = 6.67259e-11 * 1.98892e30 / (299792458.0 * 299792458.0 * 299792458.0)
+    par = np.array([mu * msun, M_map[0] * msun, s_map[0], e_traj[0], iota, gamma, Phi0.value,
+                    qS, phS, qK, phK, alph])
+    j0 = oracle().o_pn_j0(dt, length, _ptr(dt_map), steps)
+    cap = 2 * j0 + 8
+    kn = np.zeros(5 * cap)
+    info = np.zeros(9, dtype=np.int32)
+    K = oracle().o_pn_knots(dt, length, _ptr(par), _ptr(e_traj), _ptr(v_map), M, _ptr(M_map), s, _ptr(s_map),
+                            _ptr(dt_map), steps, int(backint), *[_ptr(kn[k * cap:]) for k in range(5)], cap,
+                            info.ctypes.data_as(_pi))
+    if K < 0:
+        raise RuntimeError("oracle knots failed (%d): %s" % (K, oracle().o_last_error().decode()))
+    kn = kn.reshape(5, cap)[:, :K]
+    return {"t": kn[0], "e": kn[1], "v": kn[2], "M": kn[3], "S": kn[4], "K": K, "steps": steps,
+            "e_traj": e_traj[:steps], "v_map": v_map[:steps], "M_map": M_map[:steps], "s_map": s_map[:steps],
+            "dt_map": dt_map[:steps], "par": par, "info": info, "M_phys": M, "S_phys": s}
