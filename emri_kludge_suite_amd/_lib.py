@@ -60,6 +60,10 @@ def lib():
         "aak_set_profiling": (_i, [C.c_void_p, _i]),
         "aak_get_timings": (_i, [C.c_void_p, _pd, _i]),
         "aak_get_pool_stats": (_i, [C.c_void_p, _pd, _i]),
+        "aak_PNevolution": (_i, [_pd] * 5 + [_i, _d, _i, _pd, _pd, _pd, _d, _pd, _d, _pd, _pd, _i, _pi, _pi, _i,
+                                             _pd, _pd, _pi]),
+        "aak_run_phase_trajectory": (_i, [C.c_void_p] + [_d] * 14),
+        "aak_get_trajectory": (_i, [C.c_void_p] + [_pd] * 5 + [_i, _pi, _pi, _pd, _pd, _pi]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

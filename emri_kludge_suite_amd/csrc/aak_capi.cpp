@@ -233,6 +233,7 @@ struct aak_handle_s {
   // single-waveform staging
   WaveformPlan single;
   PinnedBuf single_pin;
+  int have_traj = 0;  // single holds a whole host stage (run_phase_trajectory, gpu_gen_AAK)
   // stage timing on the compute stream: rates, guard+folds, mode sum, rFFT, reduction
   int profiling = 0;
   int force_fold_replay = 0;  // test hook: every fold takes the serial replay
@@ -683,6 +684,66 @@ const char *aak_last_error(void) { return g_err.c_str(); }
 
 double aak_OmegaPhi(double v, double e, double cosiota, double s, double M) { return omega_phi(v, e, cosiota, s, M); }
 
+// Knot outputs of one host stage: the reference's PNevolution results
+// (gpuAAK.cc:234-266) from a knot set and its table.
+static int knot_outputs(const WaveConfig &cfg, const KnotSet &ks, const SparseTrajectory &tr, const double *par,
+                        double *t_in, double *e_in, double *v_in, double *M_in, double *S_in, int cap, int *i_plunge,
+                        int *i_buffer, double *t_clip, double *interp_timestep, int *init_length) {
+  const int K = ks.K;
+  if (K > cap) return fail(-6, "knot buffer too small (" + std::to_string(K) + " knots)");
+  std::vector<double> tab(table_size(ks));
+  KnotGrid g;
+  fill_table(ks, tr, tab.data(), &g);
+  int ip = 0, ib = 0;
+  double tp = 0.0;
+  std::string err;
+  const int rc = plunge_buffer(cfg, ks, tab.data(), g, cos(par[4]), &ip, &ib, &tp, &err);
+  if (rc < 0) return fail(rc, err);
+  double *dst[5] = {t_in, e_in, v_in, M_in, S_in};
+  std::memcpy(dst[0], ks.kt(), sizeof(double) * K);
+  for (int a = 0; a < 4; a++) std::memcpy(dst[a + 1], ks.ky(a), sizeof(double) * K);
+  *i_plunge = ip - ks.i0;
+  *i_buffer = ib;
+  *t_clip = (ip + ib - ks.i0) * cfg.dt;
+  if (*t_clip > ks.kt()[K - 1]) *t_clip = ks.kt()[K - 1];
+  *interp_timestep = ks.dt_large;
+  *init_length = K;
+  return 0;
+}
+
+int aak_PNevolution(double *t_in, double *e_in, double *v_in, double *M_in, double *S_in, int cap, double timestep,
+                    int vlength, const double *par, const double *e_traj, const double *v_map, double M_phys,
+                    const double *M_map, double S_phys, const double *S_map, const double *dt_map, int steps,
+                    int *i_plunge, int *i_buffer, int backint, double *t_clip, double *interp_timestep,
+                    int *init_length) {
+  if (!t_in || !e_in || !v_in || !M_in || !S_in || !par || !e_traj || !v_map || !M_map || !S_map || !dt_map ||
+      !i_plunge || !i_buffer || !t_clip || !interp_timestep || !init_length)
+    return fail(-2, "null argument");
+  if (steps < 3 || steps > 100) return fail(-2, "steps must lie in [3, 100]");
+  if (vlength < 2 || !(timestep > 0) || cap < 1) return fail(-2, "bad grid or capacity");
+  WaveConfig cfg;
+  cfg.length = vlength;
+  cfg.dt = timestep;
+  cfg.backint = backint ? 1 : 0;
+  SparseTrajectory tr;
+  tr.steps = steps;
+  std::memcpy(tr.e_traj, e_traj, sizeof(double) * steps);
+  std::memcpy(tr.v_map, v_map, sizeof(double) * steps);
+  std::memcpy(tr.M_map, M_map, sizeof(double) * steps);
+  std::memcpy(tr.s_map, S_map, sizeof(double) * steps);
+  std::memcpy(tr.dt_map, dt_map, sizeof(double) * steps);
+  try {
+    KnotSet ks;
+    std::string err;
+    const int rc = build_knots(cfg, par, M_phys, S_phys, tr, &ks, &err);
+    if (rc < 0) return fail(rc, err);
+    return knot_outputs(cfg, ks, tr, par, t_in, e_in, v_in, M_in, S_in, cap, i_plunge, i_buffer, t_clip,
+                        interp_timestep, init_length);
+  } catch (const std::bad_alloc &) {
+    return fail(-9, "host memory exhausted by the knot arrays");
+  }
+}
+
 int aak_create(double T_fit, int init_length, int length, double init_dt, double dt, int LISA, int backint,
                aak_handle *out) {
   (void)init_length;
@@ -813,12 +874,14 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   // the staging buffer is free: every call ends with the compute stream idle,
   // and the compute stream waited for the upload
   PinnedSink sink(&h->single_pin);
+  h->have_traj = 0;
   int rc = plan_safe(h->cfg, sp, &plan, &err, &sink);
   h->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (rc < 0) {
     h->state = 0;
     return fail(rc, err);
   }
+  h->have_traj = 1;
   Lane &L = h->lane[0];
   rc = enqueue_waveform(h, L, plan.P, plan.tab, plan.tab_n, 0, nullptr);
   if (rc) return rc;
@@ -846,6 +909,36 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   return 0;
 }
 
+int aak_run_phase_trajectory(aak_handle h, double iota, double s, double p, double e, double M, double mu,
+                             double gamma, double psi, double alph, double theta_S, double phi_S, double theta_K,
+                             double phi_K, double D) {
+  if (!h) return fail(-1, "null handle");
+  SourceParams sp{iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D};
+  std::string err;
+  PinnedSink sink(&h->single_pin);
+  h->have_traj = 0;
+  const int rc = plan_safe(h->cfg, sp, &h->single, &err, &sink);
+  if (rc < 0) return fail(rc, err);
+  h->have_traj = 1;
+  return 0;
+}
+
+int aak_get_trajectory(aak_handle h, double *t_in, double *e_in, double *v_in, double *M_in, double *S_in, int cap,
+                       int *i_plunge, int *i_buffer, double *t_clip, double *interp_timestep, int *init_length) {
+  if (!h) return fail(-1, "null handle");
+  if (!h->have_traj) return fail(-2, "no trajectory: call run_phase_trajectory or gpu_gen_AAK first");
+  if (!t_in || !e_in || !v_in || !M_in || !S_in || !i_plunge || !i_buffer || !t_clip || !interp_timestep ||
+      !init_length)
+    return fail(-2, "null argument");
+  const WaveformPlan &plan = h->single;
+  try {
+    return knot_outputs(h->cfg, plan.knots, plan.traj, plan.par, t_in, e_in, v_in, M_in, S_in, cap, i_plunge,
+                        i_buffer, t_clip, interp_timestep, init_length);
+  } catch (const std::bad_alloc &) {
+    return fail(-9, "host memory exhausted by the knot table");
+  }
+}
+
 int aak_gen_phases(aak_handle h, const double *params, double *t, double *phase_r, double *phase_theta,
                    double *phase_phi, double *omega_r, double *omega_theta, double *omega_phi, double *ecc) {
   if (!h) return fail(-1, "null handle");
@@ -855,11 +948,13 @@ int aak_gen_phases(aak_handle h, const double *params, double *t, double *phase_
   WaveformPlan &plan = h->single;
   std::string err;
   PinnedSink sink(&h->single_pin);
+  h->have_traj = 0;
   int rc = plan_safe(h->cfg, sp, &plan, &err, &sink);
   if (rc < 0) {
     h->state = 0;
     return fail(rc, err);
   }
+  h->have_traj = 1;
   if (!h->d_phase) HIPCHK(hipMalloc(&h->d_phase, sizeof(double) * 8 * (size_t)h->N));
   // the phase output applies the Phidot guard to every forward sample (AAKPhase.cc:233-241)
   DenseParams P = plan.P;

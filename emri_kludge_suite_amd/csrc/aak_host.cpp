@@ -512,6 +512,25 @@ double table_eval(const double *tab, const KnotGrid &g, int array, double t) {
   return y[k] + dl * (b + dl * (c[k] + dl * d));
 }
 
+int plunge_buffer(const WaveConfig &cfg, const KnotSet &ks, const double *tab, const KnotGrid &grid, double coslam,
+                  int *ip_out, int *ib_out, double *tp_out, std::string *err) {
+  const int N = cfg.length;
+  const int ip = ks.plunged ? ks.i0 + (int)(ks.t_fit_jmin / cfg.dt) : N - 1;
+  if (ip < 0 || ip > N) {
+    *err = "plunge index outside the waveform";
+    return -6;
+  }
+  // the dense grid: sample 0 at the first knot, sample N at the last, i - i0
+  // steps of dt in between (AAK.cc:228-231)
+  const double tp = ip == 0 ? ks.kt()[0] : (ip == N ? ks.kt()[ks.K - 1] : (double)(ip - ks.i0) * cfg.dt);
+  const double ve = table_eval(tab, grid, 1, tp), ee = table_eval(tab, grid, 0, tp);
+  const double Me = table_eval(tab, grid, 2, tp), Se = table_eval(tab, grid, 3, tp);
+  *ib_out = (int)(10. / (PN::omega_phi(ve, ee, coslam, Se, Me) / 2. / kPi) / cfg.dt) + 1;
+  *ip_out = ip;
+  *tp_out = tp;
+  return 0;
+}
+
 int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *plan, std::string *err,
                   TableSink *sink) {
   int rc = sparse_stage(cfg, sp, &plan->traj, err);
@@ -566,16 +585,10 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   P.x0[1] = par[6];
   P.x0[2] = par[11];
   // plunge index and taper length (AAK.cc:240-242)
-  int ip = ks.plunged ? ks.i0 + (int)(ks.t_fit_jmin / cfg.dt) : N - 1;
-  if (ip < 0 || ip > N) {
-    *err = "plunge index outside the waveform";
-    return -6;
-  }
-  auto tsample = [&](int i) { return i == 0 ? P.t_first : (i == N ? P.t_last : (double)(i - P.i0) * P.dt); };
-  const double tp = tsample(ip);
-  const double ve = table_eval(plan->tab, P.grid, 1, tp), ee = table_eval(plan->tab, P.grid, 0, tp);
-  const double Me = table_eval(plan->tab, P.grid, 2, tp), Se = table_eval(plan->tab, P.grid, 3, tp);
-  const int ib = (int)(10. / (PN::omega_phi(ve, ee, coslam, Se, Me) / 2. / kPi) / cfg.dt) + 1;
+  int ip = 0, ib = 0;
+  double tp = 0.0;
+  rc = plunge_buffer(cfg, ks, plan->tab, P.grid, coslam, &ip, &ib, &tp, err);
+  if (rc < 0) return rc;
   P.ip = ip;
   P.ib = ib;
   P.last_fold = std::min(N - 1, ip + ib);

@@ -89,6 +89,10 @@ int spline_natural(const double *x, const double *y, int n, double *c);
 size_t table_size(const KnotSet &ks);
 void fill_table(const KnotSet &ks, const SparseTrajectory &tr, double *tab, KnotGrid *grid);
 double table_eval(const double *tab, const KnotGrid &g, int array, double t);
+// Plunge sample ip and taper length ib (ten azimuthal periods at ip, + 1)
+// from the knots and their device table (AAK.cc:240-242); *tp = ip's time.
+int plunge_buffer(const WaveConfig &cfg, const KnotSet &ks, const double *tab, const KnotGrid &grid, double coslam,
+                  int *ip, int *ib, double *tp, std::string *err);
 int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *plan, std::string *err,
                   TableSink *sink = nullptr);
 double omega_phi(double v, double e, double cosiota, double s, double M);

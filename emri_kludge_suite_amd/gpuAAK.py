@@ -15,6 +15,40 @@ def OmegaPhi(v, e, cosiota, s, M):
     return _lib.lib().aak_OmegaPhi(float(v), float(e), float(cosiota), float(s), float(M))
 
 
+def _knot_outputs(call, cap):
+    arrs = [np.zeros(cap) for _ in range(5)]
+    ip, ib, K = C.c_int(0), C.c_int(0), C.c_int(0)
+    tclip, dtl = C.c_double(0), C.c_double(0)
+    rc = call([_lib.dptr(a) for a in arrs], C.byref(ip), C.byref(ib), C.byref(tclip), C.byref(dtl), C.byref(K))
+    return rc, arrs, ip.value, ib.value, tclip.value, dtl.value, K.value
+
+
+def PNevolution(timestep, vlength, par, e_traj, v_map, M_phys, M_map, S_phys, S_map, dt_map, steps, backint,
+                cap=None):
+    """AK evolution, fits and plunge search of the host stage (gpuAAK.h:15-18).
+
+    Returns (t_in, e_in, v_in, M_in, S_in, i_plunge, i_buffer, t_clip,
+    interp_timestep, init_length); the knot arrays hold init_length values.
+    ``cap`` bounds the knot count (default: twice the knots the grid needs,
+    as the reference sizes its evolution arrays)."""
+    L = _lib.lib()
+    ins = [np.ascontiguousarray(x, dtype=np.float64) for x in (par, e_traj, v_map, M_map, S_map, dt_map)]
+    steps = int(steps)
+    if cap is None:
+        dt_large = ins[5][1]
+        j0 = int((int(vlength) * float(timestep) - ins[5][steps - 1]) / dt_large + steps + 1)
+        cap = 2 * max(j0, 1) + 8
+    par_, e_, v_, M_, S_, d_ = [_lib.dptr(x) for x in ins]
+
+    def call(out, ip, ib, tclip, dtl, K):
+        return L.aak_PNevolution(*out, int(cap), float(timestep), int(vlength), par_, e_, v_, float(M_phys), M_,
+                                 float(S_phys), S_, d_, steps, ip, ib, int(bool(backint)), tclip, dtl, K)
+
+    rc, arrs, ip, ib, tclip, dtl, K = _knot_outputs(call, int(cap))
+    _lib.check(rc, "PNevolution")
+    return tuple(a[:K] for a in arrs) + (ip, ib, tclip, dtl, K)
+
+
 class GPUAAK:
     """GPUAAK(T_fit, init_length, length, init_dt, dt, LISA, backint)
 
@@ -55,6 +89,25 @@ class GPUAAK:
         _lib.check(_lib.lib().aak_input_data(self._h, a.view(np.float64).ctypes.data_as(_lib._pd),
                                              b.view(np.float64).ctypes.data_as(_lib._pd),
                                              _lib.dptr(w1), _lib.dptr(w2), n), "input_data")
+
+    def run_phase_trajectory(self, iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D):
+        """Host stage alone: sparse inspiral, frequency map and PNevolution
+        (AAK_manager.hh:118-132); the knots stay in the handle."""
+        _lib.check(_lib.lib().aak_run_phase_trajectory(self._h, *[float(x) for x in (
+            iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D)]), "run_phase_trajectory")
+
+    def GetTrajectory(self, cap=1 << 22):
+        """Knots and PNevolution outputs kept by the last run_phase_trajectory
+        or gpu_gen_AAK: (t_in, e_in, v_in, M_in, S_in, i_plunge, i_buffer,
+        t_clip, interp_timestep, init_length)."""
+        L = _lib.lib()
+
+        def call(out, ip, ib, tclip, dtl, K):
+            return L.aak_get_trajectory(self._h, *out, int(cap), ip, ib, tclip, dtl, K)
+
+        rc, arrs, ip, ib, tclip, dtl, K = _knot_outputs(call, int(cap))
+        _lib.check(rc, "GetTrajectory")
+        return tuple(a[:K] for a in arrs) + (ip, ib, tclip, dtl, K)
 
     def gpu_gen_AAK(self, iota, s, p, e, M, mu, gamma, psi, alph, theta_S, phi_S, theta_K, phi_K, D):
         """Generate the waveform on the device (GPUAAK.pyx:69-74)."""

@@ -105,6 +105,38 @@ int aak_gen_phases(aak_handle h, const double *params, double *t, double *phase_
 /* OmegaPhi(v, e, cosiota, s, M) (gpuAAK.h:13, gpuAAK.cc:17-26). */
 double aak_OmegaPhi(double v, double e, double cosiota, double s, double M);
 
+/* PNevolution(t_in, e_in, v_in, M_in, S_in, timestep, vlength, par, e_traj,
+ * v_map, M_phys, M_map, S_phys, S_map, dt_map, steps, i_plunge, i_buffer,
+ * backint, t_clip, interp_timestep, init_length) (gpuAAK.h:15-18,
+ * gpuAAK.cc:29-267): the AK evolution, fit corrections and plunge search of
+ * the host stage.  Inputs as in the reference: par[12] (mu and M_map[0] in
+ * seconds, s_map[0], e_traj[0], iota, gamma, Phi0, theta_S, phi_S, theta_K,
+ * phi_K, alpha), the sparse trajectory of `steps` (3..100) points, M_phys (in
+ * solar masses) and S_phys.  Outputs: the knots t_in, e_in, v_in, M_in, S_in
+ * (caller-owned, `cap` entries each; -6 when the knots do not fit),
+ * i_plunge (relative to the first sample of the waveform), i_buffer, t_clip,
+ * interp_timestep (the uniform knot spacing) and init_length (the knot count).
+ * The knots span the reference CPU's whole range (AAK.cc:220-226), not the
+ * GPU reference's cut at j_max. */
+int aak_PNevolution(double *t_in, double *e_in, double *v_in, double *M_in, double *S_in, int cap,
+                    double timestep, int vlength, const double *par, const double *e_traj,
+                    const double *v_map, double M_phys, const double *M_map, double S_phys,
+                    const double *S_map, const double *dt_map, int steps, int *i_plunge, int *i_buffer,
+                    int backint, double *t_clip, double *interp_timestep, int *init_length);
+
+/* GPUAAK::run_phase_trajectory(14 parameters) (AAK_manager.hh:118-132,
+ * AAK_manager.cu:263-342): the host stage alone (sparse inspiral, frequency
+ * map, PNevolution), kept in the handle; aak_gpu_gen_AAK runs it itself.
+ * aak_get_trajectory copies the kept knots and PNevolution outputs (same
+ * meaning as in aak_PNevolution); no counterpart accessor exists in the
+ * reference, whose knots stay in private members. */
+int aak_run_phase_trajectory(aak_handle h, double iota, double s, double p, double e, double M,
+                             double mu, double gamma, double psi, double alph, double theta_S,
+                             double phi_S, double theta_K, double phi_K, double D);
+int aak_get_trajectory(aak_handle h, double *t_in, double *e_in, double *v_in, double *M_in, double *S_in,
+                       int cap, int *i_plunge, int *i_buffer, double *t_clip, double *interp_timestep,
+                       int *init_length);
+
 /* Harmonic-count override; no counterpart in the reference, whose count is
  * (int)(30 e0), 4 below e0 = 0.135 (AAK.cc:582-583).  n_max <= 0 restores it. */
 int aak_set_nmax(aak_handle h, int n_max);

@@ -204,3 +204,25 @@ def test_mode_sum_forms_agree_across_a_plunge():
     pars = [0.5, 0.3, 8.0, 0.45, 3e5, 5.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
     out, g, ref = _compare(pars, 2 ** 20, 15.0, n_max=40, label="fast-plunge")
     assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
+
+
+def test_run_phase_trajectory_keeps_the_oracle_knots():
+    # GPUAAK.run_phase_trajectory (AAK_manager.hh:118-132) keeps the host
+    # stage in the handle; its knots and PNevolution outputs equal the free
+    # function's on the oracle's sparse trajectory, and gpu_gen_AAK leaves the
+    # same trajectory behind
+    from emri_kludge_suite_amd import GPUAAK, PNevolution
+    pars, N, dt = [0.5, 0.5, 8.0, 0.1, 1e6, 10.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0], 2 ** 18, 120.0
+    print("case run_phase_trajectory N=%d dt=%g" % (N, dt), flush=True)
+    r = ol.o_knots(pars, 1.0, N, dt)
+    want = PNevolution(dt, N, r["par"], r["e_traj"], r["v_map"], r["M_phys"], r["M_map"], r["S_phys"],
+                       r["s_map"], r["dt_map"], r["steps"], False)
+    g = GPUAAK(1.0, N // 10, N, 100.0, dt, True, False)
+    with pytest.raises(RuntimeError, match="no trajectory"):
+        g.GetTrajectory()
+    for run in (g.run_phase_trajectory, g.gpu_gen_AAK):
+        run(*pars)
+        got = g.GetTrajectory()
+        for a, b in zip(got[:5], want[:5]):
+            np.testing.assert_array_equal(a, b)
+        assert got[5:] == want[5:]

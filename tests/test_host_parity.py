@@ -269,3 +269,42 @@ def test_knot_buffer_too_small_is_reported(L):
     pars = _draw(np.random.default_rng(1))
     K, *_ = _product_plan(L, pars, 2 ** 18, 10.0, cap=10)
     assert K < 0 and "small" in L.last_error()
+
+
+# ------------------------------------------------------------ PNevolution
+@pytest.mark.parametrize("case", ["config1", "plunging", "plunging_backint", "production"])
+def test_pnevolution_free_function_matches_the_oracle(case):
+    # aak_PNevolution (gpuAAK.h:15-18) on the oracle's own sparse trajectory:
+    # knots bit for bit, plunge sample and taper length as the oracle's
+    # waveform uses them, t_clip and the knot spacing by their definitions
+    from emri_kludge_suite_amd import PNevolution
+    rng = np.random.default_rng(51)
+    backint = case == "plunging_backint"
+    if case == "config1":
+        pars, N, dt = _draw(rng), 2 ** 18, 10.0
+    elif case == "production":
+        pars, N, dt = [0.5, 0.4, 9.0, 0.3, 3e5, 30.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0], 2 ** 20, 15.0
+    else:
+        pars, N, dt = [0.5, 0.5, 8.0, 0.1, 1e6, 10.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0], 2 ** 18, 120.0
+    r = ol.o_knots(pars, 1.0, N, dt, backint=backint)
+    w = ol.o_aak_wave(pars, 1.0, N, dt, backint=backint)
+    out = PNevolution(dt, N, r["par"], r["e_traj"], r["v_map"], r["M_phys"], r["M_map"], r["S_phys"],
+                      r["s_map"], r["dt_map"], r["steps"], backint)
+    t_in, e_in, v_in, M_in, S_in, ip, ib, t_clip, dtl, K = out
+    assert K == r["K"]
+    for got, want in zip((t_in, e_in, v_in, M_in, S_in), (r["t"], r["e"], r["v"], r["M"], r["S"])):
+        np.testing.assert_array_equal(got, want)
+    i0 = r["info"][0]
+    assert ip == w["i_plunge"] - i0 and ib == w["i_buffer"]
+    assert dtl == r["dt_map"][1]
+    assert t_clip == min((ip + ib) * dt, t_in[-1])
+    if case != "config1":
+        assert r["info"][7] == 1  # these sources plunge inside the window
+
+
+def test_pnevolution_reports_a_small_buffer():
+    from emri_kludge_suite_amd import PNevolution
+    r = ol.o_knots([0.5, 0.5, 8.0, 0.2, 1e6, 10.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0], 1.0, 2 ** 18, 10.0)
+    with pytest.raises(RuntimeError, match="knot buffer too small"):
+        PNevolution(10.0, 2 ** 18, r["par"], r["e_traj"], r["v_map"], r["M_phys"], r["M_map"], r["S_phys"],
+                    r["s_map"], r["dt_map"], r["steps"], False, cap=10)
