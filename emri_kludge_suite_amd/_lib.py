@@ -12,6 +12,7 @@ _d = C.c_double
 _i = C.c_int
 _pd = C.POINTER(C.c_double)
 _pi = C.POINTER(C.c_int)
+_pf = C.POINTER(C.c_float)
 
 _lib = None
 
@@ -47,8 +48,10 @@ def lib():
         "aak_test_bessel_five": (_i, [_i, _d, _d, _pd]),
         "aak_test_taylor_amplitudes": (_i, [_i, _i, _d, _d, _d, _pd]),
         "aak_likelihood_batch": (_i, [C.c_void_p, _pd, _i, _pd, _pi]),
+        "aak_likelihood_batch_nmax": (_i, [C.c_void_p, _pd, _pi, _i, _pd, _pi]),
         "aak_set_host_threads": (_i, [C.c_void_p, _i]),
         "aak_set_lanes": (_i, [C.c_void_p, _i]),
+        "aak_set_tuning": (_i, [C.c_void_p, _i, _i]),
         "aak_set_precision": (_i, [C.c_void_p, _i]),
         "aak_gen_phases": (_i, [C.c_void_p, _pd] + [_pd] * 8),
         "aak_get_batch_timings": (_i, [C.c_void_p, _pd, _i]),
@@ -64,6 +67,19 @@ def lib():
                                              _pd, _pd, _pi]),
         "aak_run_phase_trajectory": (_i, [C.c_void_p] + [_d] * 14),
         "aak_get_trajectory": (_i, [C.c_void_p] + [_pd] * 5 + [_i, _pi, _pi, _pd, _pd, _pi]),
+        "aak_mlp_create": (_i, [_i, C.POINTER(C.c_void_p)]),
+        "aak_mlp_destroy": (_i, [C.c_void_p]),
+        "aak_mlp_set_params": (_i, [C.c_void_p, _pf, _pf]),
+        "aak_mlp_set_norm": (_i, [C.c_void_p, _pf]),
+        "aak_mlp_get_params": (_i, [C.c_void_p, _pf, _pf]),
+        "aak_mlp_forward": (_i, [C.c_void_p, _pf, _i, _pf]),
+        "aak_mlp_gradient": (_i, [C.c_void_p, _pf, _pf, _i, _pf, _pd]),
+        "aak_mlp_train": (_i, [C.c_void_p, _pf, _pf, _i, _i, _i, C.c_float, C.c_float, _i, _pd]),
+        "aak_mlp_predict_maps": (_i, [C.c_void_p, _pd, _i, _pd]),
+        "aak_freqmap_batch": (_i, [_pd, _i, _i, C.c_void_p, _d, _i, _pd, _pi, _pd, _pd]),
+        "aak_p_lso": (_d, [_d, _d, _d]),
+        "aak_p_lso_batch": (_i, [_pd, _i, _pd, _i]),
+        "aak_set_map_init": (_i, [C.c_void_p, _i, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -89,3 +105,7 @@ def dptr(a):
 
 def iptr(a):
     return a.ctypes.data_as(_pi)
+
+
+def fptr(a):
+    return a.ctypes.data_as(_pf)

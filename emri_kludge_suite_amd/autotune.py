@@ -1,0 +1,102 @@
+"""Autotuner of the batched likelihood.
+
+For one grid (N, dt) and precision it times short batches of real templates
+over the pipeline's knobs and keeps the fastest setting:
+  lanes       concurrent streams, each with its own device buffers (1..8);
+  feed_depth  short host plans kept ready for the GPU (the pool's ordering);
+  exp_shift   mode-sum expansion blocks of 256 << shift samples (the Taylor
+              order adapts, so accuracy does not depend on it).
+The search is one coordinate-descent sweep from the defaults (two timed
+batches per candidate after one untimed one).  Results are kept in a JSON
+table keyed by grid and precision; `apply` looks a grid up (the nearest
+length of the same dt and precision when it is absent).
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning.json")
+DEFAULTS = {"lanes": 2, "feed_depth": 10, "exp_shift": None}
+CANDIDATES = {"lanes": (1, 2, 3, 4, 6, 8), "feed_depth": (4, 10, 24), "exp_shift": (0, 1, 2, 3)}
+
+
+def key(N, dt, precision):
+    return "%d|%g|%s" % (int(N), float(dt), precision)
+
+
+def _set(g, s):
+    g.set_lanes(s["lanes"])
+    g.set_tuning(feed_depth=s["feed_depth"], exp_shift=-2 if s["exp_shift"] is None else s["exp_shift"])
+
+
+def _time(g, P, nmax, repeats):
+    g.LikelihoodBatch(P, nmax=nmax)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        g.LikelihoodBatch(P, nmax=nmax)
+    return (time.perf_counter() - t0) / (repeats * len(P))
+
+
+def tune(g, templates, nmax=None, repeats=2, log=None):
+    """Best setting for handle g (already holding data) on `templates`;
+    returns (setting, {candidate: seconds per likelihood})."""
+    P = np.ascontiguousarray(templates, dtype=np.float64)
+    best = dict(DEFAULTS)
+    _set(g, best)
+    best_t = _time(g, P, nmax, repeats)
+    seen = {json.dumps(best, sort_keys=True): best_t}
+    for knob, values in CANDIDATES.items():
+        for v in values:
+            s = dict(best)
+            s[knob] = v
+            k = json.dumps(s, sort_keys=True)
+            if k not in seen:
+                _set(g, s)
+                seen[k] = _time(g, P, nmax, repeats)
+                if log:
+                    log("  %s -> %.3f ms per likelihood" % (k, 1e3 * seen[k]))
+            if seen[k] < best_t:
+                best, best_t = s, seen[k]
+    _set(g, best)
+    return best, seen
+
+
+def load(path=TABLE):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save(table, path=TABLE):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+
+
+def lookup(table, N, dt, precision):
+    """The entry of (N, dt, precision), else the one of the nearest length with
+    the same dt and precision, else None."""
+    k = key(N, dt, precision)
+    if k in table:
+        return table[k]
+    best, dist = None, None
+    for kk, v in table.items():
+        n2, dt2, p2 = kk.split("|")
+        if float(dt2) == float(dt) and p2 == precision:
+            d = abs(np.log2(int(n2)) - np.log2(N))
+            if dist is None or d < dist:
+                best, dist = v, d
+    return best
+
+
+def apply(g, N, dt, precision, table=None):
+    """Sets handle g to the tabulated setting of its grid; returns it (None
+    when the table has nothing for this precision and dt)."""
+    s = lookup(load() if table is None else table, N, dt, precision)
+    if s:
+        _set(g, {k: s.get(k, DEFAULTS[k]) for k in DEFAULTS})
+    return s

@@ -47,8 +47,11 @@
 #include "aak_bessel.hpp"
 #include "aak_dense.hpp"
 #include "aak_fft.hpp"
+#include "aak_freqmap.hpp"
 #include "aak_geodesic.hpp"
 #include "aak_host.hpp"
+#include "aak_inspiral.hpp"
+#include "aak_mlp.hpp"
 
 using namespace aak;
 
@@ -200,16 +203,18 @@ struct Lane {
   rocfft_plan plan = nullptr;
   rocfft_execution_info exec = nullptr;
   void *d_work = nullptr;
+  hipGraph_t g_like = nullptr;          // the batch's likelihood stage (fused FFT), captured on first use
+  hipGraphExec_t gx_like = nullptr;
   bool ready = false;
 };
 
-constexpr int kMaxLanes = 4;
+constexpr int kMaxLanes = 8;
 // short plans kept ready or in progress for the GPU by the batch pool (about
 // 3 ms of device work at N = 2^21); other picks take the longest remaining.
 // Measured over 256-template batches of the production prior on 16 host
 // threads: 6 -> 3.26e3, 8 -> 3.42e3, 10 -> 3.41e3 (FP64 +3%), 12 -> 3.09e3,
 // 16 -> 2.63e3 likelihoods/s (too few threads left for the longest plans).
-constexpr int kFeedDepth = 10;
+constexpr int kFeedDepth = 10;  // default of aak_handle_s::feed_depth
 
 struct aak_handle_s {
   WaveConfig cfg;
@@ -217,6 +222,8 @@ struct aak_handle_s {
   hipStream_t cstream = nullptr;  // knot-table uploads of every lane
   Lane lane[kMaxLanes];
   int nlanes = 2;                 // lanes used by batches (lane 0 serves single calls)
+  int feed_depth = kFeedDepth;    // short plans kept ready or in progress for the GPU
+  bool use_graphs = true;         // batches replay each lane's captured likelihood stage (AAK_GRAPHS=0: off)
   double *d_bessel = nullptr;     // c_m(k) table | log k!
   double2 *d_data = nullptr;      // [2][nbins] weighted data (fused FFT path: in its row order)
   double *d_w = nullptr;          // [2][nbins] weights (same order as d_data)
@@ -256,6 +263,7 @@ struct aak_handle_s {
   std::condition_variable cv_work, cv_ready;
   bool stop = false;
   const double *b_params = nullptr;
+  const int *b_nmax = nullptr;                // per-template harmonic counts (optional)
   long long b_n = 0, b_next = 0, b_end = 0;  // b_order[b_next, b_end) not yet taken
   std::vector<long long> b_order;  // templates in decreasing estimated host cost
   int cheap_inflight = 0;          // plans in progress taken from the cheap end (under mu)
@@ -307,6 +315,7 @@ static int alloc_lane(aak_handle h, Lane &L) {
   HIPCHK(hipMalloc(&L.d_h, sizeof(double) * 2 * h_stride(N)));
   HIPCHK(hipMalloc(&L.d_t, sizeof(double) * ((size_t)N + 2)));
   HIPCHK(hipMalloc(&L.d_flags, sizeof(int) * 4));
+  HIPCHK(hipMemset(L.d_flags, 0, sizeof(int) * 4));
   HIPCHK(hipMalloc(&L.d_guard, sizeof(int) * ((size_t)N / 4096 + 4)));
   HIPCHK(hipMalloc(&L.d_ms_p, sizeof(int) * ((size_t)N / 256 + 4)));
   HIPCHK(hipMalloc(&L.d_ms_list, sizeof(int) * ((size_t)N / 256 + 4)));
@@ -379,6 +388,8 @@ static int alloc_lane(aak_handle h, Lane &L) {
 
 static void free_lane(Lane &L) {
   if (L.stream) (void)hipStreamSynchronize(L.stream);
+  if (L.gx_like) (void)hipGraphExecDestroy(L.gx_like);
+  if (L.g_like) (void)hipGraphDestroy(L.g_like);
   if (L.plan) rocfft_plan_destroy(L.plan);
   if (L.exec) rocfft_execution_info_destroy(L.exec);
   void *ptrs[] = {L.d_table[0], L.d_table[1], L.d_dense, L.d_h, L.d_t, L.d_flags, L.d_guard, L.d_foldmem,
@@ -409,37 +420,11 @@ static int alloc_handle(aak_handle h) {
   HIPCHK(hipMalloc(&h->d_w, sizeof(double) * 2 * (size_t)nb));
   if (h->fused_fft) {
     FftArgs &F = h->fft;
-    // W_L^e = exp(-2 pi i e / L) for L = R, C, and W_M, W_N split as
-    // W^e = W^(1024 (e >> 10)) W^(e & 1023), in extended precision
-    const int nhi = F.M / 1024 + 2;
-    const size_t o_R = 0, o_C = o_R + F.R, o_Mlo = o_C + F.C, o_Mhi = o_Mlo + 1024, o_Nlo = o_Mhi + nhi,
-                 o_Nhi = o_Nlo + 1024, o_Nr = o_Nhi + nhi, total = o_Nr + F.C;
-    std::vector<double2> tw(total);
-    auto root = [](long long e, long long L) {
-      const long double pi = 3.141592653589793238462643383279502884L;
-      const long double a = -2.0L * pi * (long double)(e % L) / (long double)L;
-      return make_double2((double)cosl(a), (double)sinl(a));
-    };
-    for (int e = 0; e < F.R; e++) tw[o_R + e] = root(e, F.R);
-    for (int e = 0; e < F.C; e++) tw[o_C + e] = root(e, F.C);
-    for (int e = 0; e < 1024; e++) {
-      tw[o_Mlo + e] = root(e, F.M);
-      tw[o_Nlo + e] = root(e, F.N);
-    }
-    for (int e = 0; e < nhi; e++) {
-      tw[o_Mhi + e] = root(1024LL * e, F.M);
-      tw[o_Nhi + e] = root(1024LL * e, F.N);
-    }
-    for (int e = 0; e < F.C; e++) tw[o_Nr + e] = root((long long)F.R * e, F.N);
-    HIPCHK(hipMalloc(&h->d_fft_tw, sizeof(double2) * total));
-    HIPCHK(hipMemcpy(h->d_fft_tw, tw.data(), sizeof(double2) * total, hipMemcpyHostToDevice));
-    F.twR = h->d_fft_tw + o_R;
-    F.twC = h->d_fft_tw + o_C;
-    F.twMlo = h->d_fft_tw + o_Mlo;
-    F.twMhi = h->d_fft_tw + o_Mhi;
-    F.twNlo = h->d_fft_tw + o_Nlo;
-    F.twNhi = h->d_fft_tw + o_Nhi;
-    F.twNr = h->d_fft_tw + o_Nr;
+    std::vector<double2> tw(fft_table_size(F));
+    fft_fill_tables(F, tw.data());
+    HIPCHK(hipMalloc(&h->d_fft_tw, sizeof(double2) * tw.size()));
+    HIPCHK(hipMemcpy(h->d_fft_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+    fft_bind_tables(&F, h->d_fft_tw);
     F.Dp[0] = h->d_data;
     F.Dp[1] = h->d_data + nb;
     F.wp[0] = h->d_w;
@@ -544,12 +529,43 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
   return 0;
 }
 
-static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_spectrum) {
+// flags: the lane's waveform flags when the series is a generated waveform (a
+// waveform rejected on the device yields NaN), nullptr for a given series.
+// The batch's likelihood stage (both FFT passes and the final reduction into
+// the lane's own result slot) captured once per lane as a HIP graph: every
+// pointer it uses belongs to the lane or the handle, so one graph serves
+// every template of the lane.  It holds kernels only (no memset nodes).
+static int lane_like_graph(aak_handle h, Lane &L) {
+  if (L.gx_like) return 0;
+  HIPCHK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+  launch_fft_pass1(h->fft, L.d_h, L.d_h + h_stride(h->N), L.d_fy, L.stream);
+  launch_fft_pass2(h->fft, L.d_fy, L.d_partials, nullptr, L.stream);
+  launch_dot_final(L.d_partials, fft_partials(h->fft), L.d_like, L.d_flags, L.stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(L.stream, &g);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(-100, std::string("HIP: ") + hipGetErrorString(e) + " capturing the likelihood stage");
+  }
+  L.g_like = g;
+  HIPCHK(hipGraphInstantiate(&L.gx_like, g, nullptr, nullptr, 0));
+  return 0;
+}
+
+static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_spectrum, const int *flags) {
+  if (h->fused_fft && h->use_graphs && !keep_spectrum && !h->profiling && flags == L.d_flags) {
+    int rc = lane_like_graph(h, L);
+    if (rc) return rc;
+    HIPCHK(hipGraphLaunch(L.gx_like, L.stream));
+    if (out_dev != L.d_like)
+      HIPCHK(hipMemcpyAsync(out_dev, L.d_like, sizeof(double) * 2, hipMemcpyDeviceToDevice, L.stream));
+    return 0;
+  }
   if (h->fused_fft) {
     launch_fft_pass1(h->fft, L.d_h, L.d_h + h_stride(h->N), L.d_fy, L.stream);
     h->mark(L, 4);
     launch_fft_pass2(h->fft, L.d_fy, L.d_partials, keep_spectrum ? L.d_hf : nullptr, L.stream);
-    launch_dot_final(L.d_partials, fft_partials(h->fft), out_dev, L.d_flags, L.stream);
+    launch_dot_final(L.d_partials, fft_partials(h->fft), out_dev, flags, L.stream);
     h->mark(L, 5);
     HIPCHK(hipGetLastError());
     return 0;
@@ -559,7 +575,7 @@ static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_
   if (rocfft_execute(L.plan, in, outb, L.exec) != rocfft_status_success) return fail(-101, "rocfft execute");
   h->mark(L, 4);
   launch_weighted_dot(L.d_hf, h->d_data, h->d_w, h->nbins, keep_spectrum ? L.d_hf : nullptr, L.d_partials, out_dev,
-                      L.d_flags, L.stream);
+                      flags, L.stream);
   h->mark(L, 5);
   HIPCHK(hipGetLastError());
   return 0;
@@ -582,6 +598,7 @@ static void worker_main(aak_handle h, int id) {
   for (;;) {
     long long i;
     const double *params;
+    const int *nmax_rows = nullptr;
     RingSlot *slot;
     int sidx;
     size_t grow_to = 0;
@@ -596,10 +613,11 @@ static void worker_main(aak_handle h, int id) {
       if (has_work()) {
         // keep about kFeedDepth short plans ready or in progress for the GPU;
         // otherwise work down from the longest, which bound the batch
-        took_cheap = (int)h->ready_slots.size() + h->cheap_inflight < kFeedDepth;
+        took_cheap = (int)h->ready_slots.size() + h->cheap_inflight < h->feed_depth;
         if (took_cheap) h->cheap_inflight++;
         i = took_cheap ? h->b_order[(size_t)--h->b_end] : h->b_order[(size_t)h->b_next++];
         params = h->b_params;
+        nmax_rows = h->b_nmax;
         sidx = h->free_slots.back();
         h->free_slots.pop_back();
         slot = h->ring[(size_t)sidx].get();
@@ -636,7 +654,9 @@ static void worker_main(aak_handle h, int id) {
     PinnedSink sink(&slot->pin, &h->pin_hint);
     std::string err;
     const auto t_plan = std::chrono::steady_clock::now();
-    const int rc = plan_safe(h->cfg, sp, &plan, &err, &sink);
+    WaveConfig cfg = h->cfg;
+    if (nmax_rows) cfg.n_max = nmax_rows[i];
+    const int rc = plan_safe(cfg, sp, &plan, &err, &sink);
     slot->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
     slot->P = plan.P;
     slot->tab_n = plan.tab_n;
@@ -677,6 +697,11 @@ static int start_pool(aak_handle h) {
   for (int t = 0; t < T; t++) h->workers.emplace_back(worker_main, h, t);
   return 0;
 }
+
+namespace aak {
+// The calling thread's error message, for entry points in other files.
+int set_last_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace aak
 
 extern "C" {
 
@@ -763,6 +788,7 @@ int aak_create(double T_fit, int init_length, int length, double init_dt, double
   h->cfg.backint = backint ? 1 : 0;
   h->N = length;
   h->nbins = length / 2 + 1;
+  if (const char *env = std::getenv("AAK_GRAPHS")) h->use_graphs = std::atoi(env) != 0;
   if (const char *env = std::getenv("AAK_LANES")) {
     const int v = std::atoi(env);
     h->nlanes = v < 1 ? 1 : (v > kMaxLanes ? kMaxLanes : v);
@@ -798,6 +824,130 @@ int aak_destroy(aak_handle h) {
   return 0;
 }
 
+int aak_set_map_init(aak_handle h, int mode, aak_mlp m) {
+  if (!h) return fail(-1, "null handle");
+  if (mode != 0 && mode != 1) return fail(-2, "map init mode must be 0 or 1");
+  if (mode == 1 && !m) return fail(-2, "map init mode 1 needs a model");
+  h->cfg.map_init = mode;
+  h->cfg.mlp = mode == 1 ? m : nullptr;
+  return 0;
+}
+
+double aak_p_lso(double a, double e, double iota) {
+  Inspiral<HostCx> insp;
+  if (insp.init(a)) return std::nan("");
+  return insp.p_lso(cos(iota), e);
+}
+
+int aak_p_lso_batch(const double *aei, int n, double *out, int threads) {
+  if (n < 0 || (n > 0 && (!aei || !out))) return fail(-2, "bad arguments");
+  int T = threads > 0 ? threads : cpu_share();
+  T = std::max(1, std::min(T, n));
+  auto work = [&](int id) {
+    for (long long i = id; i < n; i += T) out[i] = aak_p_lso(aei[3 * i], aei[3 * i + 1], aei[3 * i + 2]);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; t++) pool.emplace_back(work, t);
+  work(0);
+  for (auto &t : pool) t.join();
+  return 0;
+}
+
+int aak_freqmap_batch(const double *pts, int n, int mode, aak_mlp m, double tol, int threads, double *maps,
+                      int *iters, double *resid, double *stats) {
+  if (n < 0 || (n > 0 && (!pts || !maps))) return fail(-2, "bad arguments");
+  if (mode != 0 && mode != 1) return fail(-2, "mode must be 0 or 1");
+  if (mode == 1 && !m) return fail(-2, "mode 1 needs a model");
+  if (!(tol > 0)) return fail(-2, "tolerance must be positive");
+  if (stats)
+    for (int k = 0; k < 4; k++) stats[k] = 0.0;
+  if (n == 0) return 0;
+  std::vector<double> starts;
+  if (mode == 1) {
+    starts.resize(3 * (size_t)n);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (mlp_infer(m, pts, n, starts.data()) != 0) return -100;
+    if (stats) stats[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  int T = threads > 0 ? threads : cpu_share();
+  T = std::max(1, std::min(T, n));
+  std::vector<double> geo_ms(T, 0.0), solve_ms(T, 0.0);
+  std::vector<int> bad(T, 0), fb(T, 0);
+  auto work = [&](int id) {
+    for (long long i = id; i < n; i += T) {
+      const double *pt = pts + 4 * i;
+      const double a = pt[0], p = pt[1], e = pt[2], iota = pt[3];
+      double *x = maps + 3 * i;
+      const auto t0 = std::chrono::steady_clock::now();
+      Geodesic<HostCx> g;
+      double Om[3];
+      const bool ok = g.init(p, e, cos(iota), a, 1) == 0;
+      if (ok) g.frequencies(Om);
+      const auto t1 = std::chrono::steady_clock::now();
+      geo_ms[id] += std::chrono::duration<double, std::milli>(t1 - t0).count();
+      if (!ok || !std::isfinite(Om[0]) || !std::isfinite(Om[1]) || !std::isfinite(Om[2])) {
+        x[0] = x[1] = x[2] = std::nan("");
+        if (iters) iters[i] = 0;
+        if (resid) resid[i] = std::nan("");
+        bad[id]++;
+        continue;
+      }
+      FreqMap fm{Om, 1.0, e, iota};
+      int it;
+      if (mode == 0) {
+        x[0] = 1. / sqrt(p);
+        x[1] = 1.0;
+        x[2] = a;
+        it = fm.solve(x, tol);
+      } else {
+        for (int k = 0; k < 3; k++) x[k] = starts[3 * i + k];
+        it = fm.polish(x, tol, 8);
+        if (it < 0 || !FreqMap::near_start(x, starts.data() + 3 * i, 0.05)) {
+          x[0] = 1. / sqrt(p);
+          x[1] = 1.0;
+          x[2] = a;
+          it = -fm.solve(x, tol);
+          fb[id]++;
+        }
+      }
+      solve_ms[id] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+      double f[3];
+      fm.residual(x, f);
+      const double r = fabs(f[0]) + fabs(f[1]) + fabs(f[2]);
+      if (iters) iters[i] = it;
+      if (resid) resid[i] = r;
+      if (!std::isfinite(r)) bad[id]++;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; t++) pool.emplace_back(work, t);
+  work(0);
+  for (auto &t : pool) t.join();
+  int nbad = 0;
+  for (int t = 0; t < T; t++) {
+    nbad += bad[t];
+    if (stats) {
+      stats[0] += geo_ms[t];
+      stats[2] += solve_ms[t];
+      stats[3] += fb[t];
+    }
+  }
+  return nbad;
+}
+
+int aak_set_tuning(aak_handle h, int feed_depth, int expansion_shift) {
+  if (!h) return fail(-1, "null handle");
+  if (feed_depth == 0 || feed_depth > 256) return fail(-2, "feed depth must be in [1, 256] (or < 0: keep)");
+  if (expansion_shift > 3) return fail(-2, "expansion shift must be in [0, 3] (or < 0: keep)");
+  if (feed_depth > 0) {
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->feed_depth = feed_depth;
+  }
+  if (expansion_shift >= 0) h->cfg.exp_shift = expansion_shift;
+  if (expansion_shift == -2) h->cfg.exp_shift = -1;  // back to the precision's default
+  return 0;
+}
+
 int aak_set_nmax(aak_handle h, int n_max) {
   if (!h) return fail(-1, "null handle");
   if (n_max > 200) return fail(-2, "n_max above 200");
@@ -814,7 +964,7 @@ int aak_set_precision(aak_handle h, int mode) {
 
 int aak_set_lanes(aak_handle h, int n) {
   if (!h) return fail(-1, "null handle");
-  if (n < 1 || n > kMaxLanes) return fail(-2, "lanes must be in [1, 4]");
+  if (n < 1 || n > kMaxLanes) return fail(-2, "lanes must be in [1, " + std::to_string(kMaxLanes) + "]");
   h->nlanes = n;
   return 0;
 }
@@ -981,7 +1131,7 @@ int aak_likelihood(aak_handle h, double *like_out) {
   if (h->state != 1) return fail(-2, "Likelihood needs a waveform from gpu_gen_AAK");
   Lane &L = h->lane[0];
   h->mark(L, 3);
-  int rc = enqueue_likelihood(h, L, L.d_like, true);
+  int rc = enqueue_likelihood(h, L, L.d_like, true, L.d_flags);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(like_out, L.d_like, sizeof(double) * 2, hipMemcpyDeviceToHost, L.stream));
   HIPCHK(hipStreamSynchronize(L.stream));
@@ -991,10 +1141,18 @@ int aak_likelihood(aak_handle h, double *like_out) {
 }
 
 int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like_out, int *status) {
+  return aak_likelihood_batch_nmax(h, params, nullptr, n, like_out, status);
+}
+
+int aak_likelihood_batch_nmax(aak_handle h, const double *params, const int *nmax, int n, double *like_out,
+                              int *status) {
   if (!h) return fail(-1, "null handle");
   if (!h->have_data) return fail(-2, "likelihood batch before input_data");
   if (n < 0 || (n > 0 && (!params || !like_out))) return fail(-2, "bad batch arguments");
   if (n == 0) return 0;
+  if (nmax)
+    for (int i = 0; i < n; i++)
+      if (nmax[i] < 0 || nmax[i] > 200) return fail(-2, "per-template harmonic count outside 0..200");
   const auto t_call = std::chrono::steady_clock::now();
   int rc = start_pool(h);
   if (rc) return rc;
@@ -1046,6 +1204,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
     h->ready_slots.clear();
     for (int s = (int)h->ring.size() - 1; s >= 0; s--) h->free_slots.push_back(s);
     h->b_params = params;
+    h->b_nmax = nmax;
     h->b_n = n;
     h->b_next = 0;
     h->b_end = n;
@@ -1076,7 +1235,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
       rc = enqueue_waveform(h, L, slot.P, slot.pin.p, slot.tab_n, (int)((done / nl) & 1), slot.copied);
       if (!rc) {
         slot.copy_pending = true;
-        rc = enqueue_likelihood(h, L, h->d_like_batch + 2 * i, false);
+        rc = enqueue_likelihood(h, L, h->d_like_batch + 2 * i, false, L.d_flags);
       }
       if (rc) {
         err_rc = rc;
@@ -1093,6 +1252,7 @@ int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like
   {
     std::lock_guard<std::mutex> lk(h->mu);
     h->b_params = nullptr;
+    h->b_nmax = nullptr;
   }
   h->evs = h->ev;
   h->state = 0;
@@ -1347,7 +1507,7 @@ int aak_test_inner_product(aak_handle h, const double *hI, const double *hII, do
   Lane &L = h->lane[0];
   HIPCHK(hipMemcpyAsync(L.d_h, hI, sizeof(double) * n, hipMemcpyHostToDevice, L.stream));
   HIPCHK(hipMemcpyAsync(L.d_h + ld, hII, sizeof(double) * n, hipMemcpyHostToDevice, L.stream));
-  const int rc = enqueue_likelihood(h, L, L.d_like, spec != nullptr);
+  const int rc = enqueue_likelihood(h, L, L.d_like, spec != nullptr, nullptr);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(like, L.d_like, sizeof(double) * 2, hipMemcpyDeviceToHost, L.stream));
   if (spec) HIPCHK(hipMemcpyAsync(spec, L.d_hf, sizeof(double2) * 2 * h->nbins, hipMemcpyDeviceToHost, L.stream));

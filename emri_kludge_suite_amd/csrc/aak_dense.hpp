@@ -74,10 +74,15 @@ AAK_HD void spline_interval(double x0, double x1, double y0, double y1, double c
 
 // Single-precision copies of the response constants, for the mixed form's
 // per-sample geometry.
+// Constants of the mixed form's per-sample response, built from unit
+// vectors: the orbital-plane pair E1 (the spin's projection, from which the
+// pericentre angle gamma is measured) and E2 = L x E1, both affine in
+// (sin alpha, cos alpha): E1 = e1c + e1s sin a + e1k cos a, E2 = e2s sin a +
+// e2k cos a; the source direction n; the sky polarisation basis (p, q) of
+// the LISA = false response.
 struct GeomF32 {
-  float cosqL[3], phiLup[3], phiLdown[3], betadown[3], rc[3], rs[3], nxL2[6];
-  float cosqS, sinqS, cos_phS, sin_phS, cos_phK, sin_phK, cosqK, sinqK, Sdotn, coslam, halfsqrt3;
-  float psi_c0, psi_c1, psi_c2, psi_c3, nxS2, ldc_c2psi, ldc_s2psi;
+  float e1c[3], e1s[3], e1k[3], e2s[3], e2k[3], n[3], p[3], q[3];
+  float cos_phS, sin_phS;
 };
 
 // Per-waveform scalars of the dense stage, filled on the host.
@@ -114,6 +119,7 @@ struct DenseParams {
   double t_plunge, t_zero;
   KnotGrid grid;
   int mixed;                      // 1: FP32 harmonic loop (mixed precision)
+  int xshift;                     // expansion blocks of the mode sum span 256 << xshift samples
   int full_dense;                 // 1: every dense array at every sample (single waveforms: phase
                                   // output, dense arrays); 0: only what a likelihood reads
   GeomF32 gf;                     // response constants in FP32 (mixed form)

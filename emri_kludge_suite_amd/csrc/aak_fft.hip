@@ -357,6 +357,60 @@ bool fft_supported(int N, int *R, int *C) {
 
 int fft_partials(const FftArgs &F) { return F.R; }
 
+// Table layout: W_R | W_C | W_M lo | W_M hi | W_N lo | W_N hi | W_N^(R k2).
+static void table_offsets(const FftArgs &F, size_t o[8]) {
+  const size_t nhi = (size_t)F.M / 1024 + 2;
+  o[0] = 0;
+  o[1] = o[0] + F.R;
+  o[2] = o[1] + F.C;
+  o[3] = o[2] + 1024;
+  o[4] = o[3] + nhi;
+  o[5] = o[4] + 1024;
+  o[6] = o[5] + nhi;
+  o[7] = o[6] + F.C;
+}
+
+size_t fft_table_size(const FftArgs &F) {
+  size_t o[8];
+  table_offsets(F, o);
+  return o[7];
+}
+
+void fft_fill_tables(const FftArgs &F, double2 *tw) {
+  // W_L^e = exp(-2 pi i e / L), computed in extended precision
+  auto root = [](long long e, long long L) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const long double a = -2.0L * pi * (long double)(e % L) / (long double)L;
+    return make_double2((double)cosl(a), (double)sinl(a));
+  };
+  size_t o[8];
+  table_offsets(F, o);
+  const int nhi = (int)(o[4] - o[3]);
+  for (int e = 0; e < F.R; e++) tw[o[0] + e] = root(e, F.R);
+  for (int e = 0; e < F.C; e++) tw[o[1] + e] = root(e, F.C);
+  for (int e = 0; e < 1024; e++) {
+    tw[o[2] + e] = root(e, F.M);
+    tw[o[4] + e] = root(e, F.N);
+  }
+  for (int e = 0; e < nhi; e++) {
+    tw[o[3] + e] = root(1024LL * e, F.M);
+    tw[o[5] + e] = root(1024LL * e, F.N);
+  }
+  for (int e = 0; e < F.C; e++) tw[o[6] + e] = root((long long)F.R * e, F.N);
+}
+
+void fft_bind_tables(FftArgs *F, const double2 *tw) {
+  size_t o[8];
+  table_offsets(*F, o);
+  F->twR = tw + o[0];
+  F->twC = tw + o[1];
+  F->twMlo = tw + o[2];
+  F->twMhi = tw + o[3];
+  F->twNlo = tw + o[4];
+  F->twNhi = tw + o[5];
+  F->twNr = tw + o[6];
+}
+
 void launch_fft_pass1(const FftArgs &F, const double *h0, const double *h1, double2 *Y, hipStream_t s) {
   switch (F.R) {
     case 16: launch_cols<16>(F, h0, h1, Y, s); break;

@@ -22,6 +22,12 @@ struct FftArgs {
 bool fft_supported(int N, int *R, int *C);
 // Partial-sum pairs written by launch_fft_pass2.
 int fft_partials(const FftArgs &F);
+// Twiddle tables of F's lengths: their size (double2 entries), their values
+// (host memory, extended-precision roots) and the table pointers of F bound
+// to a device copy.
+size_t fft_table_size(const FftArgs &F);
+void fft_fill_tables(const FftArgs &F, double2 *tw);
+void fft_bind_tables(FftArgs *F, const double2 *tw);
 // Channels h0 and h1 (16-byte aligned).  Pass 1 writes the 2M-complex scratch Y;
 // pass 2 writes fft_partials(F) pairs (sum of <h w, d w>, sum of |h w|^2) to
 // partials and, if hw_out is not null, the weighted spectra h~ w ([2][M+1],

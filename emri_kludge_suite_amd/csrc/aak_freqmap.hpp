@@ -248,6 +248,48 @@ struct FreqMap {
     }
     return status ? status : it;
   }
+
+  // Whether a polished root stayed near its learned start (each component
+  // within `rel` of it, relative): a Newton run that wandered further may have
+  // found another root of the system and is not trusted.
+  AAK_HD static bool near_start(const double *x, const double *x0, double rel) {
+    for (int j = 0; j < 3; j++)
+      if (!(fabs(x[j] - x0[j]) <= rel * fmax(fabs(x0[j]), 1e-3))) return false;
+    return true;
+  }
+
+  // Newton iteration from a close start x (in/out): forward-difference
+  // Jacobian (as the hybrid's), each step solved by QR.  Stops once
+  // sum|f| <= tol; returns the Newton steps taken (0 if x already met tol) or
+  // -1 when max_iter steps do not meet it or a step leaves the finite range
+  // (x is then left at its last finite iterate).
+  AAK_HD int polish(double *x, double tol, int max_iter) const {
+    double f[3];
+    residual(x, f);
+    for (int it = 0;; it++) {
+      if (!(is_finite(f[0]) && is_finite(f[1]) && is_finite(f[2]))) return -1;
+      if (fabs(f[0]) + fabs(f[1]) + fabs(f[2]) <= tol) return it;
+      if (it >= max_iter) return -1;
+      double J[3][3], Q[3][3], R[3][3], qtf[3], dx[3];
+      jacobian(x, f, J);
+      qr(J, Q, R, nullptr);
+      for (int j = 0; j < 3; j++) qtf[j] = Q[0][j] * f[0] + Q[1][j] * f[1] + Q[2][j] * f[2];
+      for (int j = 2; j >= 0; j--) {
+        double s = qtf[j];
+        for (int k = j + 1; k < 3; k++) s -= R[j][k] * dx[k];
+        if (R[j][j] == 0.0) return -1;
+        dx[j] = s / R[j][j];
+      }
+      double xt[3], ft[3];
+      for (int j = 0; j < 3; j++) xt[j] = x[j] - dx[j];
+      residual(xt, ft);
+      if (!(is_finite(ft[0]) && is_finite(ft[1]) && is_finite(ft[2]))) return -1;
+      for (int j = 0; j < 3; j++) {
+        x[j] = xt[j];
+        f[j] = ft[j];
+      }
+    }
+  }
 };
 
 }  // namespace aak

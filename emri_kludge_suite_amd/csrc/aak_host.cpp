@@ -43,6 +43,23 @@ int sparse_stage(const WaveConfig &cfg, const SourceParams &sp, SparseTrajectory
     *err = "trajectory too short (p0 at or inside the LSO?)";
     return -2;
   }
+  // learned start points of all maps at once (one batched inference)
+  const bool learned = cfg.map_init == 1 && cfg.mlp;
+  double starts[3 * 100];
+  if (learned) {
+    double q[4 * 100];
+    for (int i = 0; i < steps; i++) {
+      q[4 * i] = sp.s;
+      q[4 * i + 1] = pts[i].p;
+      q[4 * i + 2] = pts[i].e;
+      q[4 * i + 3] = sp.iota;
+    }
+    if (mlp_infer(cfg.mlp, q, steps, starts) != 0) {
+      *err = "frequency-map model inference failed";
+      return -1;
+    }
+  }
+  out->map_fallbacks = 0;
   for (int i = 0; i < steps; i++) {
     Geodesic<HostCx> g;
     if (g.init(pts[i].p, pts[i].e, pts[i].cosi, sp.s, 1)) {
@@ -57,7 +74,24 @@ int sparse_stage(const WaveConfig &cfg, const SourceParams &sp, SparseTrajectory
     }
     FreqMap fm{Om, sp.M, pts[i].e, sp.iota};
     double x[3] = {1. / sqrt(pts[i].p), sp.M, sp.s};
-    out->map_iters[i] = fm.solve(x);
+    if (learned) {
+      // Newton polish of the learned start to the reference's tolerance (or
+      // 1e-12 M, whichever is smaller); the reference's solve otherwise
+      const double y0[3] = {starts[3 * i], sp.M * starts[3 * i + 1], starts[3 * i + 2]};
+      double y[3] = {y0[0], y0[1], y0[2]};
+      const int k = fm.polish(y, fmin(1.e-6, 1.e-12 * sp.M), 8);
+      if (k >= 0 && FreqMap::near_start(y, y0, 0.05)) {
+        x[0] = y[0];
+        x[1] = y[1];
+        x[2] = y[2];
+        out->map_iters[i] = k;
+      } else {
+        out->map_iters[i] = -fm.solve(x);
+        out->map_fallbacks++;
+      }
+    } else {
+      out->map_iters[i] = fm.solve(x);
+    }
     out->e_traj[i] = pts[i].e;
     out->v_map[i] = x[0];
     out->M_map[i] = x[1];
@@ -687,36 +721,30 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   const double e_bound = e_hi * 1.01 + 0.01;
   P.bessel_series = (nmodes <= kMaxModes && e_bound <= kSeriesMaxE) ? 1 : 0;
   P.mixed = cfg.mixed ? 1 : 0;
+  P.xshift = cfg.exp_shift >= 0 ? cfg.exp_shift : (cfg.mixed ? 2 : 0);
   P.full_dense = 1;
   {
+    // mixed form: E1 = sin(lam) S - cos(lam) (cos a e1 + sin a e2) and
+    // E2 = L x E1 = sin a e1 - cos a e2, with L's precession basis
+    // e1 = (z - S cos theta_K) / sin theta_K, e2 = S x e1
     GeomF32 &F = P.gf;
+    const double e1[3] = {-P.cos_thK * P.cos_phK, -P.cos_thK * P.sin_phK, P.sin_thK};
+    const double e2[3] = {P.sin_phK, -P.cos_phK, 0.0};
+    const double pv[3] = {cos(qS) * cos(phiS), cos(qS) * sin(phiS), -sin(qS)};
+    const double *n = P.n_vec;
+    const double qv[3] = {pv[1] * n[2] - pv[2] * n[1], pv[2] * n[0] - pv[0] * n[2], pv[0] * n[1] - pv[1] * n[0]};
     for (int k = 0; k < 3; k++) {
-      F.cosqL[k] = (float)P.g_cosqL[k];
-      F.phiLup[k] = (float)P.g_phiLup[k];
-      F.phiLdown[k] = (float)P.g_phiLdown[k];
-      F.betadown[k] = (float)P.g_betadown[k];
-      F.rc[k] = (float)P.g_rc[k];
-      F.rs[k] = (float)P.g_rs[k];
+      F.e1c[k] = (float)(P.siniota_L * P.S_vec[k]);
+      F.e1s[k] = (float)(-P.cosiota_L * e2[k]);
+      F.e1k[k] = (float)(-P.cosiota_L * e1[k]);
+      F.e2s[k] = (float)e1[k];
+      F.e2k[k] = (float)(-e2[k]);
+      F.n[k] = (float)n[k];
+      F.p[k] = (float)pv[k];
+      F.q[k] = (float)qv[k];
     }
-    for (int k = 0; k < 6; k++) F.nxL2[k] = (float)P.g_nxL2[k];
-    F.cosqS = (float)P.cosqS;
-    F.sinqS = (float)P.sinqS;
     F.cos_phS = (float)P.cos_phS;
     F.sin_phS = (float)P.sin_phS;
-    F.cos_phK = (float)P.cos_phK;
-    F.sin_phK = (float)P.sin_phK;
-    F.cosqK = (float)P.cosqK;
-    F.sinqK = (float)P.sinqK;
-    F.Sdotn = (float)P.Sdotn;
-    F.coslam = (float)P.coslam;
-    F.halfsqrt3 = (float)P.halfsqrt3;
-    F.psi_c0 = (float)P.psi_c0;
-    F.psi_c1 = (float)P.psi_c1;
-    F.psi_c2 = (float)P.psi_c2;
-    F.psi_c3 = (float)P.psi_c3;
-    F.nxS2 = (float)P.nxS2;
-    F.ldc_c2psi = (float)P.ldc_c2psi;
-    F.ldc_s2psi = (float)P.ldc_s2psi;
   }
   if (P.bessel_series) bessel_series_terms(e_bound, nmodes, P.bterms);
   return 0;

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "aak_dense.hpp"
+#include "aak_mlp.hpp"
 
 namespace aak {
 
@@ -20,6 +21,14 @@ struct WaveConfig {
   int backint = 0;
   int n_max = 0;  // > 0 overrides the reference harmonic rule
   int mixed = 0;  // 1: mixed-precision mode sum
+  // mode-sum expansion blocks of 256 << exp_shift samples; -1: 2 in the
+  // mixed form, 0 in FP64 (any value keeps the accuracy: the Taylor order
+  // adapts, and blocks beyond order 8 take the per-sample path)
+  int exp_shift = -1;
+  // start point of the frequency map: 0 the reference's (1/sqrt(p), M, s),
+  // 1 the learned model `mlp` plus a Newton polish (hybrid solver as fallback)
+  int map_init = 0;
+  aak_mlp_s *mlp = nullptr;
 };
 
 // 14 source parameters in the reference order (AAK_manager.hh:134-148).
@@ -31,7 +40,8 @@ struct SparseTrajectory {
   int steps = 0;
   double e_traj[100], v_map[100], M_map[100], s_map[100], dt_map[100];
   double Phi0 = 0.0;
-  int map_iters[100];
+  int map_iters[100];  // hybrid iterations, or Newton steps of the learned start (negative: fell back)
+  int map_fallbacks = 0;
 };
 
 // Knots of the AK evolution.  The arrays span the whole index range j of the

@@ -24,6 +24,9 @@
 #include <cstdint>
 #include <cstdio>
 
+#include <algorithm>
+#include <mutex>
+
 #include "aak_dense.hpp"
 #include "aak_math.hpp"
 
@@ -1201,22 +1204,28 @@ __device__ inline float reduce_2pi_f(double x) {
 }
 
 // Hardware sine/cosine (v_sin_f32, v_cos_f32; absolute error ~1e-6 for
-// |x| <= 4 pi) and reciprocal (v_rcp_f32, 1 ulp): the response factors they
-// feed set amplitudes, where such errors are far below the FP32 harmonic
-// sum's own.
+// |x| <= 4 pi): the response factors they feed set amplitudes, where such
+// errors are far below the FP32 harmonic sum's own.
 __device__ inline void sincos_hw(float x, float *s, float *c) {
   *s = __sinf(x);
   *c = __cosf(x);
 }
-__device__ inline float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// The mixed form's per-sample geometry: the same response as sample_geometry
-// in FP32, after the large angles (gamma, alpha, the orbital phase) are
-// reduced mod 2 pi in FP64.  2 beta enters only through cos and sin of
-// 2 (gamma + beta), formed from the double-angle identities of
-// beta = atan2(y, x) (cos 2 beta = (x^2 - y^2) / r^2, sin 2 beta = 2 x y / r^2;
-// 1 and 0 at x = y = 0, where atan2 gives 0 or pi).
-// The phase, its rates, e, t and the amplitude are read as they are.
+// The mixed form's per-sample geometry: the same response as sample_geometry,
+// rebuilt in FP32 from unit vectors after the large angles (gamma, alpha, the
+// orbital phase) are reduced mod 2 pi in FP64.  With P = cos g E1 + sin g E2,
+// Q = L x P (E1, E2 the orbital-plane pair of GeomF32), the harmonic-n
+// quadrupole is -a_n (PP - QQ) - b_n (PQ + QP) + c_n (PP + QQ), and a
+// channel with (transverse-traceless) response tensor R reads it as R : H.
+// For LISA, R = sqrt 3 Lambda(D) with D = (x x - y y)/2 (channel I) or
+// (x y + y x)/2 (channel II) on the cartwheel's arm frame (x, y) at orbital
+// phase 2 pi t / yr, and Lambda the TT projection about n:
+// E Lambda(D) E' = eps D eps' + (eps . eps') (n D n) / 2, eps = E - (n.E) n.
+// Without LISA, R = pp - qq and pq + qp on the sky basis.  Nothing here
+// divides by a quantity that can vanish, so orientations where the reference's
+// angles are ill-conditioned (n along L or S, the source near the detector's
+// normal) cost no accuracy.  The phase, its rates, e, t and the amplitude are
+// read as they are.
 __device__ inline void sample_geometry_mixed(const DenseParams &P, const DenseBuffers &B, int i, SampleGeom &G) {
 #pragma clang fp contract(fast)
   const GeomF32 &F = P.gf;
@@ -1228,73 +1237,56 @@ __device__ inline void sample_geometry_mixed(const DenseParams &P, const DenseBu
   G.nu = (i == 0) ? B.nu0[0] : B.rate[1][i] * (0.5 / kPi);
   G.Amp = B.amp[i];
   const float gim = reduce_2pi_f(B.phase[0][i]), alp = reduce_2pi_f(B.phase[2][i]);
-  float sa, ca;
+  float sa, ca, s2g, c2g;
   sincos_hw(alp, &sa, &ca);
-  auto aff = [&](const float *c) { return c[0] + c[1] * sa + c[2] * ca; };
-  const float cosqL = aff(F.cosqL);
-  const float sinqL = sqrtf(fmaxf(1.f - cosqL * cosqL, 0.f));
-  const float phiLup = aff(F.phiLup), phiLdown = aff(F.phiLdown);
-  const float rL2 = phiLup * phiLup + phiLdown * phiLdown;
-  const float irL = rL2 > 0.f ? rsqrtf(rL2) : 0.f;
-  const float cphiL = rL2 > 0.f ? phiLdown * irL : 1.f, sphiL = phiLup * irL;
-  const float Ldotn = cosqL * F.cosqS + sinqL * F.sinqS * (cphiL * F.cos_phS + sphiL * F.sin_phS);
-  const float Ldotn2 = Ldotn * Ldotn;
-  const float by = -F.Sdotn + F.coslam * Ldotn, bx = aff(F.betadown);
-  const float rb2 = bx * bx + by * by;
-  const float irb2 = rb2 > 0.f ? rcp_hw(rb2) : 0.f;
-  const float c2b = rb2 > 0.f ? (bx * bx - by * by) * irb2 : 1.f, s2b = 2.f * bx * by * irb2;
-  float s2g, c2g;
   sincos_hw(2.f * gim, &s2g, &c2g);
-  const float sin2gam = s2g * c2b + c2g * s2b, cos2gam = c2g * c2b - s2g * s2b;
-  float cosorbphs = 0.f, FplusI, FcrosI, FplusII, FcrosII, h3;
+  float E1[3], E2[3];
+  for (int k = 0; k < 3; k++) {
+    E1[k] = F.e1c[k] + F.e1s[k] * sa + F.e1k[k] * ca;
+    E2[k] = F.e2s[k] * sa + F.e2k[k] * ca;
+  }
+  // r[ch][0..2] = E1 R E1, E2 R E2, E1 R E2 of each channel
+  float r[2][3];
+  float cosorbphs = 0.f;
   if (P.mich) {
     float so, co;
     sincos_hw(reduce_2pi_f(2. * kPi * ti / kYear), &so, &co);
     cosorbphs = co * F.cos_phS + so * F.sin_phS;
-    const float sinorbphs = so * F.cos_phS - co * F.sin_phS;
-    const float cosorbK = co * F.cos_phK + so * F.sin_phK;
-    const float cosq = .5f * F.cosqS - F.halfsqrt3 * F.sinqS * cosorbphs;
-    const float Yw = F.halfsqrt3 * F.cosqS + .5f * F.sinqS * cosorbphs, Xw = F.sinqS * sinorbphs;
-    const float rw = Xw * Xw + Yw * Yw;
-    const float irw = rw > 0.f ? rcp_hw(rw) : 0.f;
-    const float c2w = rw > 0.f ? (Xw * Xw - Yw * Yw) * irw : 1.f, s2w = 2.f * Xw * Yw * irw;
-    const float c2o = co * co - so * so, s2o = 2.f * so * co;
-    const float cos2phi = c2o * c2w - s2o * s2w, sin2phi = s2o * c2w + c2o * s2w;
-    const float psiup = .5f * F.cosqK - F.halfsqrt3 * F.sinqK * cosorbK - cosq * F.psi_c0;
-    const float psidown = F.psi_c1 - F.halfsqrt3 * co * F.psi_c2 - F.halfsqrt3 * so * F.psi_c3;
-    const float rp = psiup * psiup + psidown * psidown;
-    const float irp = rp > 0.f ? rcp_hw(rp) : 0.f;
-    const float cos2psi = rp > 0.f ? (psidown * psidown - psiup * psiup) * irp : 1.f;
-    const float sin2psi = 2.f * psidown * psiup * irp;
-    const float cosq1 = .5f * (1.f + cosq * cosq);
-    FplusI = cosq1 * cos2phi * cos2psi - cosq * sin2phi * sin2psi;
-    FcrosI = cosq1 * cos2phi * sin2psi + cosq * sin2phi * cos2psi;
-    FplusII = cosq1 * sin2phi * cos2psi + cosq * cos2phi * sin2psi;
-    FcrosII = cosq1 * sin2phi * sin2psi - cosq * cos2phi * cos2psi;
-    h3 = F.halfsqrt3;
+    const float h3 = 0.8660254037844386f;
+    const float x[3] = {0.5f * so * co, -(co * co + 0.5f * so * so), -h3 * so};
+    const float y[3] = {so * so + 0.5f * co * co, -0.5f * so * co, h3 * co};
+    auto dot3 = [](const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    const float nx = dot3(F.n, x), ny = dot3(F.n, y), n1 = dot3(F.n, E1), n2 = dot3(F.n, E2);
+    const float x1 = dot3(E1, x) - n1 * nx, y1 = dot3(E1, y) - n1 * ny;
+    const float x2 = dot3(E2, x) - n2 * nx, y2 = dot3(E2, y) - n2 * ny;
+    const float g11 = 1.f - n1 * n1, g22 = 1.f - n2 * n2, g12 = -n1 * n2;
+    const float tI = 0.5f * (nx * nx - ny * ny), tII = nx * ny;
+    const float s3 = 1.7320508075688772f;
+    r[0][0] = s3 * (0.5f * (x1 * x1 - y1 * y1) + 0.5f * g11 * tI);
+    r[0][1] = s3 * (0.5f * (x2 * x2 - y2 * y2) + 0.5f * g22 * tI);
+    r[0][2] = s3 * (0.5f * (x1 * x2 - y1 * y2) + 0.5f * g12 * tI);
+    r[1][0] = s3 * (x1 * y1 + 0.5f * g11 * tII);
+    r[1][1] = s3 * (x2 * y2 + 0.5f * g22 * tII);
+    r[1][2] = s3 * (0.5f * (x1 * y2 + y1 * x2) + 0.5f * g12 * tII);
   } else {
-    FplusI = F.ldc_c2psi;
-    FcrosI = -F.ldc_s2psi;
-    FplusII = F.ldc_s2psi;
-    FcrosII = F.ldc_c2psi;
-    h3 = 1.f;
+    const float p1 = F.p[0] * E1[0] + F.p[1] * E1[1] + F.p[2] * E1[2];
+    const float q1 = F.q[0] * E1[0] + F.q[1] * E1[1] + F.q[2] * E1[2];
+    const float p2 = F.p[0] * E2[0] + F.p[1] * E2[1] + F.p[2] * E2[2];
+    const float q2 = F.q[0] * E2[0] + F.q[1] * E2[1] + F.q[2] * E2[2];
+    r[0][0] = p1 * p1 - q1 * q1;
+    r[0][1] = p2 * p2 - q2 * q2;
+    r[0][2] = p1 * p2 - q1 * q2;
+    r[1][0] = 2.f * p1 * q1;
+    r[1][1] = 2.f * p2 * q2;
+    r[1][2] = p1 * q2 + q1 * p2;
   }
   G.cosorbphs = cosorbphs;
-  const float cn = aff(F.rc), sn = aff(F.rs);
-  const float *q2 = F.nxL2;
-  const float nxL2 = q2[0] + q2[1] * (sa * sa) + q2[2] * (ca * ca) + q2[3] * sa + q2[4] * ca + q2[5] * (sa * ca);
-  const float iN2 = rcp_hw(nxL2 * F.nxS2);
-  const float rot0 = 2.f * (cn * cn) * iN2 - 1.f, rot1 = cn * sn * iN2;
-  const float XI = h3 * (FplusI * rot0 - FcrosI * rot1), YI = h3 * (FplusI * rot1 + FcrosI * rot0);
-  const float XII = h3 * (FplusII * rot0 - FcrosII * rot1), YII = h3 * (FplusII * rot1 + FcrosII * rot0);
-  const float pa = -(1.f + Ldotn2) * cos2gam, pb = (1.f + Ldotn2) * sin2gam, pc = 1.f - Ldotn2;
-  const float qa = 2.f * Ldotn * sin2gam, qb = 2.f * Ldotn * cos2gam;
-  G.W[0] = pa * XI + qa * YI;
-  G.W[1] = pb * XI + qb * YI;
-  G.W[2] = pc * XI;
-  G.W[3] = pa * XII + qa * YII;
-  G.W[4] = pb * XII + qb * YII;
-  G.W[5] = pc * XII;
+  for (int ch = 0; ch < 2; ch++) {
+    const float dlt = r[ch][0] - r[ch][1], two12 = 2.f * r[ch][2];
+    G.W[3 * ch + 0] = -(c2g * dlt + s2g * two12);
+    G.W[3 * ch + 1] = -(c2g * two12 - s2g * dlt);
+    G.W[3 * ch + 2] = r[ch][0] + r[ch][1];
+  }
 }
 
 // Planck taper between plunge and zero time (AAK.cc:535-547)
@@ -1325,12 +1317,12 @@ __device__ inline double harmonic_phase(const DenseParams &P, const SampleGeom &
 // per sample by k_modesum_direct.
 constexpr int kMsThreads = 256;
 constexpr int kTaylorModes = 128;
-// Samples per expansion block: 256 << exp_shift.  The mixed form's looser
-// order bound lets one expansion serve 1024 samples (the mean order rises from
-// 2.0 to 2.5 over the production prior while the setup work falls fourfold);
-// FP64 keeps 256, where wider blocks would send more of them to the
-// per-sample path.
-__host__ __device__ inline int exp_shift(const DenseParams &P) { return P.mixed ? 2 : 0; }
+// Samples per expansion block: 256 << exp_shift (host-set, default 2 in the
+// mixed form and 0 in FP64).  The mixed form's looser order bound lets one
+// expansion serve 1024 samples (the mean order rises from 2.0 to 2.5 over the
+// production prior while the setup work falls fourfold); FP64 keeps 256,
+// where wider blocks would send more of them to the per-sample path.
+__host__ __device__ inline int exp_shift(const DenseParams &P) { return P.xshift; }
 constexpr int kSetupThreads = 64;
 // doubles reserved per block in B.ms_coef per harmonic (orders up to kTaylorMax)
 constexpr int kCoefStride = 3 * (kTaylorMax + 1);
@@ -1570,8 +1562,8 @@ __device__ inline bool modesum_tail(const DenseParams &P, const DenseBuffers &B,
 // runs in FP32 (phases, their reduction mod 2 pi, the block expansions and the
 // final scaling stay FP64).
 template <bool MIXED>
-__global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuffers B) {
-  const int blk = blockIdx.x, eb = blk >> exp_shift(P);
+__device__ inline void modesum_block(const DenseParams &P, const DenseBuffers &B, int blk) {
+  const int eb = blk >> exp_shift(P);
   const int p = block_order(B, eb);
   if (p < 0) return;  // k_modesum_direct sums this block
   const int i = blk * kMsThreads + threadIdx.x;
@@ -1620,6 +1612,20 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
   const double sc = (i <= P.ip) ? G.Amp : G.Amp / taper(P, i, G.ti);
   B.h[i] = sc * accI;
   B.h2[i] = sc * accII;
+}
+
+// Persistent and XCD-aware: the grid holds what the chip runs at once, a
+// multiple of the 8 XCDs (workgroups are dealt to the XCDs round-robin, so
+// workgroup g runs on XCD g % 8).  Each XCD walks a contiguous range of whole
+// expansion blocks, so an expansion block's Taylor coefficients are fetched
+// into one L2 only, and its workgroups stride through that range.
+template <bool MIXED>
+__global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuffers B, int nb) {
+  constexpr int kX = 8;
+  const int xs = exp_shift(P), xcd = blockIdx.x % kX, slot = blockIdx.x / kX, nslots = gridDim.x / kX;
+  const long long ne = (nb + (1 << xs) - 1) >> xs;
+  const int lo = (int)(ne * xcd / kX) << xs, hi = min(nb, (int)(ne * (xcd + 1) / kX) << xs);
+  for (int blk = lo + slot; blk < hi; blk += nslots) modesum_block<MIXED>(P, B, blk);
 }
 
 // Per-sample mode sum for the blocks listed by k_modesum_setup (B.ms_p < 0):
@@ -1686,14 +1692,33 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum_direct(DenseParams P, De
   }
 }
 
+// Workgroups of k_modesum the device runs at once (CUs x occupancy), per precision.
+static int modesum_resident(bool mixed) {
+  static int cached[2] = {0, 0};
+  static std::once_flag once[2];
+  std::call_once(once[mixed ? 1 : 0], [&] {
+    int dev = 0, cus = 0, occ = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (mixed)
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_modesum<true>, kMsThreads, 0);
+    else
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_modesum<false>, kMsThreads, 0);
+    cached[mixed ? 1 : 0] = std::max(8, (cus > 0 ? cus : 256) * (occ > 0 ? occ : 2));
+  });
+  return cached[mixed ? 1 : 0];
+}
+
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
   const int nb = modesum_blocks(P), ne = expansion_blocks(P);
+  // a multiple of the 8 XCDs, at most the resident workgroups, at least one per block
+  const int grid = (std::min(nb, modesum_resident(P.mixed != 0)) + 7) / 8 * 8;
   if (P.mixed) {
     hipLaunchKernelGGL(k_modesum_setup<true>, dim3(ne), dim3(kSetupThreads), 0, s, P, B);
-    hipLaunchKernelGGL(k_modesum<true>, dim3(nb), dim3(kMsThreads), 0, s, P, B);
+    hipLaunchKernelGGL(k_modesum<true>, dim3(grid), dim3(kMsThreads), 0, s, P, B, nb);
   } else {
     hipLaunchKernelGGL(k_modesum_setup<false>, dim3(ne), dim3(kSetupThreads), 0, s, P, B);
-    hipLaunchKernelGGL(k_modesum<false>, dim3(nb), dim3(kMsThreads), 0, s, P, B);
+    hipLaunchKernelGGL(k_modesum<false>, dim3(grid), dim3(kMsThreads), 0, s, P, B, nb);
   }
   hipLaunchKernelGGL(k_modesum_direct, dim3(kDirectGrid), dim3(kMsThreads), 0, s, P, B);
 }

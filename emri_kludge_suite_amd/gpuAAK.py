@@ -139,14 +139,16 @@ class GPUAAK:
         return self.Likelihood()
 
     # ------------------------------------------------------------ extensions
-    def LikelihoodBatch(self, params, return_status=False):
+    def LikelihoodBatch(self, params, return_status=False, nmax=None):
         """Likelihoods of many templates against the current data.
 
         ``params``: (n, 14) array in the gpu_gen_AAK order.  Returns an (n, 2)
         array whose rows equal WaveformThroughLikelihood(*params[i]); rows of
         templates whose host stage fails are NaN (their negative status codes
         are returned as well with ``return_status=True``).  The host stages run
-        on a thread pool while the GPU works on earlier templates.
+        on a thread pool while the GPU works on earlier templates.  ``nmax``
+        (optional, n ints): a harmonic count per template (0: the reference's
+        rule), instead of the handle's setting.
         """
         P = np.ascontiguousarray(params, dtype=np.float64)
         if P.ndim != 2 or P.shape[1] != 14:
@@ -154,8 +156,15 @@ class GPUAAK:
         n = P.shape[0]
         out = np.zeros((n, 2))
         st = np.zeros(n, dtype=np.int32)
-        _lib.check(_lib.lib().aak_likelihood_batch(self._h, _lib.dptr(P), n, _lib.dptr(out), _lib.iptr(st)),
-                   "LikelihoodBatch")
+        if nmax is None:
+            rc = _lib.lib().aak_likelihood_batch(self._h, _lib.dptr(P), n, _lib.dptr(out), _lib.iptr(st))
+        else:
+            nm = np.ascontiguousarray(nmax, dtype=np.int32)
+            if nm.shape != (n,):
+                raise ValueError("LikelihoodBatch: nmax needs one entry per template")
+            rc = _lib.lib().aak_likelihood_batch_nmax(self._h, _lib.dptr(P), _lib.iptr(nm), n, _lib.dptr(out),
+                                                      _lib.iptr(st))
+        _lib.check(rc, "LikelihoodBatch")
         return (out, st) if return_status else out
 
     def set_precision(self, mode):
@@ -169,8 +178,21 @@ class GPUAAK:
         _lib.check(_lib.lib().aak_set_host_threads(self._h, int(n)), "set_host_threads")
 
     def set_lanes(self, n):
-        """Concurrent streams of LikelihoodBatch (1..4); results do not depend on it."""
+        """Concurrent streams of LikelihoodBatch (1..8); results do not depend on it."""
         _lib.check(_lib.lib().aak_set_lanes(self._h, int(n)), "set_lanes")
+
+    def set_tuning(self, feed_depth=-1, exp_shift=-1):
+        """Batch pipeline knobs: short host plans kept ready for the GPU
+        (1..256), mode-sum expansion blocks of 256 << exp_shift samples (0..3;
+        -2 restores the precision's default); < 0 (-1) keeps a knob."""
+        _lib.check(_lib.lib().aak_set_tuning(self._h, int(feed_depth), int(exp_shift)), "set_tuning")
+
+    def set_map_init(self, mode, model=None):
+        """Start point of the frequency map: "solver" (the reference's) or
+        "learned" (the network `model` plus a Newton polish, solver fallback)."""
+        m = {"solver": 0, "learned": 1}[mode] if isinstance(mode, str) else int(mode)
+        self._map_model = model  # the handle uses it; keep it alive
+        _lib.check(_lib.lib().aak_set_map_init(self._h, m, model._m if model is not None else None), "set_map_init")
 
     def set_nmax(self, n_max):
         """Override the harmonic count (0 restores (int)(30 e0), min 4)."""

@@ -67,6 +67,13 @@ int aak_get_waveform(aak_handle h, double *t, double *hI, double *hII);
  * GetWaveform). */
 int aak_likelihood_batch(aak_handle h, const double *params, int n, double *like_out, int *status);
 
+/* aak_likelihood_batch with a harmonic count per template (nmax: n ints,
+ * 0 = the reference's rule, > 0 = n_max as aak_set_nmax sets it; NULL = the
+ * handle's setting for all), for batches mixing eccentricities; no
+ * counterpart in the reference. */
+int aak_likelihood_batch_nmax(aak_handle h, const double *params, const int *nmax, int n, double *like_out,
+                              int *status);
+
 /* Arithmetic of the mode sum; no counterpart in the reference.  mode 0
  * (default): plain FP64, within relative L2 1e-10 of the reference CPU AAK.
  * mode 1: mixed precision -- phases, their reduction mod 2 pi, the Bessel
@@ -87,7 +94,14 @@ int aak_set_host_threads(aak_handle h, int n);
  * (sum / (threads x wall)).  Returns the number written. */
 int aak_get_pool_stats(aak_handle h, double *out, int n);
 
-/* Concurrent streams ("lanes", 1..4) over which aak_likelihood_batch spreads
+/* Tuning of the batch pipeline (no counterpart in the reference; results do
+ * not depend on it beyond the mixed form's FP32 rounding): feed_depth = short
+ * host plans kept ready for the GPU (1..256; < 0 keeps it), expansion_shift =
+ * mode-sum expansion blocks of 256 << shift samples (0..3; -1 keeps it, -2
+ * restores the precision's default, 2 mixed / 0 FP64). */
+int aak_set_tuning(aak_handle h, int feed_depth, int expansion_shift);
+
+/* Concurrent streams ("lanes", 1..8) over which aak_likelihood_batch spreads
  * its templates, each with its own device buffers (default AAK_LANES or 2);
  * no counterpart in the reference.  Results do not depend on it. */
 int aak_set_lanes(aak_handle h, int n);
@@ -161,6 +175,71 @@ int aak_get_batch_timings(aak_handle h, double *ms, int n);
 /* Host-stage time of the last batch's templates (ms, on the host pool):
  * their sum and the longest one.  No counterpart in the reference. */
 int aak_get_batch_host_ms(aak_handle h, double *sum_ms, double *max_ms);
+
+/* ---------------------------------------------------------------------
+ * Learned start point of the frequency map (no counterpart in the
+ * reference, whose ParMap starts its hybrid solver at (1/sqrt(p), M, s):
+ * KSParMap.cc:397-440).  A 4-128-128-128-3 GELU network with BF16 weights
+ * and FP32 accumulation on the matrix cores predicts (v_map, M_map/M, s_map)
+ * from (a, p, e, iota); a Newton polish takes it to the solver's tolerance
+ * and the hybrid solver remains the fallback.  Trained on the GPU by the
+ * library's own forward, backward and Adam kernels.
+ * ------------------------------------------------------------------- */
+typedef struct aak_mlp_s *aak_mlp;
+
+/* New model with He-initialised weights (seed) on the current device. */
+int aak_mlp_create(int seed, aak_mlp *out);
+int aak_mlp_destroy(aak_mlp m);
+/* params: the 34051 FP32 parameters W1[4][128] b1[128] W2[128][128] b2[128]
+ * W3[128][128] b3[128] W4[128][3] b4[3] (W[k][n]: input k to output n);
+ * norm (optional): 14 floats, feature centres[4] and scales[4] of
+ * (a, 1/sqrt(p), e, cos iota), target means[3] and scales[3] of
+ * (v_map - 1/sqrt(p), M_map/M - 1, s_map - a).  Setting resets Adam's step. */
+int aak_mlp_set_params(aak_mlp m, const float *params, const float *norm);
+int aak_mlp_set_norm(aak_mlp m, const float *norm);
+/* Copies them out; returns the parameter count. */
+int aak_mlp_get_params(aak_mlp m, float *params, float *norm);
+/* Network outputs Y[n][3] for normalised inputs X[n][4] (no normalisation). */
+int aak_mlp_forward(aak_mlp m, const float *X, int n, float *Y);
+/* Gradient of the mean squared error over all n rows (one batch) with
+ * respect to the 34051 parameters, and the loss; no update. */
+int aak_mlp_gradient(aak_mlp m, const float *X, const float *T, int n, float *grad, double *loss);
+/* Adam training on normalised inputs X[n][4] and targets T[n][3]: `epochs`
+ * passes of n/batch shuffled batches (seed), learning rate decaying from lr0
+ * to lr1 (cosine); loss_per_epoch (optional, epochs doubles) gets the last
+ * batch's loss of each epoch. */
+int aak_mlp_train(aak_mlp m, const float *X, const float *T, int n, int epochs, int batch, float lr0, float lr1,
+                  int seed, double *loss_per_epoch);
+/* Start points maps[n][3] = (v_map, M_map/M, s_map) at pts[n][4] =
+ * (a, p, e, iota), through the model's normalisation. */
+int aak_mlp_predict_maps(aak_mlp m, const double *pts, int n, double *maps);
+
+/* Frequency maps of n points pts[n][4] = (a, p, e, iota) at M = 1 (the map
+ * is linear in M), on `threads` host threads (0: the CPUs the process may run
+ * on).  mode 0: the reference's hybrid solver from (1/sqrt(p), 1, a) down to
+ * sum|f| <= tol; mode 1: start points from the model m (one batched
+ * inference), Newton polish to tol, the hybrid solver from the reference's
+ * start where the polish fails.  maps[n][3] = (v_map, M_map, s_map);
+ * iters[n] (optional): solver iterations (mode 0) or Newton steps (mode 1,
+ * negative: fell back); resid[n] (optional): the final sum|f|; stats
+ * (optional, 4 doubles): host ms of the geodesic frequencies, of the model
+ * (mode 1), of the solves, and the fallback count.  Returns the number of
+ * points whose geodesic or map failed (their maps are NaN). */
+int aak_freqmap_batch(const double *pts, int n, int mode, aak_mlp m, double tol, int threads, double *maps,
+                      int *iters, double *resid, double *stats);
+
+/* The reference's interpolated last-stable-orbit p at (a, e, iota)
+ * (GKTrajFast.cc:20-130, P_lso), as the sparse inspiral uses it. */
+double aak_p_lso(double a, double e, double iota);
+/* The same at n points aei[n][3] = (a, e, iota) on `threads` host threads. */
+int aak_p_lso_batch(const double *aei, int n, double *out, int threads);
+
+/* Start point of the frequency map in the host stage of this handle: 0 the
+ * reference's (default; results as the reference CPU AAK), 1 the model m
+ * plus Newton polish to min(1e-6, 1e-12 M) with the hybrid solver as
+ * fallback (maps then differ from the reference's within its tolerance).
+ * The model must outlive its use by the handle. */
+int aak_set_map_init(aak_handle h, int mode, aak_mlp m);
 
 /* Message of the last failed call on this thread. */
 const char *aak_last_error(void);

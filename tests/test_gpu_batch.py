@@ -191,7 +191,22 @@ def test_mixed_precision_mismatch_below_1e6():
         g.gpu_gen_AAK(*pars)
         _, bI, bII = g.GetWaveform()
         mm = _mismatch((aI, aII), (bI, bII), w, N)
-        print("template", done, "mismatch", mm, flush=True)
+        # the bench's device-side form of the same mismatch: the FP64
+        # template's weighted spectrum as the data of the mixed template
+        dI, dII = np.fft.rfft(aI[:N]) * w, np.fft.rfft(aII[:N]) * w
+        g.input_data(dI, dII, w, w)
+        g.set_precision("fp64")
+        g.gpu_gen_AAK(*pars)
+        hh_64 = g.Likelihood()[1]
+        _, sI, sII = g.GetWaveform()
+        np.testing.assert_allclose(sI.view(np.complex128), dI, rtol=0, atol=1e-12 * np.max(np.abs(dI)))
+        g.input_data(sI.view(np.complex128), sII.view(np.complex128), w, w)
+        g.set_precision("mixed")
+        g.gpu_gen_AAK(*pars)
+        cross, hh_mx = g.Likelihood()
+        mm_dev = 1.0 - cross / np.sqrt(hh_mx * hh_64)
+        print("template", done, "mismatch", mm, "device form", mm_dev, flush=True)
+        assert abs(mm_dev - mm) <= 1e-12
         worst = max(worst, mm)
         done += 1
     assert worst <= 1e-6
