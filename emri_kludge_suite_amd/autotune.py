@@ -100,3 +100,57 @@ def apply(g, N, dt, precision, table=None):
     if s:
         _set(g, {k: s.get(k, DEFAULTS[k]) for k in DEFAULTS})
     return s
+
+
+def _grid_templates(N, dt, n, seed, uneven):
+    """n templates whose host stage succeeds on grid (N, dt): the uneven
+    batch's draws, or the production prior (n_max = 50) for the 1-yr grid."""
+    from . import _lib
+    from .multigrid import config5_workload
+    L = _lib.lib()
+    out, nm = [], []
+    traj, kn = np.zeros(500), np.zeros(5 * 4_000_000)
+    info, dp = np.zeros(16, dtype=np.int32), np.zeros(4)
+    k = 0
+    while len(out) < n:
+        P, _, _, nmax, _ = config5_workload(256, seed + k)
+        k += 1
+        for row, m in zip(P, nmax):
+            m = int(m) if uneven else 50
+            rc = L.aak_test_host_plan(_lib.dptr(np.ascontiguousarray(row)), 1.0, int(N), float(dt), 1, 0, m,
+                                      _lib.dptr(traj), _lib.dptr(kn), 4_000_000, _lib.iptr(info), _lib.dptr(dp))
+            if rc > 0:
+                out.append(row)
+                nm.append(m)
+            if len(out) == n:
+                break
+    return np.array(out), np.array(nm, dtype=np.int32)
+
+
+if __name__ == "__main__":
+    import argparse
+    from .multigrid import MultiGridBatch, config5_workload, injected_data, injection_source
+    ap = argparse.ArgumentParser(description="tune the batch pipeline on the GPU and write the table")
+    ap.add_argument("--out", default=TABLE)
+    ap.add_argument("--templates", type=int, default=48)
+    a = ap.parse_args()
+    table = load(a.out)
+    _, N, dt, _, _ = config5_workload(1024, 6)
+    grids = sorted({(int(x), float(y)) for x, y in zip(N, dt)})
+    jobs = [(2 ** 21, 15.0, "mixed", False), (2 ** 21, 15.0, "fp64", False)] + [(n, d, "mixed", True) for n, d in grids]
+    inj = injection_source(7)
+    for n, d, prec, uneven in jobs:
+        mg = MultiGridBatch(precision=prec, tuning={})
+        injected_data(mg, inj, n, d, seed=7)
+        g = mg.handle(n, d)
+        P, nm = _grid_templates(n, d, a.templates, 11, uneven)
+        if not uneven:
+            g.set_nmax(50)
+            nm = None
+        t0 = time.perf_counter()
+        best, seen = tune(g, P, nm, log=print)
+        table[key(n, d, prec)] = dict(best, ms_per_likelihood=1e3 * min(seen.values()),
+                                      default_ms=1e3 * seen[json.dumps(DEFAULTS, sort_keys=True)])
+        print("%s: %s (%.1f s)" % (key(n, d, prec), table[key(n, d, prec)], time.perf_counter() - t0), flush=True)
+        save(table, a.out)
+        del mg, g

@@ -98,6 +98,22 @@ def config5_workload(n=1024, seed=6):
     return P, N, dt, nmax, T
 
 
+def injection_source(seed=7, N=1 << 22, dt=10.0):
+    """The injected source: the first draw of the uneven batch's prior (seed)
+    whose host stage succeeds on the longest grid (2^22 samples at 10 s)."""
+    from . import _lib
+    L = _lib.lib()
+    traj, kn = np.zeros(500), np.zeros(5 * 4_000_000)
+    info, dp = np.zeros(16, dtype=np.int32), np.zeros(4)
+    for k in range(64):
+        P = config5_workload(16, seed + 1000 * k)[0]
+        for row in P:
+            if L.aak_test_host_plan(_lib.dptr(np.ascontiguousarray(row)), 1.0, N, dt, 1, 0, 0, _lib.dptr(traj),
+                                    _lib.dptr(kn), 4_000_000, _lib.iptr(info), _lib.dptr(dp)) > 0:
+                return row
+    raise RuntimeError("no injection source found")
+
+
 def psd_weights(N, dt, dur):
     """w_k = sqrt(dt / N) / sqrt(S_n(f_k)) on the rFFT grid, f_0 = f_1 / 10
     (pygpuAAK.py:72,86-96)."""

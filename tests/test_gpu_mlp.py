@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import mlp_ref as R
+import oracle_lib as ol
 
 pytestmark = pytest.mark.gpu
 
@@ -84,6 +85,16 @@ def test_training_learns_the_map():
     assert st_m["fallbacks"] < 0.15 * ok.sum()
 
 
+def _sol_fun(x, Om, e, iota):
+    import ctypes as C
+    f = np.zeros(3)
+    xx, oo = np.ascontiguousarray(x, dtype=np.float64), np.ascontiguousarray(Om, dtype=np.float64)
+    pd = C.POINTER(C.c_double)
+    ol.oracle().o_sol_fun(xx.ctypes.data_as(pd), oo.ctypes.data_as(pd), 1.0, float(e), float(iota),
+                          f.ctypes.data_as(pd))
+    return f
+
+
 def _trained():
     import os
     from emri_kludge_suite_amd import freqmap_mlp as F
@@ -107,7 +118,28 @@ def test_trained_model_on_the_evaluation_points():
           "max rel diff of the roots %.2e" % (st_m["fallbacks"], ok.sum(), np.mean(np.abs(it_m[ok])),
                                               np.mean(it_s[ok]), st_m["solve_ms"], st_s["solve_ms"], rel), flush=True)
     assert st_m["fallbacks"] < 0.1 * ok.sum()
-    assert rel < 1e-9
+    # both are roots to sum|f| <= 1e-12; where the map is ill-conditioned
+    # (near the separatrix, or where a direction leaves the three frequencies
+    # unchanged) two such roots differ by up to cond(J) * 1e-12.
+    # Same root: the step between them maps through the oracle's Jacobian
+    # of f (central differences) to no more than the two residuals
+    each = np.max(np.abs(maps_m - maps_s) / np.abs(maps_s), axis=1)
+    far = np.nonzero(ok & (each > 1e-9))[0]
+    print("roots differing by more than 1e-9: %d of %d" % (len(far), ok.sum()), flush=True)
+    worst = 0.0
+    for i in far:
+        a, pp, e, iota = ev[i]
+        Om = ol.o_iekg(pp, e, np.cos(iota), a, freq=True)[13:16]
+        J = np.zeros((3, 3))
+        for k in range(3):
+            hk = 1e-6 * abs(maps_s[i, k])
+            xp, xm = maps_s[i].copy(), maps_s[i].copy()
+            xp[k] += hk
+            xm[k] -= hk
+            J[:, k] = (_sol_fun(xp, Om, e, iota) - _sol_fun(xm, Om, e, iota)) / (2 * hk)
+        worst = max(worst, np.sum(np.abs(J @ (maps_m[i] - maps_s[i]))))
+    print("largest |J dx|_1 over those: %.3e" % worst, flush=True)
+    assert worst <= 1e-11
 
 
 def test_learned_start_in_the_waveform():

@@ -27,7 +27,8 @@ def test_rows_equal_single_calls_on_their_grid():
     P, N, dt, nmax = _workload(24, 61)
     print("case multigrid rows=%d grids=%s" % (len(P), sorted(set(zip(N.tolist(), dt.tolist())))), flush=True)
     mg = MultiGridBatch(precision="mixed", tuning={})
-    inj = P[0].copy()
+    from emri_kludge_suite_amd.multigrid import injection_source
+    inj = injection_source(7)
     data = {}
     for k in sorted(set(zip(N.tolist(), dt.tolist()))):
         data[k] = injected_data(mg, inj, k[0], k[1], seed=7)
