@@ -241,6 +241,23 @@ int aak_p_lso_batch(const double *aei, int n, double *out, int threads);
  * The model must outlive its use by the handle. */
 int aak_set_map_init(aak_handle h, int mode, aak_mlp m);
 
+/* The likelihood's inner product on caller-owned device buffers, with no
+ * device allocation.  No counterpart in the reference: for callers that
+ * cannot allocate (code under HIP graph capture, serving loops).
+ * hI, hII: the two channels' time series (length doubles each); hI_f,
+ * hII_f: the weighted data, length/2+1 complex bins as interleaved
+ * (re, im) doubles; ASDinv1/2: their weights; like_out: 2 doubles,
+ * {4 <h w, d w>, 4 <h w, h w>} as aak_likelihood; stream: a hipStream_t or
+ * NULL.  Limits: length a power of two in [2^8, 2^21] (else -2); scratch is
+ * one static device array of about 80 MiB per GPU, held until the process
+ * ends; the first call at a new length copies that length's FFT tables and
+ * waits for the copy, later calls only enqueue kernels; calls that overlap
+ * on one GPU share the scratch and race.  Series and spectra 16-byte
+ * aligned. */
+int aak_inner_product_noalloc(const double *hI, const double *hII, const double *hI_f, const double *hII_f,
+                              const double *ASDinv1, const double *ASDinv2, int length, double *like_out,
+                              void *stream);
+
 /* Message of the last failed call on this thread. */
 const char *aak_last_error(void);
 
