@@ -21,6 +21,14 @@ class LibraryMissing(RuntimeError):
     pass
 
 
+class SetPar(C.Structure):
+    """aak_setpar: the settings/parameters of AAK_Waveform (KSTools.h:21-51)."""
+    _fields_ = ([("path", C.c_char * 1024)] +
+                [(n, C.c_int) for n in ("backint", "LISA", "traj", "SNR", "timing", "length")] +
+                [(n, C.c_double) for n in ("dt", "p", "T", "f", "T_fit", "mu", "M", "s", "e", "iota", "gamma",
+                                           "psi", "theta_S", "phi_S", "theta_K", "phi_K", "alpha", "D")])
+
+
 def lib():
     """Return the loaded libaak_hip.so; raise if it has not been built."""
     global _lib
@@ -80,6 +88,10 @@ def lib():
         "aak_p_lso": (_d, [_d, _d, _d]),
         "aak_p_lso_batch": (_i, [_pd, _i, _pd, _i]),
         "aak_set_map_init": (_i, [C.c_void_p, _i, C.c_void_p]),
+        "aak_get_inspiral": (_i, [C.c_void_p, _pd, _pd, _pd, _i]),
+        "aak_kerr_constants": (_i, [_d, _d, _d, _d, _pd]),
+        "aak_load_setpar": (_i, [C.c_char_p, C.POINTER(SetPar)]),
+        "aak_elisa_noise": (_d, [_d]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -241,6 +241,10 @@ struct aak_handle_s {
   WaveformPlan single;
   PinnedBuf single_pin;
   int have_traj = 0;  // single holds a whole host stage (run_phase_trajectory, gpu_gen_AAK)
+  // M (solar masses), cos(iota) and plunge index of the last gpu_gen_AAK,
+  // whose dense arrays lane 0 keeps while state != 0
+  double gen_Mphys = 0.0, gen_coslam = 1.0;
+  int gen_ip = -1;
   // stage timing on the compute stream: rates, guard+folds, mode sum, rFFT, reduction
   int profiling = 0;
   int force_fold_replay = 0;  // test hook: every fold takes the serial replay
@@ -1045,6 +1049,9 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   }
   h->state = 1;
   const DenseParams &P = plan.P;
+  h->gen_Mphys = M;
+  h->gen_coslam = P.coslam;
+  h->gen_ip = P.ip;
   int *I = h->info;
   I[0] = plan.traj.steps;
   I[1] = P.nmodes;
@@ -1520,6 +1527,37 @@ int aak_test_inner_product(aak_handle h, const double *hI, const double *hII, do
 int aak_test_force_fold_replay(aak_handle h, int on) {
   if (!h) return fail(-1, "null handle");
   h->force_fold_replay = on ? 1 : 0;
+  return 0;
+}
+
+int aak_get_inspiral(aak_handle h, double *t, double *p, double *e, int cap) {
+  if (!h) return fail(-1, "null handle");
+  if (h->state == 0) return fail(-2, "no waveform generated");
+  if (!t || !p || !e || cap < 1) return fail(-2, "bad arguments");
+  const int n = std::min(h->gen_ip + 1, cap);
+  const size_t L = (size_t)h->N + 1;
+  std::vector<double> d(4 * L), tv(h->N);
+  const Lane &ln = h->lane[0];
+  // e, v, M, S are the first four dense arrays
+  HIPCHK(hipMemcpyAsync(d.data(), ln.d_dense, sizeof(double) * 4 * L, hipMemcpyDeviceToHost, ln.stream));
+  HIPCHK(hipMemcpyAsync(tv.data(), ln.d_t, sizeof(double) * h->N, hipMemcpyDeviceToHost, ln.stream));
+  HIPCHK(hipStreamSynchronize(ln.stream));
+  for (int i = 0; i < n; i++) {
+    const double ee = d[i], v = d[L + i], M = d[2 * L + i], S = d[3 * L + i];
+    t[i] = tv[i];
+    p[i] = (1. - ee * ee) * pow(omega_phi(v, ee, h->gen_coslam, S, M) * h->gen_Mphys * kMsunSec, -2. / 3.);
+    e[i] = ee;
+  }
+  return n;
+}
+
+int aak_kerr_constants(double p, double e, double iota, double a, double *ELQ) {
+  if (!ELQ) return fail(-2, "null output");
+  Geodesic<HostCx> g;
+  if (g.init(p, e, cos(iota), a, 1)) return fail(-5, "geodesic constants failed");
+  ELQ[0] = g.E;
+  ELQ[1] = g.Lz;
+  ELQ[2] = g.Q;
   return 0;
 }
 

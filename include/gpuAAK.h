@@ -241,6 +241,16 @@ int aak_p_lso_batch(const double *aei, int n, double *out, int threads);
  * The model must outlive its use by the handle. */
 int aak_set_map_init(aak_handle h, int mode, aak_mlp m);
 
+/* Inspiral of the last aak_gpu_gen_AAK up to its plunge sample, as the
+ * reference's AAK_Waveform writes it to <path>_traj.dat (AAK.cc:355-366):
+ * t, p = (1 - e^2) (M Omega_phi)^(-2/3) and e per sample, at most cap rows;
+ * returns the row count.  No counterpart as an entry point in the reference. */
+int aak_get_inspiral(aak_handle h, double *t, double *p, double *e, int cap);
+
+/* Constants of motion (E, L_z, Q) of the Kerr geodesic (p, e, iota) about
+ * spin a (IEKG.cc:19-159), as AAK_Waveform.cc:98-100 prints them. */
+int aak_kerr_constants(double p, double e, double iota, double a, double *ELQ);
+
 /* The likelihood's inner product on caller-owned device buffers, with no
  * device allocation.  No counterpart in the reference: for callers that
  * cannot allocate (code under HIP graph capture, serving loops).
@@ -257,6 +267,26 @@ int aak_set_map_init(aak_handle h, int mode, aak_mlp m);
 int aak_inner_product_noalloc(const double *hI, const double *hII, const double *hI_f, const double *hII_f,
                               const double *ASDinv1, const double *ASDinv2, int length, double *like_out,
                               void *stream);
+
+/* Settings and parameters of the waveform generator AAK_Waveform (SetPar,
+ * KSTools.h:21-51); booleans as 0/1, angles in radians, masses in solar
+ * masses, D in Gpc. */
+typedef struct {
+  char path[1024];
+  int backint, LISA, traj, SNR, timing;
+  int length;
+  double dt, p, T, f, T_fit;
+  double mu, M, s, e, iota, gamma, psi;
+  double theta_S, phi_S, theta_K, phi_K, alpha, D;
+} aak_setpar;
+
+/* LoadSetPar(par, filename) (KSTools.cc:26-114): 1 on success, 0 with the
+ * reference's message in aak_last_error() otherwise.  dt < 0 becomes
+ * T * 31536000 / length, p < 0 the p of the initial GW frequency f. */
+int aak_load_setpar(const char *filename, aak_setpar *out);
+
+/* eLISA_Noise(f) (KSTools.cc:116-125): analytic eLISA PSD in 1/Hz. */
+double aak_elisa_noise(double f);
 
 /* Message of the last failed call on this thread. */
 const char *aak_last_error(void);
