@@ -21,6 +21,7 @@
 // Compiled with -ffp-contract=off: every expression that feeds the phases
 // must round exactly like the reference CPU build.
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 
@@ -507,11 +508,178 @@ __device__ void fold_serial_replay(const FoldArgs &A, const FoldWork &W, int p, 
   }
 }
 
+// Rounded increment of a step on the binade grid of exponent k:
+// r = rn(d / u) with u = 2^(k-53), an exact tie (fraction 1/2) rounded down
+// and flagged.  Adding 1.5 * 2^52 rounds f to an integer at once (round to
+// nearest even; |f| < 2^51 keeps the sum in [2^52, 2^53), where the ulp is
+// 1), and the difference of the bit patterns is that integer.  Equal to
+// floor(f) + (frac(f) > 1/2) with the tie test frac(f) == 1/2.
+__device__ inline long long grid_round(double d, int k, bool &tie) {
+  constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+  const double f = ldexp(d, 53 - k);
+  const double t = f + kMagic;
+  long long r = __double_as_longlong(t) - __double_as_longlong(kMagic);
+  const double diff = f - (t - kMagic);  // exact: f - rn(f), in [-1/2, 1/2]
+  tie = (diff == 0.5) || (diff == -0.5);
+  if (diff == -0.5) r -= 1;  // the tie went up: take the floor
+  return r;
+}
+
+// The block is uniform when every step's key is the same binade grid: the
+// approximate values a_j of the block (the classification's own, a_0 = the
+// lane's start, a_(j+1) = a_j + d_j) satisfy, for the smallest and largest
+// magnitude, |a| - E > 0 and frexp(|a|min - E) == frexp(|a|max + E), with one
+// sign; every |a| +- E then lies in that binade (rounding is monotone).  It
+// also needs |d| 2^(53-k) < 2^51 for grid_round.  All increments must be
+// finite.  Returns the binade exponent, or INT_MIN when the block is not
+// uniform; sgn receives the sign.  Both passes decide alike (same inputs).
+__device__ inline int uniform_block(const double *d, double a0, double E, double *sh, int *shi, int &sgn) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  double a = a0, amin = a0, amax = a0, dmax = 0.0, tsum = 0.0;
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) {
+    a = a + d[j];
+    amin = fmin(amin, a);
+    amax = fmax(amax, a);
+    dmax = fmax(dmax, fabs(d[j]));
+    tsum += d[j];
+  }
+  int fin = __builtin_isfinite(tsum) && __builtin_isfinite(a) ? 1 : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    amin = fmin(amin, __shfl_xor(amin, o, 64));
+    amax = fmax(amax, __shfl_xor(amax, o, 64));
+    dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+    fin &= __shfl_xor(fin, o, 64);
+  }
+  constexpr int NW = kFoldThreads / 64;
+  if (lane == 0) {
+    sh[wave] = amin;
+    sh[NW + wave] = amax;
+    sh[2 * NW + wave] = dmax;
+    shi[wave] = fin;
+  }
+  __syncthreads();
+  amin = sh[0];
+  amax = sh[NW];
+  dmax = sh[2 * NW];
+  fin = shi[0];
+#pragma unroll
+  for (int w = 1; w < NW; w++) {
+    amin = fmin(amin, sh[w]);
+    amax = fmax(amax, sh[NW + w]);
+    dmax = fmax(dmax, sh[2 * NW + w]);
+    fin &= shi[w];
+  }
+  __syncthreads();
+  if (!fin) return INT_MIN;
+  double lo_abs, hi_abs;
+  if (amin > 0.0) {
+    lo_abs = amin;
+    hi_abs = amax;
+    sgn = 1;
+  } else if (amax < 0.0) {
+    lo_abs = -amax;
+    hi_abs = -amin;
+    sgn = -1;
+  } else {
+    return INT_MIN;
+  }
+  const double lo = lo_abs - E, hi = hi_abs + E;
+  if (!(lo > 0.0) || !__builtin_isfinite(hi)) return INT_MIN;
+  int klo, khi;
+  frexp(lo, &klo);
+  frexp(hi, &khi);
+  if (klo != khi) return INT_MIN;
+  if (!(ldexp(dmax, 53 - klo) < 2251799813685248.0)) return INT_MIN;  // 2^51
+  return klo;
+}
+
+// A uniform block is one run on the grid of exponent K: its item list is one
+// item (kind 0 from step 0, the block's total R) and its ties; the write pass
+// gives every step m_start + (inclusive prefix of r) + (ties rounded up so
+// far) on that grid.  The same items, ties and values as the general path
+// below, whose keys all equal K here, with a few operations per step.
+template <bool WRITE, int DIR>
+__device__ inline void fold_uniform(const FoldArgs &A, const FoldWork &W, int p, int b, const double *d, int K,
+                                    int sgn, long long *shl, int *shi, double *sd) {
+  const int tid = threadIdx.x;
+  // steps past the end have d = 0: r = 0 and no tie
+  long long lane_R = 0;
+  int tie_mask = 0;
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) {
+    bool tie;
+    lane_R += grid_round(d[j], K, tie);
+    if (tie) tie_mask |= 1 << j;
+  }
+  long long R_tot;
+  const long long carry = block_excl_sum(lane_R, shl, &R_tot);
+  int nties_tot;
+  const int tie_base = block_excl_sum((int)__popc(tie_mask), shi, &nties_tot);
+  FoldItem *items = W.items + ((size_t)p * A.nblk + b) * kFoldMaxItems;
+  FoldTie *ties = W.ties + ((size_t)p * A.nblk + b) * kFoldMaxTies;
+  if (!WRITE) {
+    if (tid == 0) {
+      W.nitems[p * A.nblk + b] = 1;
+      W.nties[p * A.nblk + b] = nties_tot;
+      if (nties_tot > kFoldMaxTies) atomicExch(&W.status[p], 1);
+      items[0].kind = 0;
+      items[0].k = K;
+      items[0].sgn = sgn;
+      items[0].first = 0;
+      items[0].tie_first = 0;
+      items[0].d = d[0];
+      items[0].R = R_tot;
+    }
+    if (nties_tot > kFoldMaxTies || !tie_mask) return;
+    long long P = carry;
+    int tc = tie_base;
+#pragma unroll
+    for (int j = 0; j < kFoldPerThread; j++) {
+      bool tie;
+      P += grid_round(d[j], K, tie);
+      if (tie) {
+        ties[tc].step = tid * kFoldPerThread + j;
+        ties[tc].P = P;
+        ties[tc].up = 0;
+        tc++;
+      }
+    }
+    return;
+  }
+  if (nties_tot > kFoldMaxTies) return;
+  const long long m0 = items[0].m_start;
+  long long ups = 0;
+  for (int q = 0; q < tie_base; q++) ups += ties[q].up;
+  long long P = carry;
+  int tc = tie_base;
+  double outv[kFoldPerThread];
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) {
+    bool tie;
+    P += grid_round(d[j], K, tie);
+    if (tie) ups += ties[tc++].up;
+    outv[j] = ldexp((double)(m0 + P + ups), K - 53);
+  }
+  // coalesced stores through LDS (the staged increments are consumed)
+#pragma unroll
+  for (int j = 0; j < kFoldPerThread; j++) sd[fold_lds_index(tid * kFoldPerThread + j)] = outv[j];
+  __syncthreads();
+  double *op = A.out[p] + (A.i0 + DIR * (b * kFoldBlock + tid + 1));
+#pragma unroll
+  for (int q = 0; q < kFoldPerThread; q++) {
+    const int s = b * kFoldBlock + tid + q * kFoldThreads;
+    if (s < A.nsteps) op[DIR * q * kFoldThreads] = sd[fold_lds_index(tid + q * kFoldThreads)];
+  }
+}
+
 // Classify steps, build the block's item list (PLAN) or write the exact
 // fold values of run steps (WRITE, after the chain resolved the run starts).
 template <bool WRITE, int DIR>
 __global__ __launch_bounds__(kFoldThreads, WRITE ? 3 : 4) void k_fold_local(FoldArgs A, FoldWork W) {
   __shared__ double shd[kFoldThreads / 64];
+  __shared__ double shu[3 * (kFoldThreads / 64)];
   __shared__ long long shl[2 * (kFoldThreads / 64)];
   __shared__ int shi[kFoldThreads];
   __shared__ int shf[kFoldThreads / 64];
@@ -535,6 +703,14 @@ __global__ __launch_bounds__(kFoldThreads, WRITE ? 3 : 4) void k_fold_local(Fold
   }
   double dummy;
   double a0 = W.xstart[p * A.nblk + b] + block_excl_sum(tsum, shd, &dummy);
+  {
+    int sgn = 0;
+    const int K = uniform_block(d, a0, E, shu, shi, sgn);
+    if (K != INT_MIN) {
+      fold_uniform<WRITE, DIR>(A, W, p, b, d, K, sgn, shl, shi, sd);
+      return;
+    }
+  }
   // keys: >= 0 binade grid, -1 exact replay, -2 beyond the end
   int key[kFoldPerThread];
   long long r[kFoldPerThread];
@@ -1614,18 +1790,23 @@ __device__ inline void modesum_block(const DenseParams &P, const DenseBuffers &B
   B.h2[i] = sc * accII;
 }
 
-// Persistent and XCD-aware: the grid holds what the chip runs at once, a
-// multiple of the 8 XCDs (workgroups are dealt to the XCDs round-robin, so
-// workgroup g runs on XCD g % 8).  Each XCD walks a contiguous range of whole
-// expansion blocks, so an expansion block's Taylor coefficients are fetched
-// into one L2 only, and its workgroups stride through that range.
+// One workgroup per 256-sample block, XCD-aware: workgroups are dispatched in
+// index order and dealt to the 8 XCDs round-robin (workgroup g runs on XCD
+// g % 8), so g is mapped to the block that puts the 2^xs sample blocks of one
+// expansion block on one XCD (their Taylor coefficients are fetched into one
+// L2) while the blocks in flight at any moment stay one contiguous stretch of
+// the waveform, spread over every XCD (no XCD idles past a plunge).  Blocks
+// beyond the last whole group of 8 << xs keep their index.
 template <bool MIXED>
 __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuffers B, int nb) {
-  constexpr int kX = 8;
-  const int xs = exp_shift(P), xcd = blockIdx.x % kX, slot = blockIdx.x / kX, nslots = gridDim.x / kX;
-  const long long ne = (nb + (1 << xs) - 1) >> xs;
-  const int lo = (int)(ne * xcd / kX) << xs, hi = min(nb, (int)(ne * (xcd + 1) / kX) << xs);
-  for (int blk = lo + slot; blk < hi; blk += nslots) modesum_block<MIXED>(P, B, blk);
+  const int xs = exp_shift(P), g = blockIdx.x;
+  const int full = nb / (8 << xs) * (8 << xs);
+  int blk = g;
+  if (g < full) {
+    const int x = g & 7, r = g >> 3;
+    blk = ((((r >> xs) << 3) | x) << xs) | (r & ((1 << xs) - 1));
+  }
+  modesum_block<MIXED>(P, B, blk);
 }
 
 // Per-sample mode sum for the blocks listed by k_modesum_setup (B.ms_p < 0):
@@ -1693,32 +1874,14 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum_direct(DenseParams P, De
 }
 
 // Workgroups of k_modesum the device runs at once (CUs x occupancy), per precision.
-static int modesum_resident(bool mixed) {
-  static int cached[2] = {0, 0};
-  static std::once_flag once[2];
-  std::call_once(once[mixed ? 1 : 0], [&] {
-    int dev = 0, cus = 0, occ = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (mixed)
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_modesum<true>, kMsThreads, 0);
-    else
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_modesum<false>, kMsThreads, 0);
-    cached[mixed ? 1 : 0] = std::max(8, (cus > 0 ? cus : 256) * (occ > 0 ? occ : 2));
-  });
-  return cached[mixed ? 1 : 0];
-}
-
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
   const int nb = modesum_blocks(P), ne = expansion_blocks(P);
-  // a multiple of the 8 XCDs, at most the resident workgroups, at least one per block
-  const int grid = (std::min(nb, modesum_resident(P.mixed != 0)) + 7) / 8 * 8;
   if (P.mixed) {
     hipLaunchKernelGGL(k_modesum_setup<true>, dim3(ne), dim3(kSetupThreads), 0, s, P, B);
-    hipLaunchKernelGGL(k_modesum<true>, dim3(grid), dim3(kMsThreads), 0, s, P, B, nb);
+    hipLaunchKernelGGL(k_modesum<true>, dim3(nb), dim3(kMsThreads), 0, s, P, B, nb);
   } else {
     hipLaunchKernelGGL(k_modesum_setup<false>, dim3(ne), dim3(kSetupThreads), 0, s, P, B);
-    hipLaunchKernelGGL(k_modesum<false>, dim3(grid), dim3(kMsThreads), 0, s, P, B, nb);
+    hipLaunchKernelGGL(k_modesum<false>, dim3(nb), dim3(kMsThreads), 0, s, P, B, nb);
   }
   hipLaunchKernelGGL(k_modesum_direct, dim3(kDirectGrid), dim3(kMsThreads), 0, s, P, B);
 }
