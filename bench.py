@@ -162,6 +162,19 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_each_way": 2 ** 30, "launches": 5}))
 
 
+def flop_probe():
+    """In-place multiplies of 2^27 float32, then float64 values, five times each
+    (one FLOP per element and launch): the reference point that scales the
+    SQ_INSTS_VALU_FLOPS_FP32/FP64 counters to FLOP."""
+    import torch
+    for dt in (torch.float32, torch.float64):
+        x = torch.ones(2 ** 27, dtype=dt, device="cuda")
+        for _ in range(5):
+            x.mul_(1.0000001)
+    torch.cuda.synchronize()
+    print(json.dumps({"flop_probe_elements": 2 ** 27, "launches_per_dtype": 5}))
+
+
 def read_profile_value(name, key):
     """A figure from a committed profile summary (a line "key value"); None
     when the summary or the line is absent."""
@@ -181,7 +194,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10, help="timed batches")
     ap.add_argument("--warmup", type=int, default=3, help="untimed batches (also brings the GPU clocks up)")
     ap.add_argument("--batch", type=int, default=256, help="templates per step")
-    ap.add_argument("--lanes", type=int, default=2, help="concurrent streams of the batched path")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="concurrent streams of the batched path (0: the autotuner's table for this grid)")
     ap.add_argument("--single", type=int, default=8, help="one-waveform-per-call likelihoods timed beside")
     ap.add_argument("--mismatch", type=int, default=0,
                     help="templates whose mixed/FP64 mismatch is measured (0: every timed template)")
@@ -192,9 +206,14 @@ def main():
     ap.add_argument("--uneven-batch", type=int, default=1024)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
+    ap.add_argument("--flop-probe", action="store_true",
+                    help="only run in-place float32/float64 multiplies of 2^27 values (FLOP counter calibration)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.flop_probe:
+        flop_probe()
         return
     if args.gpus != 1:
         print("bench.py: measures one GPU; --gpus %d is not supported "
@@ -203,7 +222,9 @@ def main():
 
     from emri_kludge_suite_amd import GPUAAK
 
-    LANES = args.lanes
+    from emri_kludge_suite_amd import autotune
+    tuning = autotune.load()
+    settings = {}
 
     injection = valid_draws(2, 1)[0][0]
     ntpl = max(args.batch, args.single)
@@ -220,6 +241,19 @@ def main():
     P = np.array(templates[:args.batch])
     flops_batch = modesum_flops(meta[:args.batch])
 
+    def configure(mode):
+        """The pipeline settings of this grid and precision: the autotuner's
+        table (data/tuning.json), --lanes overriding its lane count."""
+        g.set_precision(mode)
+        s = autotune.lookup(tuning, N_SAMPLES, DT, mode) or {}
+        s = {k: s.get(k, autotune.DEFAULTS[k]) for k in autotune.DEFAULTS}
+        if args.lanes:
+            s["lanes"] = args.lanes
+        g.set_lanes(s["lanes"])
+        g.set_tuning(feed_depth=s["feed_depth"], exp_shift=-2 if s["exp_shift"] is None else s["exp_shift"])
+        settings[mode] = s
+        return s
+
     def profiled_pass(lanes):
         """One batch with HIP events around each stage on the stream of each
         template; returns the stage sums (ms) over the batch."""
@@ -228,14 +262,12 @@ def main():
         g.LikelihoodBatch(P)
         tm = g.batch_timings()
         g.set_profiling(False)
-        g.set_lanes(LANES)
         return np.array([tm[x] for x in ("rates", "fold", "modesum", "fft", "dot")])
 
     host_plan = {}
 
     def run_batched(mode):
-        g.set_precision(mode)
-        g.set_lanes(LANES)
+        lanes = configure(mode)["lanes"]
         for _ in range(args.warmup):
             g.LikelihoodBatch(P)
         t0 = time.perf_counter()
@@ -252,7 +284,7 @@ def main():
                            "pool_threads": int(g.pool_stats()["threads"]), "pool_busy_frac": float(np.mean(busy))}
         # stage times: on one lane (no overlap with another template's kernels)
         # and as they run concurrently on the bench's lanes
-        return elapsed, profiled_pass(1), profiled_pass(LANES), like
+        return elapsed, profiled_pass(1), profiled_pass(lanes), like
 
     # batched paths: host stages on the thread pool, device work pipelined
     el_mx, st_mx, st_mx_lanes, like_mx = run_batched("mixed")
@@ -310,8 +342,24 @@ def main():
 
     ms_mx, ach_mx = roof(st_mx)
     ms_64, ach_64 = roof(st_64)
-    traffic = read_profile_value("aak_likelihood_N2p21_n50_hbm_counters.md", "traffic_bytes_per_launch modesum_stage_mixed")
-    valu_frac = read_profile_value("aak_likelihood_N2p21_n50_sq_counters.md", "valu_issue_frac modesum_stage_mixed")
+    HBM_PROF, SQ_PROF, FLOP_PROF = ("aak_likelihood_N2p21_n50_hbm_counters.md", "aak_likelihood_N2p21_n50_sq_counters.md",
+                                    "aak_likelihood_N2p21_n50_flops.md")
+    traffic = read_profile_value(HBM_PROF, "traffic_bytes_per_launch modesum_stage_mixed")
+    valu_frac = read_profile_value(SQ_PROF, "valu_issue_frac modesum_stage_mixed")
+    # executed work: FP32 + FP64 VALU FLOP per launch of the mixed stage from
+    # the SQ_INSTS_VALU_FLOPS_* counters (calibrated, in the committed
+    # summary), over the stage time measured here
+    exec_flop = read_profile_value(FLOP_PROF, "valu_flop_per_launch modesum_stage_mixed")
+    exec_tf = exec_flop / (ms_mx * 1e-3) / 1e12 if exec_flop else None
+    # HBM rooflines of the fold stage and k_rates, algorithmic bytes per
+    # active sample: the fold reads a rate and writes a phase per step and
+    # phase (3 x 16 B); k_rates writes e, three rates and the amplitude (40 B)
+    n_active = sum(n for n, _ in meta[:len(P)]) / len(P)
+
+    def hbm_roof(nbytes, ms, key):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
+                "bytes_per_launch": nbytes, "ms": ms, "traffic": read_profile_value(HBM_PROF, key)}
     # noise-weighted inner product (fused four-step FFT + PSD weighting + both
     # sums), HBM-bound; algorithmic bytes per channel: 8N (time series) +
     # 24(N/2+1) (weighted data and weights), two channels.  The four-step
@@ -334,7 +382,7 @@ def main():
         "data": "synthetic (seeded prior draws, injection seed 2, templates seed 3)",
         "config": {"workload": "1-yr LISA AAK likelihoods, N=2^21, dt=15 s, n=1..50, %d templates per step "
                                "(batched), mixed precision" % len(P),
-                   "N": N_SAMPLES, "dt": DT, "n_max": N_MAX, "batch": len(P), "lanes": LANES,
+                   "N": N_SAMPLES, "dt": DT, "n_max": N_MAX, "batch": len(P), "pipeline": settings,
                    "psd": "LISA_Noise(dur=1)"},
         "host_stage": {"mixed": host_plan.get("mixed"), "fp64": host_plan.get("fp64"),
                        "note": "host-stage time per template on the pool during the timed batches"},
@@ -345,7 +393,7 @@ def main():
         "stages_ms_per_template_overlapped": dict(zip(stages, [round(x / per_pass, 4) for x in st_mx_lanes])),
         "plain_fp64_batched": {"value": value_64, "unit": "likelihoods/s", "ms_per_step": 1e3 * el_64 / args.steps,
                                "stages_ms_per_template": dict(zip(stages, [round(x / per_pass, 4) for x in st_64])),
-                               "k_modesum_tflops": ach_64, "k_modesum_ms": ms_64},
+                               "k_modesum_algorithmic_tflops": ach_64, "k_modesum_ms": ms_64},
         "single_call_fp64": {"value": value_single, "unit": "likelihoods/s",
                              "ms_per_likelihood": 1e3 * single_elapsed / args.single,
                              "stages_ms": dict(zip(("host",) + stages,
@@ -354,14 +402,25 @@ def main():
                                  "ms_per_waveform": 1e3 * wave_elapsed / args.single},
         "speedup": {"mixed_batched_over_single_fp64": value / value_single,
                     "fp64_batched_over_single_fp64": value_64 / value_single},
-        "roofline": {"bound": "mfma", "achieved": ach_mx, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach_mx / FP32_PEAK_TFLOPS, "traffic": traffic,
+        "roofline": {"bound": "valu", "achieved": exec_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": exec_tf / FP32_PEAK_TFLOPS if exec_tf else None, "traffic": traffic,
                      "kernel": "mode-sum stage, mixed precision (k_modesum_setup + k_modesum + k_modesum_direct)",
                      "kernel_ms": ms_mx,
                      "kernel_timing": "HIP events around the stage on its stream; one profiled pass over the batch on one lane",
-                     "convention": "algorithmic N_active*(190*n_modes+460) FLOP per launch (the reference's per-sample "
-                                   "work, transcendentals at 20 FLOP); peak = published FP32; the stage is VALU-bound",
-                     "valu_issue_frac": valu_frac},
+                     "flop_per_launch": exec_flop,
+                     "convention": "executed FP32 + FP64 VALU FLOP per launch (SQ_INSTS_VALU_FLOPS_FP32/FP64, calibrated; "
+                                   "profiles/%s) over the measured stage time; peak = published FP32 vector rate "
+                                   "(packed FP32 is not emitted in this build, so its reachable peak is half)" % FLOP_PROF,
+                     "valu_issue_frac": valu_frac,
+                     "algorithmic_tflops": ach_mx,
+                     "algorithmic_convention": "N_active*(190*n_modes+460) FLOP per launch: the reference's per-sample "
+                                               "work (transcendentals at 20 FLOP), most of which the Taylor form skips; "
+                                               "an effective rate, not a fraction of any peak"},
+        "fold": dict(hbm_roof(48.0 * n_active, st_mx[1] / per_pass, "traffic_bytes_per_launch fold_stage"),
+                     kernels="guard + k_fold_sums + k_fold_plan + k_fold_local x2 + k_fold_chain",
+                     bytes_convention="48 B per active sample (three phases: rate read, phase written)"),
+        "rates": dict(hbm_roof(40.0 * n_active, st_mx[0] / per_pass, "traffic_bytes_per_launch k_rates"),
+                      kernels="k_rates", bytes_convention="40 B per active sample (e, three rates, amplitude written)"),
         "inner_product": {"bound": "hbm", "achieved": ip_bytes / (ip_ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                           "frac": ip_bytes / (ip_ms * 1e-3) / 8e12, "bytes_per_launch": ip_bytes, "ms": ip_ms,
                           "kernels": "k_fft_cols (column FFTs x twiddles) + k_fft_rows (row FFTs, real-spectrum split, "
@@ -422,7 +481,7 @@ def uneven_leg(model, n):
     grouped, each with its FFT length and tuned pipeline."""
     from emri_kludge_suite_amd import multigrid as MG
     P, N, dt, nmax, T = MG.config5_workload(n, 6)
-    inj = MG.config5_workload(1, 7)[0][0]
+    inj = MG.injection_source(7)
     mg = MG.MultiGridBatch(precision="mixed", map_model=model)
     for k in sorted(set(zip(N.tolist(), dt.tolist()))):
         MG.injected_data(mg, inj, k[0], k[1], seed=7)

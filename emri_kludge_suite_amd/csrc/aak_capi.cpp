@@ -187,7 +187,8 @@ struct Lane {
   double *d_dense = nullptr;  // e, v, M, S, rate[3], phase[3], amp: 11 x (N+1), then nu0
   double *d_h = nullptr;      // hI | hII, N+2 samples each at channel stride h_stride(N)
   double *d_t = nullptr;      // [N+2]
-  int *d_flags = nullptr;     // [4]: non-positive Phidot count, fold fallbacks
+  int *d_flags = nullptr;     // [8]: non-positive Phidot count, fold fallbacks, ... (DenseBuffers::flags)
+  bool flags_dirty = false;   // a waveform wrote d_flags and no final reduction has zeroed them since
   int *d_guard = nullptr;
   FoldWork fw{};
   int fold_nblk_cap = 0;
@@ -198,7 +199,9 @@ struct Lane {
   double *d_ms_coef = nullptr;  // mode-sum block Taylor coefficients (grown with nmodes)
   size_t ms_coef_cap = 0;
   int *d_ms_p = nullptr;
+  int *d_ms_slot = nullptr;
   int *d_ms_list = nullptr;
+  int *d_ms_split = nullptr;
   double *d_ms_eref = nullptr;
   rocfft_plan plan = nullptr;
   rocfft_execution_info exec = nullptr;
@@ -236,7 +239,7 @@ struct aak_handle_s {
   PinnedBuf like_host;
   int have_data = 0;
   int state = 0;  // 0 empty, 1 time domain, 2 weighted spectrum (lane 0)
-  int info[10] = {0};
+  int info[12] = {0};
   // single-waveform staging
   WaveformPlan single;
   PinnedBuf single_pin;
@@ -299,7 +302,9 @@ struct aak_handle_s {
     B.guard_blk = ln.d_guard;
     B.ms_coef = ln.d_ms_coef;
     B.ms_p = ln.d_ms_p;
+    B.ms_slot = ln.d_ms_slot;
     B.ms_list = ln.d_ms_list;
+    B.ms_split = ln.d_ms_split;
     B.ms_eref = ln.d_ms_eref;
     B.bessel_c = d_bessel;
     B.bessel_lgf = d_bessel + (size_t)(kBesselKMax + 1) * kBesselMMax;
@@ -318,12 +323,14 @@ static int alloc_lane(aak_handle h, Lane &L) {
   HIPCHK(hipMalloc(&L.d_dense, sizeof(double) * (11 * ((size_t)N + 1) + 1)));
   HIPCHK(hipMalloc(&L.d_h, sizeof(double) * 2 * h_stride(N)));
   HIPCHK(hipMalloc(&L.d_t, sizeof(double) * ((size_t)N + 2)));
-  HIPCHK(hipMalloc(&L.d_flags, sizeof(int) * 4));
-  HIPCHK(hipMemset(L.d_flags, 0, sizeof(int) * 4));
+  HIPCHK(hipMalloc(&L.d_flags, sizeof(int) * 8));
+  HIPCHK(hipMemset(L.d_flags, 0, sizeof(int) * 8));
   HIPCHK(hipMalloc(&L.d_guard, sizeof(int) * ((size_t)N / 4096 + 4)));
   HIPCHK(hipMalloc(&L.d_ms_p, sizeof(int) * ((size_t)N / 256 + 4)));
+  HIPCHK(hipMalloc(&L.d_ms_slot, sizeof(int) * ((size_t)N / 256 + 4)));
   HIPCHK(hipMalloc(&L.d_ms_list, sizeof(int) * ((size_t)N / 256 + 4)));
-  HIPCHK(hipMalloc(&L.d_ms_eref, sizeof(double) * ((size_t)N / 256 + 4)));
+  HIPCHK(hipMalloc(&L.d_ms_split, sizeof(int) * ((size_t)N / 256 + 4)));
+  HIPCHK(hipMalloc(&L.d_ms_eref, sizeof(double) * 2 * ((size_t)N / 256 + 4)));
   const int nblk = (N + kFoldBlock - 1) / kFoldBlock + 1;
   L.fold_nblk_cap = nblk;
   size_t bytes = 0;
@@ -397,8 +404,8 @@ static void free_lane(Lane &L) {
   if (L.plan) rocfft_plan_destroy(L.plan);
   if (L.exec) rocfft_execution_info_destroy(L.exec);
   void *ptrs[] = {L.d_table[0], L.d_table[1], L.d_dense, L.d_h, L.d_t, L.d_flags, L.d_guard, L.d_foldmem,
-                  L.d_hf, L.d_fy, L.d_partials, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_list,
-                  L.d_ms_eref};
+                  L.d_hf, L.d_fy, L.d_partials, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_slot,
+                  L.d_ms_list, L.d_ms_split, L.d_ms_eref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; k++) {
@@ -488,8 +495,9 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
     L.rates_recorded[slot] = false;
   }
   if (!Pover) {
-    // block Taylor coefficients: blocks x nmodes x 3 x (kTaylorMax+1) doubles
-    const size_t nc = (size_t)modesum_blocks(P) * (size_t)(P.nmodes > 0 ? P.nmodes : 1) * 3 * (kTaylorMax + 1);
+    // block Taylor coefficients: one slot per expansion block and one per
+    // 256-sample block (split expansions), nmodes x 3 x (kTaylorMax+1) doubles each
+    const size_t nc = (size_t)modesum_slots(P) * (size_t)(P.nmodes > 0 ? P.nmodes : 1) * 3 * (kTaylorMax + 1);
     if (nc > L.ms_coef_cap) {
       HIPCHK(hipStreamSynchronize(L.stream));
       if (L.d_ms_coef) HIPCHK(hipFree(L.d_ms_coef));
@@ -504,7 +512,8 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
   HIPCHK(hipEventRecord(L.ev_upload[slot], h->cstream));
   if (copied) HIPCHK(hipEventRecord(copied, h->cstream));
   const DenseBuffers B = h->bufs(L, slot);
-  launch_clear_flags(B, L.stream);
+  if (L.flags_dirty) launch_clear_flags(B, L.stream);
+  L.flags_dirty = true;  // the waveform's kernels write them
   HIPCHK(hipStreamWaitEvent(L.stream, L.ev_upload[slot], 0));
   h->mark(L, 0);
   launch_rates(P, B, L.stream);
@@ -535,16 +544,15 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
 
 // flags: the lane's waveform flags when the series is a generated waveform (a
 // waveform rejected on the device yields NaN), nullptr for a given series.
-// The batch's likelihood stage (both FFT passes and the final reduction into
-// the lane's own result slot) captured once per lane as a HIP graph: every
-// pointer it uses belongs to the lane or the handle, so one graph serves
-// every template of the lane.  It holds kernels only (no memset nodes).
+// The batch's two FFT passes captured once per lane as a HIP graph: every
+// pointer they use belongs to the lane or the handle, so one graph serves
+// every template of the lane.  It holds kernels only (no memset nodes).  The
+// final reduction is launched after it with the template's own output row.
 static int lane_like_graph(aak_handle h, Lane &L) {
   if (L.gx_like) return 0;
   HIPCHK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
   launch_fft_pass1(h->fft, L.d_h, L.d_h + h_stride(h->N), L.d_fy, L.stream);
   launch_fft_pass2(h->fft, L.d_fy, L.d_partials, nullptr, L.stream);
-  launch_dot_final(L.d_partials, fft_partials(h->fft), L.d_like, L.d_flags, L.stream);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(L.stream, &g);
   if (e != hipSuccess) {
@@ -556,22 +564,28 @@ static int lane_like_graph(aak_handle h, Lane &L) {
   return 0;
 }
 
+// The final reduction zeroes the lane's flags once read when they are the
+// flags of the waveform in hand, so the lane's next waveform needs no
+// clearing launch (Lane::flags_dirty).
 static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_spectrum, const int *flags) {
-  if (h->fused_fft && h->use_graphs && !keep_spectrum && !h->profiling && flags == L.d_flags) {
+  int *clear = (flags == L.d_flags) ? L.d_flags : nullptr;
+  if (h->fused_fft && h->use_graphs && !keep_spectrum && !h->profiling && clear) {
     int rc = lane_like_graph(h, L);
     if (rc) return rc;
     HIPCHK(hipGraphLaunch(L.gx_like, L.stream));
-    if (out_dev != L.d_like)
-      HIPCHK(hipMemcpyAsync(out_dev, L.d_like, sizeof(double) * 2, hipMemcpyDeviceToDevice, L.stream));
+    launch_dot_final(L.d_partials, fft_partials(h->fft), out_dev, flags, clear, L.stream);
+    HIPCHK(hipGetLastError());
+    L.flags_dirty = false;
     return 0;
   }
   if (h->fused_fft) {
     launch_fft_pass1(h->fft, L.d_h, L.d_h + h_stride(h->N), L.d_fy, L.stream);
     h->mark(L, 4);
     launch_fft_pass2(h->fft, L.d_fy, L.d_partials, keep_spectrum ? L.d_hf : nullptr, L.stream);
-    launch_dot_final(L.d_partials, fft_partials(h->fft), out_dev, flags, L.stream);
+    launch_dot_final(L.d_partials, fft_partials(h->fft), out_dev, flags, clear, L.stream);
     h->mark(L, 5);
     HIPCHK(hipGetLastError());
+    if (clear) L.flags_dirty = false;
     return 0;
   }
   void *in[1] = {L.d_h};
@@ -579,9 +593,10 @@ static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_
   if (rocfft_execute(L.plan, in, outb, L.exec) != rocfft_status_success) return fail(-101, "rocfft execute");
   h->mark(L, 4);
   launch_weighted_dot(L.d_hf, h->d_data, h->d_w, h->nbins, keep_spectrum ? L.d_hf : nullptr, L.d_partials, out_dev,
-                      flags, L.stream);
+                      flags, clear, L.stream);
   h->mark(L, 5);
   HIPCHK(hipGetLastError());
+  if (clear) L.flags_dirty = false;
   return 0;
 }
 
@@ -1039,8 +1054,8 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   Lane &L = h->lane[0];
   rc = enqueue_waveform(h, L, plan.P, plan.tab, plan.tab_n, 0, nullptr);
   if (rc) return rc;
-  int flags[4] = {0, 0, 0, 0};
-  HIPCHK(hipMemcpyAsync(flags, L.d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, L.stream));
+  int flags[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(flags, L.d_flags, sizeof(int) * 8, hipMemcpyDeviceToHost, L.stream));
   HIPCHK(hipStreamSynchronize(L.stream));
   read_stage_times(h, 0, 3);
   if (flags[3]) {
@@ -1063,6 +1078,8 @@ int aak_gpu_gen_AAK(aak_handle h, double iota, double s, double p, double e, dou
   I[7] = flags[0];
   I[8] = plan.knots.j0;
   I[9] = plan.knots.plunged;
+  I[10] = flags[2];
+  I[11] = flags[4];
   return 0;
 }
 
@@ -1376,7 +1393,7 @@ int aak_get_pool_stats(aak_handle h, double *out, int n) {
 
 int aak_get_info(aak_handle h, int *info, int n) {
   if (!h) return fail(-1, "null handle");
-  const int m = n < 10 ? n : 10;
+  const int m = n < 12 ? n : 12;
   for (int i = 0; i < m; i++) info[i] = h->info[i];
   return m;
 }

@@ -173,13 +173,16 @@ struct DenseBuffers {
   double *nu0;                     // [1] radial frequency at sample 0 from the PN maps
   double *h, *h2;                  // time domain hI, hII: N+2 samples each, h2 = h + h_stride(N)
   double *t;                       // [N+2]
-  int *flags;                      // [4]: [0] non-positive Phidot count, [1] fold replays, [2] per-sample mode-sum
-                                   // blocks, [3] non-positive Phidot at t = 0 (waveform rejected)
+  int *flags;                      // [8]: [0] non-positive Phidot count, [1] fold replays, [2] per-sample mode-sum
+                                   // blocks, [3] non-positive Phidot at t = 0 (waveform rejected), [4] expansion
+                                   // blocks split per 256 samples, [5..7] unused
   int *guard_blk;                  // [guard blocks]: last positive-Phidot index per block
-  double *ms_coef;                 // per expansion block nmodes x 3 x (kTaylorMax+1) doubles; order p uses [nmodes][3][p+1] (float in mixed mode)
-  int *ms_p;                       // [expansion blocks] Taylor order (-1: per-sample path)
-  int *ms_list;                    // [expansion blocks] blocks of the per-sample path (count in flags[2])
-  double *ms_eref;                 // [expansion blocks] expansion point
+  double *ms_coef;                 // per slot nmodes x 3 x (kTaylorMax+1) doubles; order p uses [nmodes][3][p+1] (float in mixed mode); slots: expansion blocks, then 256-sample blocks of split expansions
+  int *ms_p;                       // [256-sample blocks] Taylor order (-1: per-sample path)
+  int *ms_slot;                    // [256-sample blocks] coefficient slot of the block's expansion
+  int *ms_list;                    // [256-sample blocks] blocks of the per-sample path (count in flags[2])
+  int *ms_split;                   // [expansion blocks] blocks expanded again per 256 samples (count in flags[4])
+  double *ms_eref;                 // [slots] expansion point
   const double *bessel_c;          // series coefficients c_m(k)
   const double *bessel_lgf;        // log(k!)
 };
@@ -192,14 +195,15 @@ void launch_fold_fill(const DenseParams &P, const DenseBuffers &B, hipStream_t s
 void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 int guard_blocks(const DenseParams &P);
 int modesum_blocks(const DenseParams &P);  // mode-sum blocks (256 samples each)
+int modesum_slots(const DenseParams &P);   // coefficient slots: expansion blocks + 256-sample blocks
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 void launch_clear_flags(const DenseBuffers &B, hipStream_t s);
 void launch_phase_out(const DenseParams &P, const DenseBuffers &B, double *out, hipStream_t s);
 // flags (optional): the waveform's counters; flags[3] != 0 (rejected on the
 // device) makes both outputs NaN.
 void launch_weighted_dot(const double2 *hf, const double2 *data, const double *w, int nbins, double2 *hw_out,
-                         double *partials, double *out, const int *flags, hipStream_t s);
+                         double *partials, double *out, const int *flags, int *clear, hipStream_t s);
 // 4 x the sums of nparts (dh, hh) pairs into out[0..1].
-void launch_dot_final(const double *partials, int nparts, double *out, const int *flags, hipStream_t s);
+void launch_dot_final(const double *partials, int nparts, double *out, const int *flags, int *clear, hipStream_t s);
 
 }  // namespace aak

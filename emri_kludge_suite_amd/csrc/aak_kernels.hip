@@ -21,7 +21,6 @@
 // Compiled with -ffp-contract=off: every expression that feeds the phases
 // must round exactly like the reference CPU build.
 #include <hip/hip_runtime.h>
-#include <climits>
 #include <cstdint>
 #include <cstdio>
 
@@ -186,68 +185,78 @@ constexpr int kGuardBlock = kGuardThreads * kGuardPer;
 
 __device__ inline bool phidot_ok(double r) { return !(r <= 0.); }
 
-__global__ __launch_bounds__(kGuardThreads) void k_guard_last(DenseParams P, DenseBuffers B) {
+// Both passes walk the guard blocks with a fixed grid of kGuardGrid
+// workgroups, so that the common case (nothing to fix) costs a small launch.
+constexpr int kGuardGrid = 64;
+
+__global__ __launch_bounds__(kGuardThreads) void k_guard_last(DenseParams P, DenseBuffers B, int nb) {
   if (B.flags[0] == 0) return;
   __shared__ int s_m[kGuardThreads];
   const double *r = B.rate[1];
-  const int base = P.i0 + 1 + blockIdx.x * kGuardBlock + threadIdx.x * kGuardPer;
-  int last = -1;
-  for (int q = 0; q < kGuardPer; q++) {
-    const int i = base + q;
-    if (i <= P.guard_last && phidot_ok(r[i])) last = i;
-  }
-  s_m[threadIdx.x] = last;
-  __syncthreads();
-  for (int o = kGuardThreads / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
+  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+    const int base = P.i0 + 1 + blk * kGuardBlock + threadIdx.x * kGuardPer;
+    int last = -1;
+    for (int q = 0; q < kGuardPer; q++) {
+      const int i = base + q;
+      if (i <= P.guard_last && phidot_ok(r[i])) last = i;
+    }
+    s_m[threadIdx.x] = last;
+    __syncthreads();
+    for (int o = kGuardThreads / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) B.guard_blk[blk] = s_m[0];
     __syncthreads();
   }
-  if (threadIdx.x == 0) B.guard_blk[blockIdx.x] = s_m[0];
 }
 
-__global__ __launch_bounds__(kGuardThreads) void k_guard_fix(DenseParams P, DenseBuffers B) {
+__global__ __launch_bounds__(kGuardThreads) void k_guard_fix(DenseParams P, DenseBuffers B, int nb) {
   if (B.flags[0] == 0) return;
   __shared__ int s_m[kGuardThreads];
   __shared__ int s_carry;
   double *r = B.rate[1];
-  // last good index before this block
-  int c = P.i0;
-  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kGuardThreads) c = max(c, B.guard_blk[b]);
-  s_m[threadIdx.x] = c;
-  __syncthreads();
-  for (int o = kGuardThreads / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
+  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+    // last good index before this block
+    int c = P.i0;
+    for (int b = threadIdx.x; b < blk; b += kGuardThreads) c = max(c, B.guard_blk[b]);
+    s_m[threadIdx.x] = c;
     __syncthreads();
-  }
-  if (threadIdx.x == 0) s_carry = s_m[0];
-  __syncthreads();
-  const int carry = s_carry;
-  __syncthreads();
-  const int base = P.i0 + 1 + blockIdx.x * kGuardBlock + threadIdx.x * kGuardPer;
-  double v[kGuardPer];
-  int last = -1;
-  for (int q = 0; q < kGuardPer; q++) {
-    const int i = base + q;
-    v[q] = (i <= P.guard_last) ? r[i] : 1.0;
-    if (i <= P.guard_last && phidot_ok(v[q])) last = i;
-  }
-  // inclusive max-scan of the per-thread last good index
-  s_m[threadIdx.x] = last;
-  __syncthreads();
-  for (int o = 1; o < kGuardThreads; o <<= 1) {
-    const int add = (threadIdx.x >= o) ? s_m[threadIdx.x - o] : -1;
+    for (int o = kGuardThreads / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) s_carry = s_m[0];
     __syncthreads();
-    s_m[threadIdx.x] = max(s_m[threadIdx.x], add);
+    const int carry = s_carry;
     __syncthreads();
-  }
-  int src = max(carry, threadIdx.x > 0 ? s_m[threadIdx.x - 1] : -1);
-  for (int q = 0; q < kGuardPer; q++) {
-    const int i = base + q;
-    if (i > P.guard_last) break;
-    if (phidot_ok(v[q])) {
-      src = i;
-    } else {
-      r[i] = r[src];
+    const int base = P.i0 + 1 + blk * kGuardBlock + threadIdx.x * kGuardPer;
+    double v[kGuardPer];
+    int last = -1;
+    for (int q = 0; q < kGuardPer; q++) {
+      const int i = base + q;
+      v[q] = (i <= P.guard_last) ? r[i] : 1.0;
+      if (i <= P.guard_last && phidot_ok(v[q])) last = i;
+    }
+    // inclusive max-scan of the per-thread last good index
+    s_m[threadIdx.x] = last;
+    __syncthreads();
+    for (int o = 1; o < kGuardThreads; o <<= 1) {
+      const int add = (threadIdx.x >= o) ? s_m[threadIdx.x - o] : -1;
+      __syncthreads();
+      s_m[threadIdx.x] = max(s_m[threadIdx.x], add);
+      __syncthreads();
+    }
+    int src = max(carry, threadIdx.x > 0 ? s_m[threadIdx.x - 1] : -1);
+    __syncthreads();
+    for (int q = 0; q < kGuardPer; q++) {
+      const int i = base + q;
+      if (i > P.guard_last) break;
+      if (phidot_ok(v[q])) {
+        src = i;
+      } else {
+        r[i] = r[src];
+      }
     }
   }
 }
@@ -260,8 +269,9 @@ int guard_blocks(const DenseParams &P) {
 void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
   const int nb = guard_blocks(P);
   if (nb == 0) return;
-  hipLaunchKernelGGL(k_guard_last, dim3(nb), dim3(kGuardThreads), 0, s, P, B);
-  hipLaunchKernelGGL(k_guard_fix, dim3(nb), dim3(kGuardThreads), 0, s, P, B);
+  const int grid = std::min(nb, kGuardGrid);
+  hipLaunchKernelGGL(k_guard_last, dim3(grid), dim3(kGuardThreads), 0, s, P, B, nb);
+  hipLaunchKernelGGL(k_guard_fix, dim3(grid), dim3(kGuardThreads), 0, s, P, B, nb);
 }
 
 // ------------------------------------------------------------------ folds
@@ -508,178 +518,11 @@ __device__ void fold_serial_replay(const FoldArgs &A, const FoldWork &W, int p, 
   }
 }
 
-// Rounded increment of a step on the binade grid of exponent k:
-// r = rn(d / u) with u = 2^(k-53), an exact tie (fraction 1/2) rounded down
-// and flagged.  Adding 1.5 * 2^52 rounds f to an integer at once (round to
-// nearest even; |f| < 2^51 keeps the sum in [2^52, 2^53), where the ulp is
-// 1), and the difference of the bit patterns is that integer.  Equal to
-// floor(f) + (frac(f) > 1/2) with the tie test frac(f) == 1/2.
-__device__ inline long long grid_round(double d, int k, bool &tie) {
-  constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
-  const double f = ldexp(d, 53 - k);
-  const double t = f + kMagic;
-  long long r = __double_as_longlong(t) - __double_as_longlong(kMagic);
-  const double diff = f - (t - kMagic);  // exact: f - rn(f), in [-1/2, 1/2]
-  tie = (diff == 0.5) || (diff == -0.5);
-  if (diff == -0.5) r -= 1;  // the tie went up: take the floor
-  return r;
-}
-
-// The block is uniform when every step's key is the same binade grid: the
-// approximate values a_j of the block (the classification's own, a_0 = the
-// lane's start, a_(j+1) = a_j + d_j) satisfy, for the smallest and largest
-// magnitude, |a| - E > 0 and frexp(|a|min - E) == frexp(|a|max + E), with one
-// sign; every |a| +- E then lies in that binade (rounding is monotone).  It
-// also needs |d| 2^(53-k) < 2^51 for grid_round.  All increments must be
-// finite.  Returns the binade exponent, or INT_MIN when the block is not
-// uniform; sgn receives the sign.  Both passes decide alike (same inputs).
-__device__ inline int uniform_block(const double *d, double a0, double E, double *sh, int *shi, int &sgn) {
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  double a = a0, amin = a0, amax = a0, dmax = 0.0, tsum = 0.0;
-#pragma unroll
-  for (int j = 0; j < kFoldPerThread; j++) {
-    a = a + d[j];
-    amin = fmin(amin, a);
-    amax = fmax(amax, a);
-    dmax = fmax(dmax, fabs(d[j]));
-    tsum += d[j];
-  }
-  int fin = __builtin_isfinite(tsum) && __builtin_isfinite(a) ? 1 : 0;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    amin = fmin(amin, __shfl_xor(amin, o, 64));
-    amax = fmax(amax, __shfl_xor(amax, o, 64));
-    dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
-    fin &= __shfl_xor(fin, o, 64);
-  }
-  constexpr int NW = kFoldThreads / 64;
-  if (lane == 0) {
-    sh[wave] = amin;
-    sh[NW + wave] = amax;
-    sh[2 * NW + wave] = dmax;
-    shi[wave] = fin;
-  }
-  __syncthreads();
-  amin = sh[0];
-  amax = sh[NW];
-  dmax = sh[2 * NW];
-  fin = shi[0];
-#pragma unroll
-  for (int w = 1; w < NW; w++) {
-    amin = fmin(amin, sh[w]);
-    amax = fmax(amax, sh[NW + w]);
-    dmax = fmax(dmax, sh[2 * NW + w]);
-    fin &= shi[w];
-  }
-  __syncthreads();
-  if (!fin) return INT_MIN;
-  double lo_abs, hi_abs;
-  if (amin > 0.0) {
-    lo_abs = amin;
-    hi_abs = amax;
-    sgn = 1;
-  } else if (amax < 0.0) {
-    lo_abs = -amax;
-    hi_abs = -amin;
-    sgn = -1;
-  } else {
-    return INT_MIN;
-  }
-  const double lo = lo_abs - E, hi = hi_abs + E;
-  if (!(lo > 0.0) || !__builtin_isfinite(hi)) return INT_MIN;
-  int klo, khi;
-  frexp(lo, &klo);
-  frexp(hi, &khi);
-  if (klo != khi) return INT_MIN;
-  if (!(ldexp(dmax, 53 - klo) < 2251799813685248.0)) return INT_MIN;  // 2^51
-  return klo;
-}
-
-// A uniform block is one run on the grid of exponent K: its item list is one
-// item (kind 0 from step 0, the block's total R) and its ties; the write pass
-// gives every step m_start + (inclusive prefix of r) + (ties rounded up so
-// far) on that grid.  The same items, ties and values as the general path
-// below, whose keys all equal K here, with a few operations per step.
-template <bool WRITE, int DIR>
-__device__ inline void fold_uniform(const FoldArgs &A, const FoldWork &W, int p, int b, const double *d, int K,
-                                    int sgn, long long *shl, int *shi, double *sd) {
-  const int tid = threadIdx.x;
-  // steps past the end have d = 0: r = 0 and no tie
-  long long lane_R = 0;
-  int tie_mask = 0;
-#pragma unroll
-  for (int j = 0; j < kFoldPerThread; j++) {
-    bool tie;
-    lane_R += grid_round(d[j], K, tie);
-    if (tie) tie_mask |= 1 << j;
-  }
-  long long R_tot;
-  const long long carry = block_excl_sum(lane_R, shl, &R_tot);
-  int nties_tot;
-  const int tie_base = block_excl_sum((int)__popc(tie_mask), shi, &nties_tot);
-  FoldItem *items = W.items + ((size_t)p * A.nblk + b) * kFoldMaxItems;
-  FoldTie *ties = W.ties + ((size_t)p * A.nblk + b) * kFoldMaxTies;
-  if (!WRITE) {
-    if (tid == 0) {
-      W.nitems[p * A.nblk + b] = 1;
-      W.nties[p * A.nblk + b] = nties_tot;
-      if (nties_tot > kFoldMaxTies) atomicExch(&W.status[p], 1);
-      items[0].kind = 0;
-      items[0].k = K;
-      items[0].sgn = sgn;
-      items[0].first = 0;
-      items[0].tie_first = 0;
-      items[0].d = d[0];
-      items[0].R = R_tot;
-    }
-    if (nties_tot > kFoldMaxTies || !tie_mask) return;
-    long long P = carry;
-    int tc = tie_base;
-#pragma unroll
-    for (int j = 0; j < kFoldPerThread; j++) {
-      bool tie;
-      P += grid_round(d[j], K, tie);
-      if (tie) {
-        ties[tc].step = tid * kFoldPerThread + j;
-        ties[tc].P = P;
-        ties[tc].up = 0;
-        tc++;
-      }
-    }
-    return;
-  }
-  if (nties_tot > kFoldMaxTies) return;
-  const long long m0 = items[0].m_start;
-  long long ups = 0;
-  for (int q = 0; q < tie_base; q++) ups += ties[q].up;
-  long long P = carry;
-  int tc = tie_base;
-  double outv[kFoldPerThread];
-#pragma unroll
-  for (int j = 0; j < kFoldPerThread; j++) {
-    bool tie;
-    P += grid_round(d[j], K, tie);
-    if (tie) ups += ties[tc++].up;
-    outv[j] = ldexp((double)(m0 + P + ups), K - 53);
-  }
-  // coalesced stores through LDS (the staged increments are consumed)
-#pragma unroll
-  for (int j = 0; j < kFoldPerThread; j++) sd[fold_lds_index(tid * kFoldPerThread + j)] = outv[j];
-  __syncthreads();
-  double *op = A.out[p] + (A.i0 + DIR * (b * kFoldBlock + tid + 1));
-#pragma unroll
-  for (int q = 0; q < kFoldPerThread; q++) {
-    const int s = b * kFoldBlock + tid + q * kFoldThreads;
-    if (s < A.nsteps) op[DIR * q * kFoldThreads] = sd[fold_lds_index(tid + q * kFoldThreads)];
-  }
-}
-
 // Classify steps, build the block's item list (PLAN) or write the exact
 // fold values of run steps (WRITE, after the chain resolved the run starts).
 template <bool WRITE, int DIR>
 __global__ __launch_bounds__(kFoldThreads, WRITE ? 3 : 4) void k_fold_local(FoldArgs A, FoldWork W) {
   __shared__ double shd[kFoldThreads / 64];
-  __shared__ double shu[3 * (kFoldThreads / 64)];
   __shared__ long long shl[2 * (kFoldThreads / 64)];
   __shared__ int shi[kFoldThreads];
   __shared__ int shf[kFoldThreads / 64];
@@ -703,14 +546,6 @@ __global__ __launch_bounds__(kFoldThreads, WRITE ? 3 : 4) void k_fold_local(Fold
   }
   double dummy;
   double a0 = W.xstart[p * A.nblk + b] + block_excl_sum(tsum, shd, &dummy);
-  {
-    int sgn = 0;
-    const int K = uniform_block(d, a0, E, shu, shi, sgn);
-    if (K != INT_MIN) {
-      fold_uniform<WRITE, DIR>(A, W, p, b, d, K, sgn, shl, shi, sd);
-      return;
-    }
-  }
   // keys: >= 0 binade grid, -1 exact replay, -2 beyond the end
   int key[kFoldPerThread];
   long long r[kFoldPerThread];
@@ -1178,7 +1013,7 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
 // per-waveform counters (non-positive Phidot samples, fold fallbacks); a
 // kernel rather than a memset so the sequence can be captured in a graph
 __global__ void k_clear_flags(int *flags) {
-  if (threadIdx.x < 4) flags[threadIdx.x] = 0;
+  if (threadIdx.x < 8) flags[threadIdx.x] = 0;
 }
 
 void launch_clear_flags(const DenseBuffers &B, hipStream_t s) {
@@ -1510,8 +1345,11 @@ typedef const __attribute__((address_space(4))) double *cdptr;
 typedef const __attribute__((address_space(4))) float *cfptr;
 typedef const __attribute__((address_space(4))) int *ciptr;
 
-// B.ms_p[block]: the Taylor order, or -1 for the per-sample path.
+// B.ms_p[blk]: the Taylor order of 256-sample block blk, or -1 for the
+// per-sample path; B.ms_slot[blk]: the slot of its expansion (coefficients
+// and e_ref).
 __device__ inline int block_order(const DenseBuffers &B, int blk) { return ((ciptr)B.ms_p)[blk]; }
+__device__ inline int block_slot(const DenseBuffers &B, int blk) { return ((ciptr)B.ms_slot)[blk]; }
 
 template <int PO, bool MIXED>
 __device__ void setup_harmonics(const DenseParams &P, const DenseBuffers &B, int blk, double eref, const double *sq,
@@ -1530,20 +1368,13 @@ __device__ void setup_harmonics(const DenseParams &P, const DenseBuffers &B, int
   }
 }
 
-// Block setup, one wave per expansion block: the range of e over the block's
-// signal samples, the order p and e_ref (every lane derives the same values
-// from a butterfly reduction), then one harmonic per lane.  The Bessel run of
-// a lane is staged in LDS (dynamic orders) and the expansion is formed in
-// registers.  A block with no signal sample gets p = 0 and no coefficients.
-template <bool MIXED>
-__global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, DenseBuffers B) {
-  __shared__ double s_J[kTaylorOrders][kSetupThreads];
-  const int lane = threadIdx.x, blk = blockIdx.x;
-  const int n_end = min(P.N, P.ip + P.ib);
-  const int span = kMsThreads << exp_shift(P);
-  double lo = 1e300, hi = -1e300;
-  for (int q = lane; q < span; q += kSetupThreads) {
-    const int i = blk * span + q;
+// Range of e over samples [i0, i0 + len) below n_end, reduced over the wave
+// (every lane holds it); lo > hi when the range holds no signal sample.
+__device__ inline void e_range(const DenseBuffers &B, int i0, int len, int n_end, double &lo, double &hi) {
+  lo = 1e300;
+  hi = -1e300;
+  for (int q = threadIdx.x; q < len; q += kSetupThreads) {
+    const int i = i0 + q;
     if (i < n_end) {
       const double e = B.e[i];
       lo = fmin(lo, e);
@@ -1554,34 +1385,103 @@ __global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, 
     lo = fmin(lo, __shfl_xor(lo, o));
     hi = fmax(hi, __shfl_xor(hi, o));
   }
+}
+
+// The expansion of one slot at order p about eref (one harmonic per lane).
+template <bool MIXED>
+__device__ inline void setup_slot(const DenseParams &P, const DenseBuffers &B, int slot, int p, double eref,
+                                  double *sJ) {
+  double sq[kTaylorMax + 1];
+  taylor_sqrt1me2(eref, p, sq);
+  switch (p) {
+    case 0: setup_harmonics<0, MIXED>(P, B, slot, eref, sq, sJ); break;
+    case 1: setup_harmonics<1, MIXED>(P, B, slot, eref, sq, sJ); break;
+    case 2: setup_harmonics<2, MIXED>(P, B, slot, eref, sq, sJ); break;
+    case 3: setup_harmonics<3, MIXED>(P, B, slot, eref, sq, sJ); break;
+    case 4: setup_harmonics<4, MIXED>(P, B, slot, eref, sq, sJ); break;
+    case 5: setup_harmonics<5, MIXED>(P, B, slot, eref, sq, sJ); break;
+    case 6: setup_harmonics<6, MIXED>(P, B, slot, eref, sq, sJ); break;
+    case 7: setup_harmonics<7, MIXED>(P, B, slot, eref, sq, sJ); break;
+    default: setup_harmonics<8, MIXED>(P, B, slot, eref, sq, sJ); break;
+  }
+}
+
+// Block setup, one wave per expansion block of 256 << xs samples: the range
+// of e over the block's signal samples, the order p and e_ref (every lane
+// derives the same values from a butterfly reduction), then one harmonic per
+// lane.  The Bessel run of a lane is staged in LDS (dynamic orders) and the
+// expansion is formed in registers.  An expansion block of more than 256
+// samples that no order up to kTaylorMax serves (fast evolution near a
+// plunge) is listed for k_modesum_split (count in flags[4]); other failures
+// (no Bessel series: e > kSeriesMaxE, more than kTaylorModes harmonics) list
+// the block's 256-sample blocks for the per-sample path.  A block with no
+// signal sample gets p = 0 and no coefficients.
+template <bool MIXED>
+__global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, DenseBuffers B) {
+  __shared__ double s_J[kTaylorOrders][kSetupThreads];
+  const int lane = threadIdx.x, eb = blockIdx.x;
+  const int n_end = min(P.N, P.ip + P.ib);
+  const int xs = exp_shift(P), span = kMsThreads << xs, sub = 1 << xs;
+  const int nb = (P.N + 2 + kMsThreads - 1) / kMsThreads;
+  double lo, hi;
+  e_range(B, eb * span, span, n_end, lo, hi);
+  const bool any = lo <= hi;
+  const bool series = P.bessel_series && P.nmodes <= kTaylorModes && hi <= kSeriesMaxE;
   int p = -1;
   double eref = 0.0;
-  const bool any = lo <= hi;
   if (!any) {
     p = 0;
-  } else if (P.bessel_series && P.nmodes <= kTaylorModes && hi <= kSeriesMaxE) {
+  } else if (series) {
     eref = 0.5 * (lo + hi);
     p = taylor_order(P.nmodes, 0.5 * (hi - lo), MIXED ? kTaylorTolFp32 : kTaylorTolFp64);
   }
+  const bool split = p < 0 && series && sub > 1;
   if (lane == 0) {
-    B.ms_p[blk] = p;
-    B.ms_eref[blk] = eref;
-    if (p < 0) B.ms_list[atomicAdd(&B.flags[2], 1)] = blk;
+    B.ms_eref[eb] = eref;
+    if (split) B.ms_split[atomicAdd(&B.flags[4], 1)] = eb;
+    for (int blk = eb * sub; blk < min(nb, eb * sub + sub); blk++) {
+      B.ms_p[blk] = p;
+      B.ms_slot[blk] = eb;
+      if (p < 0 && !split) B.ms_list[atomicAdd(&B.flags[2], 1)] = blk;
+    }
   }
   if (p < 0 || !any) return;
-  double sq[kTaylorMax + 1];
-  taylor_sqrt1me2(eref, p, sq);
-  double *sJ = &s_J[0][lane];
-  switch (p) {
-    case 0: setup_harmonics<0, MIXED>(P, B, blk, eref, sq, sJ); break;
-    case 1: setup_harmonics<1, MIXED>(P, B, blk, eref, sq, sJ); break;
-    case 2: setup_harmonics<2, MIXED>(P, B, blk, eref, sq, sJ); break;
-    case 3: setup_harmonics<3, MIXED>(P, B, blk, eref, sq, sJ); break;
-    case 4: setup_harmonics<4, MIXED>(P, B, blk, eref, sq, sJ); break;
-    case 5: setup_harmonics<5, MIXED>(P, B, blk, eref, sq, sJ); break;
-    case 6: setup_harmonics<6, MIXED>(P, B, blk, eref, sq, sJ); break;
-    case 7: setup_harmonics<7, MIXED>(P, B, blk, eref, sq, sJ); break;
-    default: setup_harmonics<8, MIXED>(P, B, blk, eref, sq, sJ); break;
+  setup_slot<MIXED>(P, B, eb, p, eref, &s_J[0][lane]);
+}
+
+// The 256-sample blocks of the expansion blocks listed by k_modesum_setup,
+// each expanded on its own (slot ne + blk), one wave per block over a fixed
+// grid; blocks that still need an order above kTaylorMax go to the
+// per-sample path.  Returns at once when nothing was listed.
+constexpr int kSplitGrid = 256;
+template <bool MIXED>
+__global__ __launch_bounds__(kSetupThreads) void k_modesum_split(DenseParams P, DenseBuffers B, int ne) {
+  const int count = B.flags[4];
+  if (count == 0) return;
+  __shared__ double s_J[kTaylorOrders][kSetupThreads];
+  const int lane = threadIdx.x, sub = 1 << exp_shift(P);
+  const int n_end = min(P.N, P.ip + P.ib);
+  const int nb = (P.N + 2 + kMsThreads - 1) / kMsThreads;
+  for (int k = blockIdx.x; k < count * sub; k += gridDim.x) {
+    const int blk = B.ms_split[k / sub] * sub + k % sub;
+    if (blk >= nb) continue;
+    double lo, hi;
+    e_range(B, blk * kMsThreads, kMsThreads, n_end, lo, hi);
+    const bool any = lo <= hi;
+    int p = 0;
+    double eref = 0.0;
+    if (any) {
+      eref = 0.5 * (lo + hi);
+      p = taylor_order(P.nmodes, 0.5 * (hi - lo), MIXED ? kTaylorTolFp32 : kTaylorTolFp64);
+    }
+    const int slot = ne + blk;
+    if (lane == 0) {
+      B.ms_p[blk] = p;
+      B.ms_slot[blk] = slot;
+      B.ms_eref[slot] = eref;
+      if (p < 0) B.ms_list[atomicAdd(&B.flags[2], 1)] = blk;
+    }
+    if (p >= 0 && any) setup_slot<MIXED>(P, B, slot, p, eref, &s_J[0][lane]);
   }
 }
 
@@ -1590,6 +1490,7 @@ static int expansion_blocks(const DenseParams &P) {
   const int span = kMsThreads << exp_shift(P);
   return (P.N + 2 + span - 1) / span;
 }
+int modesum_slots(const DenseParams &P) { return expansion_blocks(P) + modesum_blocks(P); }
 
 // Per-sample phase factors shared by both harmonic loops: u = exp(i (Phi + dA))
 // with Phi reduced mod 2 pi (Cody-Waite, three parts of 2 pi) and q0 = exp(i dB),
@@ -1739,9 +1640,9 @@ __device__ inline bool modesum_tail(const DenseParams &P, const DenseBuffers &B,
 // final scaling stay FP64).
 template <bool MIXED>
 __device__ inline void modesum_block(const DenseParams &P, const DenseBuffers &B, int blk) {
-  const int eb = blk >> exp_shift(P);
-  const int p = block_order(B, eb);
+  const int p = block_order(B, blk);
   if (p < 0) return;  // k_modesum_direct sums this block
+  const int slot = block_slot(B, blk);
   const int i = blk * kMsThreads + threadIdx.x;
   if (i >= P.N + 2) return;
   const int n_end = min(P.N, P.ip + P.ib);  // samples [0, n_end) carry the signal
@@ -1752,8 +1653,8 @@ __device__ inline void modesum_block(const DenseParams &P, const DenseBuffers &B
   B.t[i] = G.ti;
   PhaseSeed S;
   phase_seed<MIXED>(P, G, S);
-  const double d = G.e - B.ms_eref[eb];
-  const size_t base = (size_t)eb * P.nmodes * kCoefStride;
+  const double d = G.e - B.ms_eref[slot];
+  const size_t base = (size_t)slot * P.nmodes * kCoefStride;
   double sum[3];
   if (MIXED) {
     cfptr cf = (cfptr)(B.ms_coef + base);
@@ -1809,7 +1710,7 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum(DenseParams P, DenseBuff
   modesum_block<MIXED>(P, B, blk);
 }
 
-// Per-sample mode sum for the blocks listed by k_modesum_setup (B.ms_p < 0):
+// Per-sample mode sum for the 256-sample blocks listed by k_modesum_setup (B.ms_p < 0):
 // eight lanes per sample, each summing every eighth harmonic (the per-harmonic
 // Bessel and phase evaluation is the long part), partial sums combined by
 // shuffles.  A fixed grid walks the list, 32 samples per workgroup.
@@ -1841,8 +1742,8 @@ __device__ inline void modesum_direct_part(const DenseParams &P, const DenseBuff
 
 __global__ __launch_bounds__(kMsThreads) void k_modesum_direct(DenseParams P, DenseBuffers B) {
   const int count = B.flags[2];
-  const int span = kMsThreads << exp_shift(P);  // samples per listed expansion block
-  const int per = span / kDirectSamples;        // workgroup passes per listed block
+  const int span = kMsThreads;           // samples per listed block
+  const int per = span / kDirectSamples;  // workgroup passes per listed block
   const int units = count * per;
   const int g = threadIdx.x % kDirectGroups, sl = threadIdx.x / kDirectGroups;
   const int n_end = min(P.N, P.ip + P.ib);
@@ -1878,9 +1779,12 @@ void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) 
   const int nb = modesum_blocks(P), ne = expansion_blocks(P);
   if (P.mixed) {
     hipLaunchKernelGGL(k_modesum_setup<true>, dim3(ne), dim3(kSetupThreads), 0, s, P, B);
+    if (exp_shift(P) > 0) hipLaunchKernelGGL(k_modesum_split<true>, dim3(kSplitGrid), dim3(kSetupThreads), 0, s, P, B, ne);
     hipLaunchKernelGGL(k_modesum<true>, dim3(nb), dim3(kMsThreads), 0, s, P, B, nb);
   } else {
     hipLaunchKernelGGL(k_modesum_setup<false>, dim3(ne), dim3(kSetupThreads), 0, s, P, B);
+    if (exp_shift(P) > 0)
+      hipLaunchKernelGGL(k_modesum_split<false>, dim3(kSplitGrid), dim3(kSetupThreads), 0, s, P, B, ne);
     hipLaunchKernelGGL(k_modesum<false>, dim3(nb), dim3(kMsThreads), 0, s, P, B, nb);
   }
   hipLaunchKernelGGL(k_modesum_direct, dim3(kDirectGrid), dim3(kMsThreads), 0, s, P, B);
@@ -1924,8 +1828,11 @@ __global__ __launch_bounds__(kDotThreads) void k_weighted_dot(const double2 *hf,
 
 // out = 4 x (sum of the partial pairs); NaN when the waveform was rejected
 // on the device (flags[3], a non-positive radial rate at t = 0)
+// flags: the waveform's flags (flags[3]: rejected on the device, the result
+// is NaN) or null; clear: flags to zero once read (the lane's, so that its
+// next waveform starts from zeros without a clearing launch) or null.
 __global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *partials, int nparts, double *out,
-                                                           const int *flags) {
+                                                           const int *flags, int *clear) {
   __shared__ double s_dh[kDotThreads], s_hh[kDotThreads];
   double dh = 0.0, hh = 0.0;
   for (int q = threadIdx.x; q < nparts; q += blockDim.x) {
@@ -1947,18 +1854,20 @@ __global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *partial
     out[0] = bad ? __builtin_nan("") : 4.0 * s_dh[0];
     out[1] = bad ? __builtin_nan("") : 4.0 * s_hh[0];
   }
+  __syncthreads();
+  if (clear && threadIdx.x < 8) clear[threadIdx.x] = 0;
 }
 
-void launch_dot_final(const double *partials, int nparts, double *out, const int *flags, hipStream_t s) {
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nparts, out, flags);
+void launch_dot_final(const double *partials, int nparts, double *out, const int *flags, int *clear, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nparts, out, flags, clear);
 }
 
 void launch_weighted_dot(const double2 *hf, const double2 *data, const double *w, int nbins, double2 *hw_out,
-                         double *partials, double *out, const int *flags, hipStream_t s) {
+                         double *partials, double *out, const int *flags, int *clear, hipStream_t s) {
   const int need = (2 * nbins + kDotThreads - 1) / kDotThreads;
   const int nb = need < kDotBlocks ? need : kDotBlocks;
   hipLaunchKernelGGL(k_weighted_dot, dim3(nb), dim3(kDotThreads), 0, s, hf, data, w, nbins, hw_out, partials);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nb, out, flags);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, partials, nb, out, flags, clear);
 }
 
 }  // namespace aak

@@ -142,7 +142,7 @@ extern "C" int aak_inner_product_noalloc(const double *hI, const double *hII, co
   hipLaunchKernelGGL(k_permute_bins<double>, grid, block, 0, s, ASDinv1, ASDinv2, W, R, C, F.M);
   launch_fft_pass1(F, hI, hII, Y, s);
   launch_fft_pass2(F, Y, partials, nullptr, s);
-  launch_dot_final(partials, fft_partials(F), like_out, nullptr, s);
+  launch_dot_final(partials, fft_partials(F), like_out, nullptr, nullptr, s);
   e = hipGetLastError();
   if (e != hipSuccess) return fail(-100, std::string("HIP: ") + hipGetErrorString(e) + " launching the inner product");
   return 0;

@@ -200,10 +200,10 @@ class GPUAAK:
 
     def info(self):
         """Diagnostics of the last gpu_gen_AAK."""
-        buf = np.zeros(10, dtype=np.int32)
-        _lib.lib().aak_get_info(self._h, _lib.iptr(buf), 10)
+        buf = np.zeros(12, dtype=np.int32)
+        _lib.lib().aak_get_info(self._h, _lib.iptr(buf), 12)
         keys = ("steps", "nmodes", "i_plunge", "i_buffer", "K", "i0", "fold_fallbacks",
-                "nonpositive_phidot", "j0", "plunged")
+                "nonpositive_phidot", "j0", "plunged", "per_sample_blocks", "split_blocks")
         return dict(zip(keys, (int(x) for x in buf)))
 
     def set_profiling(self, on=True):

@@ -156,8 +156,9 @@ int aak_get_trajectory(aak_handle h, double *t_in, double *e_in, double *v_in, d
 int aak_set_nmax(aak_handle h, int n_max);
 
 /* Diagnostics of the last aak_gpu_gen_AAK; no counterpart in the reference.
- * info[0..9] = steps, nmodes, i_plunge, i_buffer, knots K, i0, fold fallbacks,
- * non-positive Phidot count, j0, plunged.  Returns the number written. */
+ * info[0..11] = steps, nmodes, i_plunge, i_buffer, knots K, i0, fold fallbacks,
+ * non-positive Phidot count, j0, plunged, 256-sample blocks summed per sample,
+ * expansion blocks expanded again per 256 samples.  Returns the number written. */
 int aak_get_info(aak_handle h, int *info, int n);
 
 /* Stage timing; no counterpart in the reference (its timers are commented out,

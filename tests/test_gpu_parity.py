@@ -200,9 +200,12 @@ def test_high_eccentricity_parity():
 
 def test_mode_sum_forms_agree_across_a_plunge():
     # fast inspiral close to plunge: blocks with a wide e range fall back to the
-    # per-sample Bessel evaluation, the others use the block Taylor form
-    pars = [0.5, 0.3, 8.0, 0.45, 3e5, 5.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
-    out, g, ref = _compare(pars, 2 ** 20, 15.0, n_max=40, label="fast-plunge")
+    # per-sample Bessel evaluation (about 90 of them here), the others use the
+    # block Taylor form
+    pars = [0.5, 0.5818, 9.4211, 0.3929, 119708.4349, 13.4395, 4.9582, 0.2884, 0.8303, 1.5676, 4.4618, 1.3913,
+            0.2321, 1.0]
+    out, g, ref = _compare(pars, 2 ** 18, 15.0, n_max=50, label="fast-plunge")
+    assert out["info"]["per_sample_blocks"] > 0
     assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
 
 
@@ -226,3 +229,40 @@ def test_run_phase_trajectory_keeps_the_oracle_knots():
         for a, b in zip(got[:5], want[:5]):
             np.testing.assert_array_equal(a, b)
         assert got[5:] == want[5:]
+
+
+def test_mixed_split_expansions_near_a_plunge():
+    # mixed form, 1024-sample expansion blocks: the blocks near the plunge that
+    # no order <= 8 serves are expanded again per 256 samples, so only the
+    # 256-sample blocks that fail on their own are summed per sample -- the
+    # same blocks as with 256-sample expansions throughout -- and the waveform
+    # stays within the mixed form's mismatch bar against the oracle
+    from emri_kludge_suite_amd import GPUAAK
+    from emri_kludge_suite_amd.multigrid import psd_weights
+    # a production-prior source plunging after 8 days: nine of its 1024-sample
+    # expansion blocks need an order above 8, one 256-sample block still does
+    pars = [0.5, 0.5818, 9.4211, 0.3929, 119708.4349, 13.4395, 4.9582, 0.2884, 0.8303, 1.5676, 4.4618, 1.3913,
+            0.2321, 1.0]
+    N, dt = 2 ** 18, 15.0
+    print("case mixed split N=2^18 dt=15 n_max=50", flush=True)
+    ref = ol.o_aak_wave(pars, length=N, dt=dt, n_max=50)
+    w = psd_weights(N, dt, 1)
+    fr = [np.fft.rfft(ref[k]) * w for k in ("hI", "hII")]
+    g = GPUAAK(1.0, N // 100, N, 100.0, dt, True, False)
+    g.set_nmax(50)
+    g.set_precision("mixed")
+    res = {}
+    for xs in (0, 2):
+        g.set_tuning(exp_shift=xs)
+        g.gpu_gen_AAK(*pars)
+        info = g.info()
+        _, hI, hII = g.GetWaveform()
+        fg = [np.fft.rfft(x[:N]) * w for x in (hI, hII)]
+        ab = sum(np.vdot(a, b).real for a, b in zip(fr, fg))
+        mm = 1 - ab / np.sqrt(sum(np.vdot(a, a).real for a in fr) * sum(np.vdot(b, b).real for b in fg))
+        res[xs] = (info["per_sample_blocks"], info["split_blocks"], mm)
+        print("  exp_shift %d: per-sample blocks %d, split expansion blocks %d, mismatch vs oracle %.3e"
+              % (xs, *res[xs]), flush=True)
+    assert res[2][1] > 0
+    assert res[2][0] == res[0][0]
+    assert res[0][2] <= 1e-6 and res[2][2] <= 1e-6
