@@ -124,6 +124,7 @@ struct DenseParams {
                                   // output, dense arrays); 0: only what a likelihood reads
   GeomF32 gf;                     // response constants in FP32 (mixed form)
   int bessel_series;              // 1: series + recurrence, 0: Miller from above
+  double e_series;  // series valid for e <= e_series (0: recurrence only)
   unsigned char bterms[kMaxModes + 1];  // series length per harmonic ([0]: orders 0..4)
 };
 

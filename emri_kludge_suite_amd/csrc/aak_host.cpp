@@ -715,11 +715,22 @@ int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *p
   }
   P.zeta = par[0] / sp.D / kGpc;
   P.Mphys = sp.M;
-  // Bessel evaluation of the mode sum: series for e below kSeriesMaxE
+  // Bessel evaluation of the mode sum: series for e below kSeriesMaxE.  Only
+  // the knots that shape e on the signal (samples below ip + ib: the knots up
+  // to the first one past the last signal sample) bound it; past a plunge the
+  // AK evolution can run far off before it is cut, and those knots set no
+  // sample the mode sum evaluates.  Samples above the bound (spline
+  // overshoot) take the recurrence instead (P.e_series).
+  const int n_sig = std::min(N, ip + ib);
+  const double t_sig = (double)(n_sig - 1 - P.i0) * P.dt;
   double e_hi = 0.0;
-  for (int k = 0; k < ks.K; k++) e_hi = std::max(e_hi, ks.ky(0)[k]);
+  for (int k = 0; k < ks.K; k++) {
+    e_hi = std::max(e_hi, ks.ky(0)[k]);
+    if (ks.kt()[k] > t_sig) break;
+  }
   const double e_bound = e_hi * 1.01 + 0.01;
   P.bessel_series = (nmodes <= kMaxModes && e_bound <= kSeriesMaxE) ? 1 : 0;
+  P.e_series = P.bessel_series ? e_bound : 0.0;
   P.mixed = cfg.mixed ? 1 : 0;
   P.xshift = cfg.exp_shift >= 0 ? cfg.exp_shift : (cfg.mixed ? 2 : 0);
   P.full_dense = 1;

@@ -1426,7 +1426,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, 
   double lo, hi;
   e_range(B, eb * span, span, n_end, lo, hi);
   const bool any = lo <= hi;
-  const bool series = P.bessel_series && P.nmodes <= kTaylorModes && hi <= kSeriesMaxE;
+  const bool series = P.bessel_series && P.nmodes <= kTaylorModes && hi <= P.e_series;
   int p = -1;
   double eref = 0.0;
   if (!any) {
@@ -1724,7 +1724,7 @@ __device__ inline void modesum_direct_part(const DenseParams &P, const DenseBuff
   double accI = 0.0, accII = 0.0, J[5];
   for (int nn = 1 + g; nn <= P.nmodes; nn += kDirectGroups) {
     const double nPhi = harmonic_phase(P, G, (double)nn, gdpi);
-    if (P.bessel_series)
+    if (P.bessel_series && e <= P.e_series)
       bessel_five_series(nn, e, B.bessel_c, B.bessel_lgf, P.bterms[nn], P.bterms[0], J);
     else
       bessel_five(nn, nn * e, J);

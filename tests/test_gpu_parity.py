@@ -266,3 +266,16 @@ def test_mixed_split_expansions_near_a_plunge():
     assert res[2][1] > 0
     assert res[2][0] == res[0][0]
     assert res[0][2] <= 1e-6 and res[2][2] <= 1e-6
+
+
+def test_runaway_evolution_past_the_plunge_keeps_the_series():
+    # a production-prior source that plunges after 21 hours: past the plunge
+    # the AK evolution runs off (e reaches 1.45 on the knots) before it is cut,
+    # but no sample the mode sum evaluates lies there; the Bessel series bound
+    # comes from the knots that shape the signal (e <= 0.04 here), so every
+    # block keeps the block Taylor form instead of the per-sample recurrence
+    pars = [0.5, 0.092934, 7.375436, 0.036833, 166902.580916, 29.647813, 1.893011, 6.231121, 3.951397, 0.509965,
+            4.605533, 0.649373, 5.930568, 1.0]
+    out, g, ref = _compare(pars, 2 ** 18, 15.0, n_max=50, label="runaway")
+    assert out["info"]["per_sample_blocks"] == 0
+    assert out["relL2_hI"] <= 1e-10 and out["relL2_hII"] <= 1e-10
