@@ -2,7 +2,7 @@
 
 bench.py measures one GPU and refuses --gpus N > 1 before touching a device;
 the figures it quotes from the committed rocprofv3 summaries (VALU issue
-fraction, HBM traffic per launch) must be present in those summaries.
+fraction, HBM traffic per launch, executed VALU FLOP per launch) must be present in those summaries.
 """
 import os
 import subprocess
@@ -25,6 +25,9 @@ def test_bench_refuses_more_than_one_gpu():
 @pytest.mark.parametrize("name,key", [
     ("aak_likelihood_N2p21_n50_sq_counters.md", "valu_issue_frac modesum_stage_mixed"),
     ("aak_likelihood_N2p21_n50_hbm_counters.md", "traffic_bytes_per_launch modesum_stage_mixed"),
+    ("aak_likelihood_N2p21_n50_hbm_counters.md", "traffic_bytes_per_launch fold_stage"),
+    ("aak_likelihood_N2p21_n50_hbm_counters.md", "traffic_bytes_per_launch k_rates"),
+    ("aak_likelihood_N2p21_n50_flops.md", "valu_flop_per_launch modesum_stage_mixed"),
 ])
 def test_profile_figures_the_bench_quotes_are_committed(name, key):
     import bench
