@@ -251,6 +251,7 @@ struct aak_handle_s {
   // stage timing on the compute stream: rates, guard+folds, mode sum, rFFT, reduction
   int profiling = 0;
   int force_fold_replay = 0;  // test hook: every fold takes the serial replay
+  int fold_mode = 0;          // test hook: 0 single-pass look-back fold, 1 three-pass fold
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   hipEvent_t *evs = ev;               // events the marks go to (per template in a batch)
   std::vector<hipEvent_t> batch_ev;   // 6 per template of the last profiled batch
@@ -345,6 +346,8 @@ static int alloc_lane(aak_handle h, Lane &L) {
   const size_t o_items = carve(sizeof(FoldItem) * 3 * (size_t)nblk * kFoldMaxItems);
   const size_t o_ties = carve(sizeof(FoldTie) * 3 * (size_t)nblk * kFoldMaxTies);
   const size_t o_st = carve(sizeof(int) * 3);
+  const size_t o_la = carve(sizeof(long long) * 3 * nblk), o_li = carve(sizeof(long long) * 3 * nblk);
+  const size_t o_lc = carve(sizeof(unsigned) * 4);
   HIPCHK(hipMalloc(&L.d_foldmem, bytes));
   char *base = (char *)L.d_foldmem;
   L.fw.bsum = (double *)(base + o_bsum);
@@ -357,6 +360,9 @@ static int alloc_lane(aak_handle h, Lane &L) {
   L.fw.items = (FoldItem *)(base + o_items);
   L.fw.ties = (FoldTie *)(base + o_ties);
   L.fw.status = (int *)(base + o_st);
+  L.fw.look_agg = (long long *)(base + o_la);
+  L.fw.look_incl = (long long *)(base + o_li);
+  L.fw.look_ctr = (unsigned *)(base + o_lc);
   L.fw.fallbacks = L.d_flags + 1;
   const int nb = h->nbins;
   HIPCHK(hipMalloc(&L.d_hf, sizeof(double2) * 2 * (size_t)nb));
@@ -528,7 +534,7 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
     if (nsteps <= 0) continue;
     FoldWork fw = L.fw;
     fw.force_replay = h->force_fold_replay;
-    if (fold_exact(P, B, fw, L.fold_nblk_cap, dir, nsteps, L.stream)) return fail(-7, "fold block capacity");
+    if (fold_exact(P, B, fw, L.fold_nblk_cap, dir, nsteps, h->fold_mode, L.stream)) return fail(-7, "fold block capacity");
   }
   launch_fold_fill(P, B, L.stream);
   h->mark(L, 2);
@@ -1543,7 +1549,14 @@ int aak_test_inner_product(aak_handle h, const double *hI, const double *hII, do
 
 int aak_test_force_fold_replay(aak_handle h, int on) {
   if (!h) return fail(-1, "null handle");
-  h->force_fold_replay = on ? 1 : 0;
+  h->force_fold_replay = on == 2 ? 2 : (on ? 1 : 0);
+  return 0;
+}
+
+int aak_test_fold_mode(aak_handle h, int mode) {
+  if (!h) return fail(-1, "null handle");
+  if (mode != 0 && mode != 1) return fail(-2, "fold mode must be 0 or 1");
+  h->fold_mode = mode;
   return 0;
 }
 

@@ -160,8 +160,10 @@ struct FoldWork {
   FoldItem *items;                      // [3][nblk][kFoldMaxItems]
   FoldTie *ties;                        // [3][nblk][kFoldMaxTies]
   int *status;                          // [3]: 0 ok, 1 overflow/fallback
+  long long *look_agg, *look_incl;      // [3][nblk]: run total of a one-run block; exact end value (bits)
+  unsigned *look_ctr;                   // [1]: block tickets of the single-pass fold
   int *fallbacks;                       // serial replays since the last counter reset
-  int force_replay;                     // 1: every phase takes the serial replay (test hook)
+  int force_replay;                     // test hook: 1 every phase takes the serial replay, 2 the middle block fails
 };
 
 // Device buffers of one waveform slot.
@@ -190,8 +192,10 @@ struct DenseBuffers {
 
 // Launch wrappers (aak_kernels.hip).
 void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
+// mode 0: single pass with a look-back (k_fold_resolve); 1: three passes
+// (k_fold_local, k_fold_chain, k_fold_local)
 int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, int nblk_cap, int dir, int nsteps,
-               hipStream_t s);
+               int mode, hipStream_t s);
 void launch_fold_fill(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 int guard_blocks(const DenseParams &P);

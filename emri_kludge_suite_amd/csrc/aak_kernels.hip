@@ -440,6 +440,12 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_sums(FoldArgs A, FoldWork
   }
 }
 
+// Look-back words of the single-pass fold (k_fold_resolve), reset by the plan
+constexpr long long kAggEmpty = (long long)0x8000000000000000ULL;  // not classified yet
+constexpr long long kAggNone = (long long)0x8000000000000001ULL;   // not a pure block
+constexpr long long kInclEmpty = 0x7FF4A5A5A5A5A5A5LL;             // end value not known yet
+constexpr long long kInclFail = 0x7FF4A5A5A5A5A5A6LL;              // the phase failed
+
 // approximate block start values and the error bound E
 __global__ __launch_bounds__(1024) void k_fold_plan(FoldArgs A, FoldWork W) {
   __shared__ double sh[1024];
@@ -479,8 +485,14 @@ __global__ __launch_bounds__(1024) void k_fold_plan(FoldArgs A, FoldWork W) {
     int ex = 0;
     frexp(shm[0] > 0.0 ? shm[0] : 1.0, &ex);
     W.E[p] = ((double)A.nsteps + 2.0 * kFoldBlock) * ldexp(1.0, ex - 52);
-    W.status[p] = W.force_replay ? 1 : 0;
+    W.status[p] = W.force_replay == 1 ? 1 : 0;
   }
+  // look-back state of the single-pass resolve
+  for (int b = tid; b < A.nblk; b += 1024) {
+    W.look_agg[p * A.nblk + b] = kAggEmpty;
+    W.look_incl[p * A.nblk + b] = kInclEmpty;
+  }
+  if (p == 0 && tid == 0) *W.look_ctr = 0;
 }
 
 __device__ inline int binade_key(double a, double E) {
@@ -984,8 +996,382 @@ __global__ __launch_bounds__(kChainThreads) void k_fold_chain(FoldArgs A, FoldWo
   }
 }
 
+// Single-pass alternative to k_fold_local<false> + k_fold_chain +
+// k_fold_local<true>: each workgroup classifies its block as above, keeps the
+// item and tie lists in LDS, and obtains the exact value before its first
+// step by a decoupled look-back over the blocks before it (Merrill and
+// Garland, "Single-pass Parallel Prefix Scan with Decoupled Look-back",
+// 2016), with the exact fold as the operator:
+// - a block that is one run on one binade grid without ties ("pure") acts on
+//   the value before it as m -> m + R (m the value in units of its binade's
+//   ulp), so consecutive pure blocks on one grid compose by adding their
+//   totals; it publishes R as soon as it has classified its steps.  Its grid
+//   is that of its first step, which every block can compute from the plan's
+//   approximate block start, so R is published alone;
+// - every block publishes its exact end value after its own serial walk.
+// Each published value is one 64-bit word written and polled with relaxed
+// device-scope atomics, so no fence orders it against anything else; "not
+// yet" and "failed" are signalling-NaN patterns arithmetic never produces.
+// The look-back reads 64 predecessors at a time with one wave, adds the
+// totals of the nearest run of pure blocks on one grid and stops at the first
+// block with an exact end value; it waits only on blocks that are not pure on
+// that grid.  Every block checks its own start and end against its binades
+// as the chain does, so a result that passes is the chain's result bit for
+// bit; a failed check, an overflowing list or a wait longer than kLookSpinCap
+// polls marks the phase, which k_fold_replay then replays serially.
+// Blocks take tickets from a counter in the order they start, so every block
+// a workgroup waits on has started before it and waits only on earlier
+// blocks itself: the waits cannot form a cycle.
+constexpr int kLookSpinCap = 1 << 18;
+
+__device__ inline long long look_load(const long long *v) {
+  return __hip_atomic_load(v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void look_store(long long *v, long long x) {
+  __hip_atomic_store(v, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// binade key of an exact value, as binade_key with E = 0 (-1 for 0, inf, NaN)
+__device__ inline int exact_key(double x) {
+  if (!(fabs(x) > 0.0) || !(fabs(x) < __builtin_inf())) return -1;
+  int kx;
+  frexp(x, &kx);
+  return (kx + 4096) * 2 + (x > 0.0 ? 1 : 0);
+}
+
+// Look-back state of block j: 2 end value known (or failed), 1 pure with its
+// total published, 0 neither yet (or not pure).
+struct LookState {
+  int st;
+  long long v;  // st 2: end value bits; st 1: the run total
+};
+__device__ inline LookState look_state(const long long *agg, const long long *incl, int j) {
+  const long long iw = look_load(incl + j);  // both loads in flight at once
+  const long long aw = look_load(agg + j);
+  if (iw != kInclEmpty) return {2, iw};
+  if (aw != kAggEmpty && aw != kAggNone) return {1, aw};
+  return {0, 0};
+}
+
+// The value before a block from an exact end value w and the pure blocks
+// combined after it (key K, total acc; K < 0: none).
+__device__ inline bool look_finish(long long w, int K, long long acc, double &x_out) {
+  if (w == kInclFail) return false;
+  double x = __longlong_as_double(w);
+  if (K >= 0) {
+    if (exact_key(x) != K) return false;
+    const int k = (K >> 1) - 4096;
+    long long m = (long long)ldexp(x, 53 - k);
+    m += acc;
+    const long long am = m < 0 ? -m : m;
+    if (am < (1LL << 52) || am >= (1LL << 53)) return false;
+    x = ldexp((double)m, k - 53);
+  }
+  x_out = x;
+  return true;
+}
+
+// Wave 0: exact value before block b's first step.  Returns false when the
+// phase failed (a failed predecessor, a failed binade check, a wait past the
+// cap).
+__device__ inline bool fold_look_back(const FoldArgs &A, const FoldWork &W, int p, int b, double E, double &x_out) {
+  const int lane = threadIdx.x & 63;
+  const long long *agg = W.look_agg + (size_t)p * A.nblk;
+  const long long *incl = W.look_incl + (size_t)p * A.nblk;
+  const double *xs = W.xstart + (size_t)p * A.nblk;
+  int j = b - 1;      // nearest block not yet combined
+  int K = -1;         // binade key of the combined pure blocks
+  long long acc = 0;  // their summed totals
+  int spins = 0;
+  for (;;) {
+    const int jb = j - lane;
+    // block -1 stands for the start value x0
+    LookState S = {2, __double_as_longlong(A.x0[p])};
+    int key = -1;
+    if (jb >= 0) {
+      S = look_state(agg, incl, jb);
+      if (S.st == 1) key = binade_key(xs[jb], E);
+    }
+    const int s0 = __shfl(S.st, 0, 64);
+    const int k0 = __shfl(key, 0, 64);
+    const int Kc = (K >= 0) ? K : (s0 == 1 ? k0 : -2);
+    const bool ext = S.st == 1 && key == Kc && key >= 0;
+    const unsigned long long mask = __ballot(ext);
+    const int s = (~mask) ? __builtin_ctzll(~mask) : 64;
+    long long r = (lane < s) ? S.v : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+    acc += r;
+    if (s > 0) K = Kc;
+    if (s == 64) {
+      j -= 64;
+      continue;
+    }
+    const int js = j - s;
+    const int ss = __shfl(S.st, s, 64);
+    if (ss == 2) return look_finish(__shfl(S.v, s, 64), K, acc, x_out);
+    // not yet published, or pure on another grid: wait for a change of state;
+    // an end value is used at once, a newly published total is combined by
+    // the next window, which starts at that block
+    LookState T;
+    do {
+      if (++spins > kLookSpinCap) return false;
+      __builtin_amdgcn_s_sleep(1);
+      T = look_state(agg, incl, js);
+    } while (T.st == ss);
+    if (T.st == 2) return look_finish(T.v, K, acc, x_out);
+    j = js;
+  }
+}
+
+template <int DIR>
+__global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, FoldWork W) {
+  __shared__ double shd[kFoldThreads / 64];
+  __shared__ long long shl[2 * (kFoldThreads / 64)];
+  __shared__ int shi[kFoldThreads];
+  __shared__ int shf[kFoldThreads / 64];
+  __shared__ double sd[kFoldLds];
+  __shared__ FoldItem s_items[kFoldMaxItems];
+  __shared__ FoldTie s_ties[kFoldMaxTies];
+  __shared__ double s_x[kFoldMaxItems];
+  __shared__ int s_pb[2], s_ok;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    const unsigned t = atomicAdd(W.look_ctr, 1u);
+    s_pb[0] = (int)(t % 3u);
+    s_pb[1] = (int)(t / 3u);
+  }
+  __syncthreads();
+  const int p = s_pb[0], b = s_pb[1];
+  long long *aggw = W.look_agg + (size_t)p * A.nblk;
+  long long *inclw = W.look_incl + (size_t)p * A.nblk;
+  // entry: a phase already marked (the replay hook) skips to the end
+  if (tid == 0) s_ok = __hip_atomic_load(W.status + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+  __syncthreads();
+  bool ok = s_ok != 0;
+  const int base = b * kFoldBlock + tid * kFoldPerThread;
+  double d[kFoldPerThread];
+  int key[kFoldPerThread];
+  long long r[kFoldPerThread];
+  int tie_mask = 0, head_mask = 0;
+  bool next_starts = true;
+  long long carry = 0;
+  int item_base = 0, tie_base = 0;
+  const double E = W.E[p];
+  if (ok) {
+    stage_increments<DIR>(A, p, b, sd);
+    double tsum = 0.0;
+#pragma unroll
+    for (int j = 0; j < kFoldPerThread; j++) {
+      d[j] = sd[fold_lds_index(tid * kFoldPerThread + j)];
+      tsum += d[j];
+    }
+    double dummy;
+    double a0 = W.xstart[p * A.nblk + b] + block_excl_sum(tsum, shd, &dummy);
+    // keys: >= 0 binade grid, -1 exact replay, -2 beyond the end
+#pragma unroll
+    for (int j = 0; j < kFoldPerThread; j++) {
+      const int s = base + j;
+      if (s >= A.nsteps) {
+        key[j] = -2;
+        r[j] = 0;
+        continue;
+      }
+      const double a1 = a0 + d[j];
+      const int k0 = binade_key(a0, E);
+      const int k1 = binade_key(a1, E);
+      if (k0 >= 0 && k0 == k1) {
+        key[j] = k0;
+        const int kexp = (k0 >> 1) - 4096;
+        const double f = ldexp(d[j], 53 - kexp);
+        const double fl = floor(f);
+        const double fr = f - fl;
+        long long ri = (long long)fl;
+        if (fr > 0.5) ri += 1;
+        else if (fr == 0.5) tie_mask |= (1 << j);
+        r[j] = ri;
+      } else {
+        key[j] = -1;
+        r[j] = 0;
+      }
+      a0 = a1;
+    }
+    shi[tid] = key[kFoldPerThread - 1];
+    __syncthreads();
+    const int prev_key = (tid > 0) ? shi[tid - 1] : -3;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kFoldPerThread; j++) {
+      const int kprev = (j == 0) ? prev_key : key[j - 1];
+      const bool h = (key[j] != -2) && (key[j] == -1 || kprev == -1 || kprev == -3 || key[j] != kprev);
+      if (h) head_mask |= (1 << j);
+    }
+    shi[tid] = ((head_mask & 1) || key[0] == -2) ? 1 : 0;
+    __syncthreads();
+    next_starts = (tid + 1 < kFoldThreads) ? (shi[tid + 1] != 0) : true;
+    __syncthreads();
+    long long tail = 0;
+    int has_head = 0;
+#pragma unroll
+    for (int j = 0; j < kFoldPerThread; j++) {
+      if (head_mask & (1 << j)) {
+        has_head = 1;
+        tail = 0;
+      }
+      tail += r[j];
+    }
+    const long long seg_incl = block_seg_incl(tail, has_head, shl, shf);
+    carry = __shfl_up(seg_incl, 1, 64);
+    if ((tid & 63) == 63) shl[kFoldThreads / 64 + (tid >> 6)] = seg_incl;
+    __syncthreads();
+    if ((tid & 63) == 0) carry = (tid > 0) ? shl[kFoldThreads / 64 + (tid >> 6) - 1] : 0;
+    __syncthreads();
+    int nheads_tot, nties_tot;
+    item_base = block_excl_sum((int)__popc(head_mask), shi, &nheads_tot);
+    tie_base = block_excl_sum((int)__popc(tie_mask), shi, &nties_tot);
+    if (nheads_tot > kFoldMaxItems || nties_tot > kFoldMaxTies) ok = false;
+    if (ok) {
+      // the block's item and tie lists, in LDS
+      long long P = carry;
+      int item = item_base - 1, tcount = tie_base;
+#pragma unroll
+      for (int j = 0; j < kFoldPerThread; j++) {
+        if (key[j] == -2) break;
+        const int sl = tid * kFoldPerThread + j;
+        if (head_mask & (1 << j)) {
+          item++;
+          P = 0;
+          s_items[item].kind = (key[j] == -1) ? 1 : 0;
+          s_items[item].k = (key[j] >= 0) ? (key[j] >> 1) - 4096 : 0;
+          s_items[item].sgn = (key[j] >= 0) ? ((key[j] & 1) ? 1 : -1) : 0;
+          s_items[item].first = sl;
+          s_items[item].tie_first = tcount;
+          s_items[item].d = d[j];
+        }
+        P += r[j];
+        const bool last_of_run = (key[j] >= 0) && ((j + 1 < kFoldPerThread)
+                                                       ? (((head_mask >> (j + 1)) & 1) || key[j + 1] == -2)
+                                                       : next_starts);
+        if (tie_mask & (1 << j)) {
+          s_ties[tcount].step = sl;
+          s_ties[tcount].P = P;
+          s_ties[tcount].up = 0;
+          tcount++;
+        }
+        if (last_of_run) s_items[item].R = P;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_pb[0] = nheads_tot;
+      s_pb[1] = nties_tot;
+      // a block that is one run without ties publishes its total at once
+      const bool pure = ok && nheads_tot == 1 && nties_tot == 0 && s_items[0].kind == 0;
+      look_store(aggw + b, pure ? s_items[0].R : kAggNone);
+    }
+    __syncthreads();
+  }
+  // wave 0: look back, then the block's own walk; publishes the end value
+  if (tid < 64) {
+    double x = 0.0;
+    bool wok = ok && (b == 0 ? (x = A.x0[p], true) : fold_look_back(A, W, p, b, E, x));
+    if (W.force_replay == 2 && b == A.nblk / 2) wok = false;  // test hook
+    if (tid == 0) {
+      const int ni = s_pb[0], nt = s_pb[1];
+      if (wok && b == 0) A.out[p][A.i0] = x;
+      for (int q = 0; wok && q < ni; q++) {
+        FoldItem &I = s_items[q];
+        if (I.kind == 1) {
+          x = x + I.d;
+          s_x[q] = x;
+          continue;
+        }
+        int kx;
+        frexp(x, &kx);
+        if (kx != I.k || (x > 0.0 ? 1 : -1) != I.sgn) {
+          wok = false;
+          break;
+        }
+        const long long m = (long long)ldexp(x, 53 - I.k);
+        I.m_start = m;
+        const int t1 = (q + 1 < ni) ? s_items[q + 1].tie_first : nt;
+        long long ups = 0;
+        for (int t = I.tie_first; t < t1; t++) {
+          const int up = (int)((m + s_ties[t].P + ups) & 1LL);
+          s_ties[t].up = up;
+          ups += up;
+        }
+        const long long mend = m + I.R + ups;
+        const long long am = mend < 0 ? -mend : mend;
+        if (am < (1LL << 52) || am >= (1LL << 53)) {
+          wok = false;
+          break;
+        }
+        x = ldexp((double)mend, I.k - 53);
+      }
+      if (wok && !(x == x)) wok = false;  // a NaN phase takes the serial replay
+      if (wok) {
+        look_store(inclw + b, __double_as_longlong(x));
+      } else {
+        __hip_atomic_store(W.status + p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        look_store(inclw + b, kInclFail);
+      }
+      s_ok = wok ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  if (s_ok) {
+    // exact values of every step: runs from their exact starts, single steps
+    // from the walk; staged in LDS and stored coalesced
+    long long P = carry;
+    int item = item_base - 1, tcount = tie_base;
+    int c_item = -1, c_k = 0, c_tf = 0;
+    long long c_m = 0;
+    double outv[kFoldPerThread];
+#pragma unroll
+    for (int j = 0; j < kFoldPerThread; j++) {
+      outv[j] = 0.0;
+      if (key[j] == -2) continue;
+      if (head_mask & (1 << j)) {
+        item++;
+        P = 0;
+      }
+      P += r[j];
+      if (tie_mask & (1 << j)) tcount++;
+      if (key[j] >= 0) {
+        if (item != c_item) {
+          c_item = item;
+          c_m = s_items[item].m_start;
+          c_k = s_items[item].k;
+          c_tf = s_items[item].tie_first;
+        }
+        long long ups = 0;
+        for (int q = c_tf; q < tcount; q++) ups += s_ties[q].up;
+        outv[j] = ldexp((double)(c_m + P + ups), c_k - 53);
+      } else {
+        outv[j] = s_x[item];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kFoldPerThread; j++) sd[fold_lds_index(tid * kFoldPerThread + j)] = outv[j];
+    __syncthreads();
+    double *op = A.out[p] + (A.i0 + DIR * (b * kFoldBlock + tid + 1));
+#pragma unroll
+    for (int q = 0; q < kFoldPerThread; q++) {
+      const int s = b * kFoldBlock + tid + q * kFoldThreads;
+      if (s < A.nsteps) op[DIR * q * kFoldThreads] = sd[fold_lds_index(tid + q * kFoldThreads)];
+    }
+  }
+}
+
+// serial replay of the phases whose single-pass fold failed
+__global__ __launch_bounds__(kFoldThreads) void k_fold_replay(FoldArgs A, FoldWork W) {
+  __shared__ double sd[kFoldLds];
+  if (W.status[blockIdx.x]) fold_serial_replay(A, W, blockIdx.x, sd);
+}
+
 int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, int nblk_cap, int dir, int nsteps,
-               hipStream_t s) {
+               int mode, hipStream_t s) {
   if (nsteps <= 0) return 0;
   FoldArgs A;
   for (int p = 0; p < 3; p++) {
@@ -1002,6 +1388,12 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
   if (dir > 0) hipLaunchKernelGGL(k_fold_sums<1>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   else hipLaunchKernelGGL(k_fold_sums<-1>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_plan, dim3(3), dim3(1024), 0, s, A, W);
+  if (mode == 0) {
+    if (dir > 0) hipLaunchKernelGGL(k_fold_resolve<1>, dim3(3 * A.nblk), dim3(kFoldThreads), 0, s, A, W);
+    else hipLaunchKernelGGL(k_fold_resolve<-1>, dim3(3 * A.nblk), dim3(kFoldThreads), 0, s, A, W);
+    hipLaunchKernelGGL(k_fold_replay, dim3(3), dim3(kFoldThreads), 0, s, A, W);
+    return 0;
+  }
   if (dir > 0) hipLaunchKernelGGL((k_fold_local<false, 1>), dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   else hipLaunchKernelGGL((k_fold_local<false, -1>), dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_chain, dim3(3), dim3(kChainThreads), 0, s, A, W);

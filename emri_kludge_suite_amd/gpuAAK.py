@@ -263,8 +263,14 @@ class GPUAAK:
         return (float(like[0]), float(like[1])), spec.reshape(2, n // 2 + 1), bool(fused[0])
 
     def force_fold_replay(self, on=True):
-        """Route every later phase fold through the serial replay (test hook)."""
-        _lib.check(_lib.lib().aak_test_force_fold_replay(self._h, int(bool(on))), "force_fold_replay")
+        """Route every later phase fold through the serial replay (test hook);
+        on=2: the middle block of each phase fails after its look-back."""
+        mode = 2 if on == 2 else int(bool(on))
+        _lib.check(_lib.lib().aak_test_force_fold_replay(self._h, mode), "force_fold_replay")
+
+    def fold_mode(self, mode):
+        """0: single-pass look-back fold (default); 1: three-pass fold (test hook)."""
+        _lib.check(_lib.lib().aak_test_fold_mode(self._h, int(mode)), "fold_mode")
 
     def dense_arrays(self):
         """Device dense arrays of the last waveform (test hook)."""

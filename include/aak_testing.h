@@ -54,10 +54,20 @@ struct aak_handle_s;
 int aak_test_inner_product(struct aak_handle_s *h, const double *hI, const double *hII, double *like, double *spec,
                            int *fused);
 
-/* on != 0: every phase fold of later waveforms on this handle skips the
+/* on = 1: every phase fold of later waveforms on this handle skips the
  * parallel resolution and takes the serial replay (the fallback used when the
- * fold's bookkeeping overflows), so that path can be checked. */
+ * fold's bookkeeping overflows), so that path can be checked.  on = 2 (the
+ * single-pass fold only): the middle block of every phase fails its own
+ * check after its look-back, so the failure reaches the blocks after it and
+ * the phase is replayed once every block has finished.  0: off. */
 int aak_test_force_fold_replay(struct aak_handle_s *h, int on);
+
+/* Form of the exact phase fold for later waveforms on this handle: 0 (the
+ * default) one pass in which every block looks back over the blocks before
+ * it for its exact start, 1 three passes (classification, a one-workgroup
+ * chain over all blocks, write).  Both reproduce the sequential FP64 sum of
+ * AAK.cc:244-264 bit for bit. */
+int aak_test_fold_mode(struct aak_handle_s *h, int mode);
 
 /* Copies the dense arrays of the last aak_gpu_gen_AAK to the host:
  * out[10*(length+1)] = e | v | M | S | gimdot | Phidot | alpdot | gim | Phi | alp. */
