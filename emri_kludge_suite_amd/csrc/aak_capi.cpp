@@ -187,9 +187,9 @@ struct Lane {
   double *d_dense = nullptr;  // e, v, M, S, rate[3], phase[3], amp: 11 x (N+1), then nu0
   double *d_h = nullptr;      // hI | hII, N+2 samples each at channel stride h_stride(N)
   double *d_t = nullptr;      // [N+2]
+  int *d_guardw = nullptr;    // [kGuardGrid + 1] (DenseBuffers::guard_work)
   int *d_flags = nullptr;     // [8]: non-positive Phidot count, fold fallbacks, ... (DenseBuffers::flags)
   bool flags_dirty = false;   // a waveform wrote d_flags and no final reduction has zeroed them since
-  int *d_guard = nullptr;
   FoldWork fw{};
   int fold_nblk_cap = 0;
   void *d_foldmem = nullptr;
@@ -300,7 +300,7 @@ struct aak_handle_s {
     B.h2 = ln.d_h + h_stride(N);
     B.t = ln.d_t;
     B.flags = ln.d_flags;
-    B.guard_blk = ln.d_guard;
+    B.guard_work = ln.d_guardw;
     B.ms_coef = ln.d_ms_coef;
     B.ms_p = ln.d_ms_p;
     B.ms_slot = ln.d_ms_slot;
@@ -325,8 +325,8 @@ static int alloc_lane(aak_handle h, Lane &L) {
   HIPCHK(hipMalloc(&L.d_h, sizeof(double) * 2 * h_stride(N)));
   HIPCHK(hipMalloc(&L.d_t, sizeof(double) * ((size_t)N + 2)));
   HIPCHK(hipMalloc(&L.d_flags, sizeof(int) * 8));
+  HIPCHK(hipMalloc(&L.d_guardw, sizeof(int) * (kGuardGrid + 1)));
   HIPCHK(hipMemset(L.d_flags, 0, sizeof(int) * 8));
-  HIPCHK(hipMalloc(&L.d_guard, sizeof(int) * ((size_t)N / 4096 + 4)));
   HIPCHK(hipMalloc(&L.d_ms_p, sizeof(int) * ((size_t)N / 256 + 4)));
   HIPCHK(hipMalloc(&L.d_ms_slot, sizeof(int) * ((size_t)N / 256 + 4)));
   HIPCHK(hipMalloc(&L.d_ms_list, sizeof(int) * ((size_t)N / 256 + 4)));
@@ -346,7 +346,7 @@ static int alloc_lane(aak_handle h, Lane &L) {
   const size_t o_items = carve(sizeof(FoldItem) * 3 * (size_t)nblk * kFoldMaxItems);
   const size_t o_ties = carve(sizeof(FoldTie) * 3 * (size_t)nblk * kFoldMaxTies);
   const size_t o_st = carve(sizeof(int) * 3);
-  const size_t o_la = carve(sizeof(long long) * 3 * nblk), o_li = carve(sizeof(long long) * 3 * nblk);
+  const size_t o_la = carve(sizeof(long long) * 6 * nblk), o_li = carve(sizeof(long long) * 3 * nblk);
   const size_t o_lc = carve(sizeof(unsigned) * 4);
   HIPCHK(hipMalloc(&L.d_foldmem, bytes));
   char *base = (char *)L.d_foldmem;
@@ -409,7 +409,7 @@ static void free_lane(Lane &L) {
   if (L.g_like) (void)hipGraphDestroy(L.g_like);
   if (L.plan) rocfft_plan_destroy(L.plan);
   if (L.exec) rocfft_execution_info_destroy(L.exec);
-  void *ptrs[] = {L.d_table[0], L.d_table[1], L.d_dense, L.d_h, L.d_t, L.d_flags, L.d_guard, L.d_foldmem,
+  void *ptrs[] = {L.d_table[0], L.d_table[1], L.d_dense, L.d_h, L.d_t, L.d_flags, L.d_guardw, L.d_foldmem,
                   L.d_hf, L.d_fy, L.d_partials, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_slot,
                   L.d_ms_list, L.d_ms_split, L.d_ms_eref};
   for (void *p : ptrs)

@@ -129,6 +129,7 @@ struct DenseParams {
 };
 
 // Exact-order parallel fold bookkeeping.
+constexpr int kGuardGrid = 64;  // workgroups of the Phidot guard
 constexpr int kFoldThreads = 256;
 constexpr int kFoldPerThread = 16;
 constexpr int kFoldBlock = kFoldThreads * kFoldPerThread;
@@ -160,7 +161,8 @@ struct FoldWork {
   FoldItem *items;                      // [3][nblk][kFoldMaxItems]
   FoldTie *ties;                        // [3][nblk][kFoldMaxTies]
   int *status;                          // [3]: 0 ok, 1 overflow/fallback
-  long long *look_agg, *look_incl;      // [3][nblk]: run total of a one-run block; exact end value (bits)
+  long long *look_agg;                  // [3][nblk][2]: action of a one-run block (even, odd start)
+  long long *look_incl;                 // [3][nblk]: exact end value (bits)
   unsigned *look_ctr;                   // [1]: block tickets of the single-pass fold
   int *fallbacks;                       // serial replays since the last counter reset
   int force_replay;                     // test hook: 1 every phase takes the serial replay, 2 the middle block fails
@@ -176,10 +178,11 @@ struct DenseBuffers {
   double *nu0;                     // [1] radial frequency at sample 0 from the PN maps
   double *h, *h2;                  // time domain hI, hII: N+2 samples each, h2 = h + h_stride(N)
   double *t;                       // [N+2]
+  int *guard_work;                 // [kGuardGrid + 1]: last good index of each guard segment (-2: not yet), ticket
   int *flags;                      // [8]: [0] non-positive Phidot count, [1] fold replays, [2] per-sample mode-sum
                                    // blocks, [3] non-positive Phidot at t = 0 (waveform rejected), [4] expansion
-                                   // blocks split per 256 samples, [5..7] unused
-  int *guard_blk;                  // [guard blocks]: last positive-Phidot index per block
+                                   // blocks split per 256 samples, [5] 0x7fffffff minus the first non-positive
+                                   // Phidot sample (0: none), [6..7] unused
   double *ms_coef;                 // per slot nmodes x 3 x (kTaylorMax+1) doubles; order p uses [nmodes][3][p+1] (float in mixed mode); slots: expansion blocks, then 256-sample blocks of split expansions
   int *ms_p;                       // [256-sample blocks] Taylor order (-1: per-sample path)
   int *ms_slot;                    // [256-sample blocks] coefficient slot of the block's expansion
@@ -198,7 +201,6 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
                int mode, hipStream_t s);
 void launch_fold_fill(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
-int guard_blocks(const DenseParams &P);
 int modesum_blocks(const DenseParams &P);  // mode-sum blocks (256 samples each)
 int modesum_slots(const DenseParams &P);   // coefficient slots: expansion blocks + 256-sample blocks
 void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s);

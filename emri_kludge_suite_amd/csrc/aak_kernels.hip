@@ -80,6 +80,8 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   __shared__ double s_c[16][kStageCap];
   __shared__ int s_lo, s_cnt;
   const int tid = threadIdx.x;
+  // the guard's segment words and ticket start cleared for each waveform
+  if (blockIdx.x == 0 && tid <= kGuardGrid) B.guard_work[tid] = (tid < kGuardGrid) ? -2 : 0;
   const int i_lo = blockIdx.x * kRatesThreads;
   if (i_lo > P.N) return;
   // likelihoods read the rates up to the last folded step and e and the
@@ -161,7 +163,10 @@ __global__ __launch_bounds__(kRatesThreads) void k_rates(DenseParams P, DenseBuf
   B.amp[i] = (cr * cr) * P.zeta;
   if (i == 0) B.nu0[0] = R / T / (2. * kPi * M);
   // the reference's test is Phidot <= 0 (AAK.cc:254): a NaN rate is kept as it is
-  if (i > P.i0 && i <= P.guard_last && phidot <= 0.) atomicAdd(&B.flags[0], 1);
+  if (i > P.i0 && i <= P.guard_last && phidot <= 0.) {
+    atomicAdd(&B.flags[0], 1);
+    atomicMax(&B.flags[5], 0x7fffffff - i);  // the first such sample, for the guard
+  }
   // a non-positive rate at t = 0 has no earlier value to take: the waveform is
   // rejected (flags[3]; the likelihood of such a template reads NaN)
   if (i == P.i0 && phidot <= 0.) B.flags[3] = 1;
@@ -176,102 +181,117 @@ void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
 // (AAK.cc:254): sample i in (i0, guard_last] with Phidot <= 0 gets the rate of
 // the last sample j in (i0, i] whose rate is not <= 0 (a NaN rate counts as
 // such a sample and is kept, as the reference's test leaves it), or of i0 if
-// there is none.  That source index is a running maximum, computed as a two-pass
-// parallel scan; both passes return at once when k_rates counted no such
-// sample, so they are always enqueued and need no host round trip.
-constexpr int kGuardThreads = 256;
-constexpr int kGuardPer = 16;
-constexpr int kGuardBlock = kGuardThreads * kGuardPer;
+// there is none.  Every sample before the first non-positive one (recorded
+// by k_rates) is such a sample, so only [first, guard_last] is walked, split
+// into kGuardGrid segments.  Each workgroup takes a ticket, finds its
+// segment's last good index and publishes it at once; its carry is the
+// maximum over the segments before it, which never wait on anything, so the
+// waits are short and cannot form a cycle.  The fix then runs in chunks of
+// kGuardThreads x kGuardPer with an inclusive max-scan of the last good index
+// over the threads.  Good samples are never written, so a replaced sample
+// reads a final value.  The kernel returns at once when k_rates counted no
+// such sample, so it is always enqueued and needs no host round trip.
+constexpr int kGuardThreads = 1024;
+constexpr int kGuardPer = 32;  // consecutive samples per thread and chunk
+constexpr int kGuardSpinCap = 1 << 22;
 
 __device__ inline bool phidot_ok(double r) { return !(r <= 0.); }
 
-// Both passes walk the guard blocks with a fixed grid of kGuardGrid
-// workgroups, so that the common case (nothing to fix) costs a small launch.
-constexpr int kGuardGrid = 64;
-
-__global__ __launch_bounds__(kGuardThreads) void k_guard_last(DenseParams P, DenseBuffers B, int nb) {
+__global__ __launch_bounds__(kGuardThreads) void k_guard(DenseParams P, DenseBuffers B) {
   if (B.flags[0] == 0) return;
-  __shared__ int s_m[kGuardThreads];
-  const double *r = B.rate[1];
-  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
-    const int base = P.i0 + 1 + blk * kGuardBlock + threadIdx.x * kGuardPer;
-    int last = -1;
-    for (int q = 0; q < kGuardPer; q++) {
-      const int i = base + q;
-      if (i <= P.guard_last && phidot_ok(r[i])) last = i;
-    }
-    s_m[threadIdx.x] = last;
-    __syncthreads();
-    for (int o = kGuardThreads / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) B.guard_blk[blk] = s_m[0];
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(kGuardThreads) void k_guard_fix(DenseParams P, DenseBuffers B, int nb) {
-  if (B.flags[0] == 0) return;
-  __shared__ int s_m[kGuardThreads];
-  __shared__ int s_carry;
+  __shared__ int s_w[kGuardThreads / 64];
+  __shared__ int s_t, s_carry;
   double *r = B.rate[1];
-  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
-    // last good index before this block
-    int c = P.i0;
-    for (int b = threadIdx.x; b < blk; b += kGuardThreads) c = max(c, B.guard_blk[b]);
-    s_m[threadIdx.x] = c;
-    __syncthreads();
-    for (int o = kGuardThreads / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
-      __syncthreads();
+  int *gw = B.guard_work;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_t = atomicAdd(gw + kGuardGrid, 1);
+  __syncthreads();
+  const int t = s_t;
+  const int first = 0x7fffffff - B.flags[5];
+  const int n = P.guard_last - first + 1;
+  const int per = ((n + kGuardGrid - 1) / kGuardGrid + kGuardThreads - 1) / kGuardThreads * kGuardThreads;
+  const int lo = first + t * per, hi = min(P.guard_last, lo + per - 1);
+  // the segment's last good index, published at once
+  int last = -1;
+  for (int i = lo + tid; i <= hi; i += kGuardThreads)
+    if (phidot_ok(r[i])) last = i;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) last = max(last, __shfl_xor(last, o, 64));
+  if (lane == 0) s_w[wave] = last;
+  __syncthreads();
+  if (wave == 0) {
+    int agg = -1;
+    for (int w = 0; w < kGuardThreads / 64; w++) agg = max(agg, s_w[w]);
+    if (lane == 0) __hip_atomic_store(gw + t, agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // carry: the last good index of the segments before this one
+    int c = first - 1;
+    bool late = false;
+    if (lane < t) {
+      int v, spins = 0;
+      while ((v = __hip_atomic_load(gw + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == -2) {
+        if (++spins > kGuardSpinCap) {
+          late = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      if (!late) c = max(c, v);
     }
-    if (threadIdx.x == 0) s_carry = s_m[0];
-    __syncthreads();
-    const int carry = s_carry;
-    __syncthreads();
-    const int base = P.i0 + 1 + blk * kGuardBlock + threadIdx.x * kGuardPer;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o, 64));
+    const bool any_late = __ballot(late) != 0;
+    if (lane == 0) {
+      s_carry = c;
+      if (any_late) B.flags[3] = 1;  // a wait past the cap rejects the waveform
+    }
+  }
+  __syncthreads();
+  int carry = s_carry;
+  for (int base = lo; base <= hi; base += kGuardThreads * kGuardPer) {
+    const int i0 = base + tid * kGuardPer;
     double v[kGuardPer];
-    int last = -1;
+    int lst = -1;
+#pragma unroll
     for (int q = 0; q < kGuardPer; q++) {
-      const int i = base + q;
-      v[q] = (i <= P.guard_last) ? r[i] : 1.0;
-      if (i <= P.guard_last && phidot_ok(v[q])) last = i;
+      const int i = i0 + q;
+      v[q] = (i <= hi) ? r[i] : 1.0;
+      if (i <= hi && phidot_ok(v[q])) lst = i;
     }
-    // inclusive max-scan of the per-thread last good index
-    s_m[threadIdx.x] = last;
-    __syncthreads();
-    for (int o = 1; o < kGuardThreads; o <<= 1) {
-      const int add = (threadIdx.x >= o) ? s_m[threadIdx.x - o] : -1;
-      __syncthreads();
-      s_m[threadIdx.x] = max(s_m[threadIdx.x], add);
-      __syncthreads();
+    // inclusive max-scan of the last good index over the threads
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(lst, o, 64);
+      if (lane >= o) lst = max(lst, u);
     }
-    int src = max(carry, threadIdx.x > 0 ? s_m[threadIdx.x - 1] : -1);
+    if (lane == 63) s_w[wave] = lst;
     __syncthreads();
+    int before = carry, chunk = carry;
+    for (int w = 0; w < kGuardThreads / 64; w++) {
+      const int x = s_w[w];
+      if (w < wave) before = max(before, x);
+      chunk = max(chunk, x);
+    }
+    // last good index before this thread's samples; its value is read only
+    // when the thread's first sample needs it, later ones come from registers
+    int src = max(before, __shfl_up(lst, 1, 64));
+    if (lane == 0) src = before;
+    double srcv = (i0 <= hi && !phidot_ok(v[0])) ? r[src] : 0.0;
+#pragma unroll
     for (int q = 0; q < kGuardPer; q++) {
-      const int i = base + q;
-      if (i > P.guard_last) break;
-      if (phidot_ok(v[q])) {
-        src = i;
-      } else {
-        r[i] = r[src];
+      const int i = i0 + q;
+      if (i <= hi) {
+        if (phidot_ok(v[q])) srcv = v[q];
+        else r[i] = srcv;
       }
     }
+    carry = chunk;
+    __syncthreads();
   }
-}
-
-int guard_blocks(const DenseParams &P) {
-  const int n = P.guard_last - P.i0;
-  return n > 0 ? (n + kGuardBlock - 1) / kGuardBlock : 0;
 }
 
 void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
-  const int nb = guard_blocks(P);
-  if (nb == 0) return;
-  const int grid = std::min(nb, kGuardGrid);
-  hipLaunchKernelGGL(k_guard_last, dim3(grid), dim3(kGuardThreads), 0, s, P, B, nb);
-  hipLaunchKernelGGL(k_guard_fix, dim3(grid), dim3(kGuardThreads), 0, s, P, B, nb);
+  if (P.guard_last <= P.i0) return;
+  hipLaunchKernelGGL(k_guard, dim3(kGuardGrid), dim3(kGuardThreads), 0, s, P, B);
 }
 
 // ------------------------------------------------------------------ folds
@@ -489,7 +509,8 @@ __global__ __launch_bounds__(1024) void k_fold_plan(FoldArgs A, FoldWork W) {
   }
   // look-back state of the single-pass resolve
   for (int b = tid; b < A.nblk; b += 1024) {
-    W.look_agg[p * A.nblk + b] = kAggEmpty;
+    W.look_agg[2 * (p * A.nblk + b)] = kAggEmpty;
+    W.look_agg[2 * (p * A.nblk + b) + 1] = kAggEmpty;
     W.look_incl[p * A.nblk + b] = kInclEmpty;
   }
   if (p == 0 && tid == 0) *W.look_ctr = 0;
@@ -1031,6 +1052,13 @@ __device__ inline void look_store(long long *v, long long x) {
   __hip_atomic_store(v, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// a 64-bit value of lane q, read by every lane of the wave
+__device__ inline long long lane_read64(long long v, int q) {
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffLL), q);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), q);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // binade key of an exact value, as binade_key with E = 0 (-1 for 0, inf, NaN)
 __device__ inline int exact_key(double x) {
   if (!(fabs(x) > 0.0) || !(fabs(x) < __builtin_inf())) return -1;
@@ -1040,29 +1068,43 @@ __device__ inline int exact_key(double x) {
 }
 
 // Look-back state of block j: 2 end value known (or failed), 1 pure with its
-// total published, 0 neither yet (or not pure).
+// action published, 0 neither yet (or not pure).  The action of a pure block
+// with ties on its grid depends on the parity of the value before it (each
+// tie rounds to even), so it is published as two totals: a[q] is added to m
+// when m is even (q = 0) or odd (q = 1).  Such actions compose: m -> m +
+// F[m & 1] followed by m -> m + G[m & 1] is m -> m + H[m & 1] with
+// H[q] = F[q] + G[(q + F[q]) & 1].
 struct LookState {
   int st;
-  long long v;  // st 2: end value bits; st 1: the run total
+  long long v;       // st 2: end value bits
+  long long a0, a1;  // st 1: the totals for an even and an odd start
 };
 __device__ inline LookState look_state(const long long *agg, const long long *incl, int j) {
-  const long long iw = look_load(incl + j);  // both loads in flight at once
-  const long long aw = look_load(agg + j);
-  if (iw != kInclEmpty) return {2, iw};
-  if (aw != kAggEmpty && aw != kAggNone) return {1, aw};
-  return {0, 0};
+  const long long iw = look_load(incl + j);  // all loads in flight at once
+  const long long w0 = look_load(agg + 2 * j), w1 = look_load(agg + 2 * j + 1);
+  if (iw != kInclEmpty) return {2, iw, 0, 0};
+  if (w0 != kAggEmpty && w0 != kAggNone && w1 != kAggEmpty) return {1, 0, w0, w1};
+  return {0, 0, 0, 0};
+}
+
+// (far then near) as one action
+__device__ inline void act_compose(long long f0, long long f1, long long &n0, long long &n1) {
+  const long long h0 = f0 + ((f0 & 1) ? n1 : n0);
+  const long long h1 = f1 + ((f1 & 1) ? n0 : n1);
+  n0 = h0;
+  n1 = h1;
 }
 
 // The value before a block from an exact end value w and the pure blocks
-// combined after it (key K, total acc; K < 0: none).
-__device__ inline bool look_finish(long long w, int K, long long acc, double &x_out) {
+// combined after it (key K, action c0/c1; K < 0: none).
+__device__ inline bool look_finish(long long w, int K, long long c0, long long c1, double &x_out) {
   if (w == kInclFail) return false;
   double x = __longlong_as_double(w);
   if (K >= 0) {
     if (exact_key(x) != K) return false;
     const int k = (K >> 1) - 4096;
     long long m = (long long)ldexp(x, 53 - k);
-    m += acc;
+    m += (m & 1) ? c1 : c0;
     const long long am = m < 0 ? -m : m;
     if (am < (1LL << 52) || am >= (1LL << 53)) return false;
     x = ldexp((double)m, k - 53);
@@ -1076,17 +1118,17 @@ __device__ inline bool look_finish(long long w, int K, long long acc, double &x_
 // cap).
 __device__ inline bool fold_look_back(const FoldArgs &A, const FoldWork &W, int p, int b, double E, double &x_out) {
   const int lane = threadIdx.x & 63;
-  const long long *agg = W.look_agg + (size_t)p * A.nblk;
+  const long long *agg = W.look_agg + 2 * (size_t)p * A.nblk;
   const long long *incl = W.look_incl + (size_t)p * A.nblk;
   const double *xs = W.xstart + (size_t)p * A.nblk;
-  int j = b - 1;      // nearest block not yet combined
-  int K = -1;         // binade key of the combined pure blocks
-  long long acc = 0;  // their summed totals
+  int j = b - 1;              // nearest block not yet combined
+  int K = -1;                 // binade key of the combined pure blocks
+  long long c0 = 0, c1 = 0;   // their combined action
   int spins = 0;
   for (;;) {
     const int jb = j - lane;
     // block -1 stands for the start value x0
-    LookState S = {2, __double_as_longlong(A.x0[p])};
+    LookState S = {2, __double_as_longlong(A.x0[p]), 0, 0};
     int key = -1;
     if (jb >= 0) {
       S = look_state(agg, incl, jb);
@@ -1098,10 +1140,19 @@ __device__ inline bool fold_look_back(const FoldArgs &A, const FoldWork &W, int 
     const bool ext = S.st == 1 && key == Kc && key >= 0;
     const unsigned long long mask = __ballot(ext);
     const int s = (~mask) ? __builtin_ctzll(~mask) : 64;
-    long long r = (lane < s) ? S.v : 0;
+    // ordered composition of lanes 0..s-1 (lane 0 nearest): after the step
+    // with offset o, lane l (a multiple of 2o) holds lanes l..l+2o-1
+    long long n0 = (lane < s) ? S.a0 : 0, n1 = (lane < s) ? S.a1 : 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
-    acc += r;
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long f0 = __shfl_down(n0, o, 64), f1 = __shfl_down(n1, o, 64);
+      if (lane + o < 64) act_compose(f0, f1, n0, n1);
+    }
+    {
+      // the window lies before the blocks combined so far
+      long long w0 = __shfl(n0, 0, 64), w1 = __shfl(n1, 0, 64);
+      act_compose(w0, w1, c0, c1);
+    }
     if (s > 0) K = Kc;
     if (s == 64) {
       j -= 64;
@@ -1109,9 +1160,9 @@ __device__ inline bool fold_look_back(const FoldArgs &A, const FoldWork &W, int 
     }
     const int js = j - s;
     const int ss = __shfl(S.st, s, 64);
-    if (ss == 2) return look_finish(__shfl(S.v, s, 64), K, acc, x_out);
+    if (ss == 2) return look_finish(__shfl(S.v, s, 64), K, c0, c1, x_out);
     // not yet published, or pure on another grid: wait for a change of state;
-    // an end value is used at once, a newly published total is combined by
+    // an end value is used at once, a newly published action is combined by
     // the next window, which starts at that block
     LookState T;
     do {
@@ -1119,7 +1170,7 @@ __device__ inline bool fold_look_back(const FoldArgs &A, const FoldWork &W, int 
       __builtin_amdgcn_s_sleep(1);
       T = look_state(agg, incl, js);
     } while (T.st == ss);
-    if (T.st == 2) return look_finish(T.v, K, acc, x_out);
+    if (T.st == 2) return look_finish(T.v, K, c0, c1, x_out);
     j = js;
   }
 }
@@ -1143,7 +1194,7 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
   }
   __syncthreads();
   const int p = s_pb[0], b = s_pb[1];
-  long long *aggw = W.look_agg + (size_t)p * A.nblk;
+  long long *aggw = W.look_agg + 2 * (size_t)p * A.nblk;
   long long *inclw = W.look_incl + (size_t)p * A.nblk;
   // entry: a phase already marked (the replay hook) skips to the end
   if (tid == 0) s_ok = __hip_atomic_load(W.status + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
@@ -1265,51 +1316,84 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
     if (tid == 0) {
       s_pb[0] = nheads_tot;
       s_pb[1] = nties_tot;
-      // a block that is one run without ties publishes its total at once
-      const bool pure = ok && nheads_tot == 1 && nties_tot == 0 && s_items[0].kind == 0;
-      look_store(aggw + b, pure ? s_items[0].R : kAggNone);
+      // a block that is one run publishes its action at once: the run total
+      // plus the ties rounded up, for an even and for an odd start
+      const bool pure = ok && nheads_tot == 1 && s_items[0].kind == 0;
+      long long a0 = kAggNone, a1 = kAggNone;
+      if (pure) {
+        long long u0 = 0, u1 = 0;
+        for (int t = 0; t < nties_tot; t++) {
+          const long long P = s_ties[t].P;
+          u0 += (P + u0) & 1;
+          u1 += (1 + P + u1) & 1;
+        }
+        a0 = s_items[0].R + u0;
+        a1 = s_items[0].R + u1;
+      }
+      look_store(aggw + 2 * b + 1, a1);
+      look_store(aggw + 2 * b, a0);
     }
     __syncthreads();
   }
-  // wave 0: look back, then the block's own walk; publishes the end value
+  // wave 0: look back, then the block's own walk; publishes the end value.
+  // Lane q holds item q, with its action on an even and an odd start (its
+  // ties round to even), so the walk reads uniform values from registers and
+  // the end value is published before the ties are resolved for the write.
   if (tid < 64) {
+    const int lane = tid, ni = ok ? s_pb[0] : 0, nt = ok ? s_pb[1] : 0;
+    int it_kind = 1, it_k = 0, it_sgn = 0, it_t0 = 0, it_t1 = 0;
+    long long it_d = 0, it_a0 = 0, it_a1 = 0;
+    if (lane < ni) {
+      const FoldItem &I = s_items[lane];
+      it_kind = I.kind;
+      it_k = I.k;
+      it_sgn = I.sgn;
+      it_d = __double_as_longlong(I.d);
+      if (I.kind == 0) {
+        it_t0 = I.tie_first;
+        it_t1 = (lane + 1 < ni) ? s_items[lane + 1].tie_first : nt;
+        long long u0 = 0, u1 = 0;
+        for (int t = it_t0; t < it_t1; t++) {
+          const long long P = s_ties[t].P;
+          u0 += (P + u0) & 1;
+          u1 += (1 + P + u1) & 1;
+        }
+        it_a0 = I.R + u0;
+        it_a1 = I.R + u1;
+      }
+    }
     double x = 0.0;
     bool wok = ok && (b == 0 ? (x = A.x0[p], true) : fold_look_back(A, W, p, b, E, x));
     if (W.force_replay == 2 && b == A.nblk / 2) wok = false;  // test hook
-    if (tid == 0) {
-      const int ni = s_pb[0], nt = s_pb[1];
-      if (wok && b == 0) A.out[p][A.i0] = x;
-      for (int q = 0; wok && q < ni; q++) {
-        FoldItem &I = s_items[q];
-        if (I.kind == 1) {
-          x = x + I.d;
-          s_x[q] = x;
-          continue;
-        }
-        int kx;
-        frexp(x, &kx);
-        if (kx != I.k || (x > 0.0 ? 1 : -1) != I.sgn) {
-          wok = false;
-          break;
-        }
-        const long long m = (long long)ldexp(x, 53 - I.k);
-        I.m_start = m;
-        const int t1 = (q + 1 < ni) ? s_items[q + 1].tie_first : nt;
-        long long ups = 0;
-        for (int t = I.tie_first; t < t1; t++) {
-          const int up = (int)((m + s_ties[t].P + ups) & 1LL);
-          s_ties[t].up = up;
-          ups += up;
-        }
-        const long long mend = m + I.R + ups;
-        const long long am = mend < 0 ? -mend : mend;
-        if (am < (1LL << 52) || am >= (1LL << 53)) {
-          wok = false;
-          break;
-        }
-        x = ldexp((double)mend, I.k - 53);
+    if (wok && b == 0 && lane == 0) A.out[p][A.i0] = x;
+    long long my_m = 0;
+    double my_x = 0.0;
+    for (int q = 0; wok && q < ni; q++) {
+      const int kind = __builtin_amdgcn_readlane(it_kind, q);
+      if (kind == 1) {
+        x = x + __longlong_as_double(lane_read64(it_d, q));
+        if (lane == q) my_x = x;
+        continue;
       }
-      if (wok && !(x == x)) wok = false;  // a NaN phase takes the serial replay
+      const int k = __builtin_amdgcn_readlane(it_k, q), sgn = __builtin_amdgcn_readlane(it_sgn, q);
+      int kx;
+      frexp(x, &kx);
+      if (kx != k || (x > 0.0 ? 1 : -1) != sgn) {
+        wok = false;
+        break;
+      }
+      long long m = (long long)ldexp(x, 53 - k);
+      if (lane == q) my_m = m;
+      m += (m & 1) ? lane_read64(it_a1, q) : lane_read64(it_a0, q);
+      const long long am = m < 0 ? -m : m;
+      if (am < (1LL << 52) || am >= (1LL << 53)) {
+        wok = false;
+        break;
+      }
+      x = ldexp((double)m, k - 53);
+    }
+    if (wok && !(x == x)) wok = false;  // a NaN phase takes the serial replay
+    if (lane == 0) {
       if (wok) {
         look_store(inclw + b, __double_as_longlong(x));
       } else {
@@ -1317,6 +1401,20 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
         look_store(inclw + b, kInclFail);
       }
       s_ok = wok ? 1 : 0;
+    }
+    if (wok && lane < ni) {
+      // exact start of each run and the rounding of its ties, for the write
+      if (it_kind == 1) {
+        s_x[lane] = my_x;
+      } else {
+        s_items[lane].m_start = my_m;
+        long long ups = 0;
+        for (int t = it_t0; t < it_t1; t++) {
+          const int up = (int)((my_m + s_ties[t].P + ups) & 1LL);
+          s_ties[t].up = up;
+          ups += up;
+        }
+      }
     }
   }
   __syncthreads();
