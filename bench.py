@@ -404,7 +404,7 @@ def main():
                     "fp64_batched_over_single_fp64": value_64 / value_single},
         "roofline": {"bound": "valu", "achieved": exec_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": exec_tf / FP32_PEAK_TFLOPS if exec_tf else None, "traffic": traffic,
-                     "kernel": "mode-sum stage, mixed precision (k_modesum_setup + k_modesum + k_modesum_direct)",
+                     "kernel": "mode-sum stage, mixed precision (k_modesum_setup + k_modesum_split + k_modesum + k_modesum_direct)",
                      "kernel_ms": ms_mx,
                      "kernel_timing": "HIP events around the stage on its stream; one profiled pass over the batch on one lane",
                      "flop_per_launch": exec_flop,

@@ -68,6 +68,7 @@ def lib():
         "aak_test_inner_product": (_i, [C.c_void_p, _pd, _pd, _pd, _pd, _pi]),
         "aak_test_force_fold_replay": (_i, [C.c_void_p, _i]),
         "aak_test_fold_mode": (_i, [C.c_void_p, _i]),
+        "aak_test_phidot_guard": (_i, [_pd, _i, _i, _i]),
         "aak_get_batch_host_ms": (_i, [C.c_void_p, _pd, _pd]),
         "aak_set_profiling": (_i, [C.c_void_p, _i]),
         "aak_get_timings": (_i, [C.c_void_p, _pd, _i]),

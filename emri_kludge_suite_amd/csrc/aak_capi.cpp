@@ -1478,6 +1478,54 @@ int aak_test_knot_eval(const double *params, double T_fit, int length, double dt
   return 0;
 }
 
+int aak_test_phidot_guard(double *rate, int n, int i0, int guard_last) {
+  if (!rate || n < 1 || i0 < 0 || guard_last >= n || i0 > guard_last) return fail(-2, "guard range outside the array");
+  // the count and the first non-positive sample, as k_rates records them
+  int count = 0, first = -1;
+  for (int i = i0 + 1; i <= guard_last; i++)
+    if (rate[i] <= 0.) {
+      if (first < 0) first = i;
+      count++;
+    }
+  int flags[8] = {count, 0, 0, 0, 0, first < 0 ? 0 : 0x7fffffff - first, 0, 0};
+  int gw[kGuardGrid + 1];
+  for (int k = 0; k < kGuardGrid; k++) gw[k] = -2;
+  gw[kGuardGrid] = 0;
+  double *d_rate = nullptr;
+  int *d_flags = nullptr, *d_gw = nullptr;
+  auto release = [&]() {
+    if (d_rate) (void)hipFree(d_rate);
+    if (d_flags) (void)hipFree(d_flags);
+    if (d_gw) (void)hipFree(d_gw);
+  };
+  if (hipMalloc(&d_rate, sizeof(double) * n) != hipSuccess || hipMalloc(&d_flags, sizeof(flags)) != hipSuccess ||
+      hipMalloc(&d_gw, sizeof(gw)) != hipSuccess) {
+    release();
+    return fail(-6, "device allocation failed");
+  }
+  DenseParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.N = n - 1;
+  P.i0 = i0;
+  P.guard_last = guard_last;
+  DenseBuffers B;
+  std::memset(&B, 0, sizeof(B));
+  B.rate[1] = d_rate;
+  B.flags = d_flags;
+  B.guard_work = d_gw;
+  hipError_t st = hipMemcpy(d_rate, rate, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (st == hipSuccess) st = hipMemcpy(d_flags, flags, sizeof(flags), hipMemcpyHostToDevice);
+  if (st == hipSuccess) st = hipMemcpy(d_gw, gw, sizeof(gw), hipMemcpyHostToDevice);
+  if (st == hipSuccess) {
+    launch_phidot_guard(P, B, 0);
+    st = hipGetLastError();
+  }
+  if (st == hipSuccess) st = hipMemcpy(rate, d_rate, sizeof(double) * n, hipMemcpyDeviceToHost);
+  release();
+  if (st != hipSuccess) return fail(-6, hipGetErrorString(st));
+  return count;
+}
+
 int aak_test_bessel_five(int n, double e, double e_bound, double *J) {
   if (n < 1 || n > kMaxModes) return fail(-2, "harmonic outside 1..128");
   if (!(e >= 0.0) || e > e_bound || e_bound > kSeriesMaxE) return fail(-2, "eccentricity outside the series range");

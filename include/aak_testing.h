@@ -69,6 +69,12 @@ int aak_test_force_fold_replay(struct aak_handle_s *h, int on);
  * AAK.cc:244-264 bit for bit. */
 int aak_test_fold_mode(struct aak_handle_s *h, int mode);
 
+/* The Phidot guard (k_guard) alone on rate[0..n): every sample i in
+ * (i0, guard_last] with rate <= 0 takes the rate of the last earlier sample
+ * in (i0, i] that is not <= 0 (NaN counts as such), or of i0 (AAK.cc:254).
+ * rate is rewritten in place; returns the number of replaced samples. */
+int aak_test_phidot_guard(double *rate, int n, int i0, int guard_last);
+
 /* Copies the dense arrays of the last aak_gpu_gen_AAK to the host:
  * out[10*(length+1)] = e | v | M | S | gimdot | Phidot | alpdot | gim | Phi | alp. */
 int aak_test_get_dense(struct aak_handle_s *h, double *out);
