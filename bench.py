@@ -272,16 +272,21 @@ def main():
             g.LikelihoodBatch(P)
         t0 = time.perf_counter()
         host = np.zeros(2)
-        busy = []
+        busy, issue, wait = [], [], []
         for _ in range(args.steps):
             like = g.LikelihoodBatch(P)
             hs, hm = g.batch_host_ms()
             host += (hs, hm)
-            busy.append(g.pool_stats()["busy"])
+            ps = g.pool_stats()
+            busy.append(ps["busy"])
+            issue.append(ps["issue_ms"] / ps["wall_ms"])
+            wait.append(ps["wait_ms"] / ps["wall_ms"])
         elapsed = time.perf_counter() - t0
         assert np.all(np.isfinite(like)), "bench: a template failed"
         host_plan[mode] = {"ms_per_template": host[0] / (args.steps * len(P)), "longest_ms": host[1] / args.steps,
-                           "pool_threads": int(g.pool_stats()["threads"]), "pool_busy_frac": float(np.mean(busy))}
+                           "pool_threads": int(g.pool_stats()["threads"]), "pool_busy_frac": float(np.mean(busy)),
+                           "issue_thread_enqueue_frac": float(np.mean(issue)),
+                           "issue_thread_wait_frac": float(np.mean(wait))}
         # stage times: on one lane (no overlap with another template's kernels)
         # and as they run concurrently on the bench's lanes
         return elapsed, profiled_pass(1), profiled_pass(lanes), like

@@ -259,6 +259,7 @@ struct aak_handle_s {
   double batch_ms[5] = {0, 0, 0, 0, 0};  // stage sums over the last profiled batch
   double batch_host_sum = 0.0, batch_host_max = 0.0;  // host stage ms over the last batch
   double batch_wall_ms = 0.0;                         // wall time of the last batch call
+  double batch_issue_ms = 0.0, batch_wait_ms = 0.0;   // issuing thread: enqueueing, waiting for plans
   int batch_profiled = 0;
   double host_ms = 0.0;
   // batch machinery
@@ -1243,15 +1244,18 @@ int aak_likelihood_batch_nmax(aak_handle h, const double *params, const int *nma
   h->cv_work.notify_all();
   int err_rc = 0;
   std::string err_msg;
-  double host_sum = 0.0, host_max = 0.0;
+  double host_sum = 0.0, host_max = 0.0, issue_ms = 0.0, wait_ms = 0.0;
   for (long long done = 0; done < n; done++) {
     int sidx;
+    const auto t_wait = std::chrono::steady_clock::now();
     {
       std::unique_lock<std::mutex> lk(h->mu);
       h->cv_ready.wait(lk, [&] { return !h->ready_slots.empty(); });
       sidx = h->ready_slots.front();
       h->ready_slots.pop_front();
     }
+    const auto t_issue = std::chrono::steady_clock::now();
+    wait_ms += std::chrono::duration<double, std::milli>(t_issue - t_wait).count();
     RingSlot &slot = *h->ring[(size_t)sidx];
     const long long i = slot.tpl;
     st[i] = slot.rc < 0 ? slot.rc : 0;
@@ -1278,6 +1282,7 @@ int aak_likelihood_batch_nmax(aak_handle h, const double *params, const int *nma
       h->free_slots.push_back(sidx);
     }
     h->cv_work.notify_one();
+    issue_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_issue).count();
   }
   {
     std::lock_guard<std::mutex> lk(h->mu);
@@ -1315,6 +1320,8 @@ int aak_likelihood_batch_nmax(aak_handle h, const double *params, const int *nma
   }
   h->batch_host_sum = host_sum;
   h->batch_host_max = host_max;
+  h->batch_issue_ms = issue_ms;
+  h->batch_wait_ms = wait_ms;
   h->batch_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call).count();
   if (prof) {
     for (int k = 0; k < 5; k++) h->batch_ms[k] = 0.0;
@@ -1391,8 +1398,9 @@ int aak_get_pool_stats(aak_handle h, double *out, int n) {
   if (!h) return fail(-1, "null handle");
   const double threads = (double)h->workers.size();
   const double busy = (threads > 0 && h->batch_wall_ms > 0) ? h->batch_host_sum / (threads * h->batch_wall_ms) : 0.0;
-  const double v[5] = {threads, h->batch_host_sum, h->batch_host_max, h->batch_wall_ms, busy};
-  const int m = n < 5 ? n : 5;
+  const double v[7] = {threads, h->batch_host_sum, h->batch_host_max, h->batch_wall_ms, busy, h->batch_issue_ms,
+                       h->batch_wait_ms};
+  const int m = n < 7 ? n : 7;
   for (int i = 0; i < m; i++) out[i] = v[i];
   return m;
 }

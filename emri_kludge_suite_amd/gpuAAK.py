@@ -238,10 +238,11 @@ class GPUAAK:
     def pool_stats(self):
         """Host pool of the last LikelihoodBatch: threads, host ms summed over
         the templates, longest template's host ms, wall ms of the call, busy
-        fraction."""
-        buf = np.zeros(5)
-        _lib.check(_lib.lib().aak_get_pool_stats(self._h, _lib.dptr(buf), 5), "pool_stats")
-        return dict(zip(("threads", "host_ms_sum", "host_ms_max", "wall_ms", "busy"), (float(x) for x in buf)))
+        fraction, and the issuing thread's ms enqueueing and waiting for plans."""
+        buf = np.zeros(7)
+        _lib.check(_lib.lib().aak_get_pool_stats(self._h, _lib.dptr(buf), 7), "pool_stats")
+        return dict(zip(("threads", "host_ms_sum", "host_ms_max", "wall_ms", "busy", "issue_ms", "wait_ms"),
+                        (float(x) for x in buf)))
 
     def inner_product_stage(self, hI, hII):
         """Runs the inner-product stage alone on two real series (test hook).

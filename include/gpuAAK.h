@@ -89,9 +89,10 @@ int aak_set_precision(aak_handle h, int mode);
 int aak_set_host_threads(aak_handle h, int n);
 
 /* Host pool of the last batch; no counterpart in the reference.
- * out[0..4] = pool threads, host-stage ms summed over the templates, longest
+ * out[0..6] = pool threads, host-stage ms summed over the templates, longest
  * template's host ms, wall ms of the batch call, busy fraction
- * (sum / (threads x wall)).  Returns the number written. */
+ * (sum / (threads x wall)), ms the issuing thread spent enqueueing device
+ * work, ms it spent waiting for finished plans.  Returns the number written. */
 int aak_get_pool_stats(aak_handle h, double *out, int n);
 
 /* Tuning of the batch pipeline (no counterpart in the reference; results do
