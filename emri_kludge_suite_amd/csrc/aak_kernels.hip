@@ -1863,13 +1863,23 @@ __device__ void setup_harmonics(const DenseParams &P, const DenseBuffers &B, int
 __device__ inline void e_range(const DenseBuffers &B, int i0, int len, int n_end, double &lo, double &hi) {
   lo = 1e300;
   hi = -1e300;
-  for (int q = threadIdx.x; q < len; q += kSetupThreads) {
-    const int i = i0 + q;
-    if (i < n_end) {
-      const double e = B.e[i];
-      lo = fmin(lo, e);
-      hi = fmax(hi, e);
+  // eight loads in flight per lane before the first use
+  constexpr int kU = 8;
+  for (int q0 = threadIdx.x; q0 < len; q0 += kU * kSetupThreads) {
+    double v[kU];
+    bool in[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int q = q0 + u * kSetupThreads, i = i0 + q;
+      in[u] = q < len && i < n_end;
+      v[u] = in[u] ? B.e[i] : 0.0;
     }
+#pragma unroll
+    for (int u = 0; u < kU; u++)
+      if (in[u]) {
+        lo = fmin(lo, v[u]);
+        hi = fmax(hi, v[u]);
+      }
   }
   for (int o = 1; o < 64; o <<= 1) {
     lo = fmin(lo, __shfl_xor(lo, o));
