@@ -2335,9 +2335,22 @@ __global__ __launch_bounds__(kDotThreads) void k_dot_final(const double *partial
                                                            const int *flags, int *clear) {
   __shared__ double s_dh[kDotThreads], s_hh[kDotThreads];
   double dh = 0.0, hh = 0.0;
-  for (int q = threadIdx.x; q < nparts; q += blockDim.x) {
-    dh += partials[2 * q];
-    hh += partials[2 * q + 1];
+  // four pairs in flight per lane before the first add; the adds keep the
+  // order of one pair per step
+  constexpr int kU = 4;
+  for (int q0 = threadIdx.x; q0 < nparts; q0 += kU * kDotThreads) {
+    double2 v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int q = q0 + u * kDotThreads;
+      v[u] = (q < nparts) ? reinterpret_cast<const double2 *>(partials)[q] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++)
+      if (q0 + u * kDotThreads < nparts) {
+        dh += v[u].x;
+        hh += v[u].y;
+      }
   }
   s_dh[threadIdx.x] = dh;
   s_hh[threadIdx.x] = hh;
