@@ -529,13 +529,22 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
   h->mark(L, 1);
   launch_phidot_guard(P, B, L.stream);
   const int nfwd = P.last_fold - P.i0 + 1;
+  FoldWork fw = L.fw;
+  fw.force_replay = h->force_fold_replay;
+  // the replay check of the last fold direction rides on the mode-sum setup
+  // when one follows (the two directions share the fold's work arrays, so an
+  // earlier direction keeps its own launch)
+  FoldArgs deferred{};
+  bool have_deferred = false;
   for (int pass = 0; pass < 2; pass++) {
     const int dir = pass == 0 ? 1 : -1;
     const int nsteps = pass == 0 ? nfwd : P.i0;
     if (nsteps <= 0) continue;
-    FoldWork fw = L.fw;
-    fw.force_replay = h->force_fold_replay;
-    if (fold_exact(P, B, fw, L.fold_nblk_cap, dir, nsteps, h->fold_mode, L.stream)) return fail(-7, "fold block capacity");
+    const bool last = pass == 1 || P.i0 <= 0;
+    const bool defer = last && !Pover && h->fold_mode == 0;
+    if (fold_exact(P, B, fw, L.fold_nblk_cap, dir, nsteps, h->fold_mode, defer ? &deferred : nullptr, L.stream))
+      return fail(-7, "fold block capacity");
+    have_deferred = defer;
   }
   launch_fold_fill(P, B, L.stream);
   h->mark(L, 2);
@@ -543,7 +552,7 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
     HIPCHK(hipGetLastError());
     return 0;
   }
-  launch_modesum(P, B, L.stream);
+  launch_modesum(P, B, have_deferred ? &deferred : nullptr, &fw, L.stream);
   h->mark(L, 3);
   HIPCHK(hipGetLastError());
   return 0;

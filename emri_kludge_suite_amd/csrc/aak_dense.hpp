@@ -154,6 +154,15 @@ struct FoldTie {
   long long P;  // inclusive run prefix at the tie (ties rounded down)
 };
 
+// One direction of the three phase folds (kernel argument).
+struct FoldArgs {
+  const double *rate[3];
+  double *out[3];
+  double x0[3];
+  int i0, dir, nsteps, nblk;
+  double dt;
+};
+
 struct FoldWork {
   double *bsum, *bmin, *bmax, *xstart;  // [3][nblk]
   double *E;                            // [3]
@@ -197,13 +206,18 @@ struct DenseBuffers {
 void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 // mode 0: single pass with a look-back (k_fold_resolve); 1: three passes
 // (k_fold_local, k_fold_chain, k_fold_local)
+// defer_replay (single-pass form only): instead of launching k_fold_replay,
+// store the direction's arguments there for launch_modesum to carry out
 int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, int nblk_cap, int dir, int nsteps,
-               int mode, hipStream_t s);
+               int mode, FoldArgs *defer_replay, hipStream_t s);
 void launch_fold_fill(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 int modesum_blocks(const DenseParams &P);  // mode-sum blocks (256 samples each)
 int modesum_slots(const DenseParams &P);   // coefficient slots: expansion blocks + 256-sample blocks
-void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
+// replay (optional): a fold direction whose serial replay check the setup
+// kernel carries out in three extra workgroups (it reads no phase)
+void launch_modesum(const DenseParams &P, const DenseBuffers &B, const FoldArgs *replay, const FoldWork *fw,
+                    hipStream_t s);
 void launch_clear_flags(const DenseBuffers &B, hipStream_t s);
 void launch_phase_out(const DenseParams &P, const DenseBuffers &B, double *out, hipStream_t s);
 // flags (optional): the waveform's counters; flags[3] != 0 (rejected on the

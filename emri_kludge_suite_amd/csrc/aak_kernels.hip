@@ -295,14 +295,6 @@ void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_
 }
 
 // ------------------------------------------------------------------ folds
-struct FoldArgs {
-  const double *rate[3];
-  double *out[3];
-  double x0[3];
-  int i0, dir, nsteps, nblk;
-  double dt;
-};
-
 __device__ inline double fold_incr(const double *rate, int i0, int dir, int s, double dt) {
   const int i = i0 + dir * s;
   const double r = rate[i];
@@ -468,24 +460,28 @@ constexpr long long kInclFail = 0x7FF4A5A5A5A5A5A6LL;              // the phase 
 
 // approximate block start values and the error bound E
 __global__ __launch_bounds__(1024) void k_fold_plan(FoldArgs A, FoldWork W) {
-  __shared__ double sh[1024];
-  __shared__ double shm[1024];
-  const int p = blockIdx.x, tid = threadIdx.x;
+  __shared__ double s_w[16], s_m[16];
+  const int p = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int per = (A.nblk + 1023) / 1024;
   double loc = 0.0;
   for (int j = 0; j < per; j++) {
     const int b = tid * per + j;
     if (b < A.nblk) loc += W.bsum[p * A.nblk + b];
   }
-  sh[tid] = loc;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const double add = (tid >= off) ? sh[tid - off] : 0.0;
-    __syncthreads();
-    sh[tid] += add;
-    __syncthreads();
+  // exclusive prefix of loc over the threads: shuffles inside each wave, then
+  // the wave totals (approximate starts: the bound E covers their rounding)
+  double incl = loc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
   }
-  double run = A.x0[p] + (sh[tid] - loc);
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  double before = 0.0;
+  for (int w = 0; w < wave; w++) before += s_w[w];
+  const double up = __shfl_up(incl, 1, 64);
+  double run = A.x0[p] + (before + (lane > 0 ? up : 0.0));
   double amax = fabs(A.x0[p]);
   for (int j = 0; j < per; j++) {
     const int b = tid * per + j;
@@ -495,15 +491,15 @@ __global__ __launch_bounds__(1024) void k_fold_plan(FoldArgs A, FoldWork W) {
       run += W.bsum[p * A.nblk + b];
     }
   }
-  shm[tid] = amax;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o, 64));
+  if (lane == 0) s_m[wave] = amax;
   __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
-    if (tid < o) shm[tid] = fmax(shm[tid], shm[tid + o]);
-    __syncthreads();
-  }
   if (tid == 0) {
+    double m = s_m[0];
+    for (int w = 1; w < 16; w++) m = fmax(m, s_m[w]);
     int ex = 0;
-    frexp(shm[0] > 0.0 ? shm[0] : 1.0, &ex);
+    frexp(m > 0.0 ? m : 1.0, &ex);
     W.E[p] = ((double)A.nsteps + 2.0 * kFoldBlock) * ldexp(1.0, ex - 52);
     W.status[p] = W.force_replay == 1 ? 1 : 0;
   }
@@ -528,16 +524,15 @@ __device__ inline int binade_key(double a, double E) {
 }
 
 // Fallback: plain serial replay of one phase's whole fold, by one workgroup
-// of kFoldThreads lanes with buf (4096 doubles of LDS) as staging; used when
-// the item bookkeeping overflowed or a chain check failed (correct for any
-// input).
-__device__ void fold_serial_replay(const FoldArgs &A, const FoldWork &W, int p, double *buf) {
+// with buf (bufn doubles of LDS) as staging; used when the item bookkeeping
+// overflowed or a check failed (correct for any input).
+__device__ void fold_serial_replay(const FoldArgs &A, const FoldWork &W, int p, double *buf, int bufn) {
   if (threadIdx.x == 0) atomicAdd(W.fallbacks, 1);
   double x = A.x0[p];
   if (threadIdx.x == 0) A.out[p][A.i0] = x;
-  for (int base = 0; base < A.nsteps; base += 4096) {
-    const int n = min(4096, A.nsteps - base);
-    for (int j = threadIdx.x; j < n; j += kFoldThreads) buf[j] = fold_incr(A.rate[p], A.i0, A.dir, base + j, A.dt);
+  for (int base = 0; base < A.nsteps; base += bufn) {
+    const int n = min(bufn, A.nsteps - base);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) buf[j] = fold_incr(A.rate[p], A.i0, A.dir, base + j, A.dt);
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int j = 0; j < n; j++) {
@@ -546,7 +541,7 @@ __device__ void fold_serial_replay(const FoldArgs &A, const FoldWork &W, int p, 
       }
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < n; j += kFoldThreads) A.out[p][out_index(A, base + j)] = buf[j];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) A.out[p][out_index(A, base + j)] = buf[j];
     __syncthreads();
   }
 }
@@ -564,7 +559,7 @@ __global__ __launch_bounds__(kFoldThreads, WRITE ? 3 : 4) void k_fold_local(Fold
   if (W.status[p]) {
     // the write pass of a phase the chain could not resolve: its first
     // workgroup replays the fold serially
-    if (WRITE && b == 0) fold_serial_replay(A, W, p, sd);
+    if (WRITE && b == 0) fold_serial_replay(A, W, p, sd, 4096);
     return;
   }
   const int base = b * kFoldBlock + tid * kFoldPerThread;
@@ -1465,11 +1460,11 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
 // serial replay of the phases whose single-pass fold failed
 __global__ __launch_bounds__(kFoldThreads) void k_fold_replay(FoldArgs A, FoldWork W) {
   __shared__ double sd[kFoldLds];
-  if (W.status[blockIdx.x]) fold_serial_replay(A, W, blockIdx.x, sd);
+  if (W.status[blockIdx.x]) fold_serial_replay(A, W, blockIdx.x, sd, 4096);
 }
 
 int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, int nblk_cap, int dir, int nsteps,
-               int mode, hipStream_t s) {
+               int mode, FoldArgs *defer_replay, hipStream_t s) {
   if (nsteps <= 0) return 0;
   FoldArgs A;
   for (int p = 0; p < 3; p++) {
@@ -1489,7 +1484,10 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
   if (mode == 0) {
     if (dir > 0) hipLaunchKernelGGL(k_fold_resolve<1>, dim3(3 * A.nblk), dim3(kFoldThreads), 0, s, A, W);
     else hipLaunchKernelGGL(k_fold_resolve<-1>, dim3(3 * A.nblk), dim3(kFoldThreads), 0, s, A, W);
-    hipLaunchKernelGGL(k_fold_replay, dim3(3), dim3(kFoldThreads), 0, s, A, W);
+    // the replay check can ride on the next launch that does not read the
+    // phases (the mode-sum setup); otherwise it is a launch of its own
+    if (defer_replay) *defer_replay = A;
+    else hipLaunchKernelGGL(k_fold_replay, dim3(3), dim3(kFoldThreads), 0, s, A, W);
     return 0;
   }
   if (dir > 0) hipLaunchKernelGGL((k_fold_local<false, 1>), dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
@@ -1917,9 +1915,17 @@ __device__ inline void setup_slot(const DenseParams &P, const DenseBuffers &B, i
 // the block's 256-sample blocks for the per-sample path.  A block with no
 // signal sample gets p = 0 and no coefficients.
 template <bool MIXED>
-__global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, DenseBuffers B) {
+__global__ __launch_bounds__(kSetupThreads) void k_modesum_setup(DenseParams P, DenseBuffers B, FoldArgs FA,
+                                                                 FoldWork FW, int ne) {
   __shared__ double s_J[kTaylorOrders][kSetupThreads];
   const int lane = threadIdx.x, eb = blockIdx.x;
+  if (eb >= ne) {
+    // workgroups past the expansion blocks: the serial replay of a phase
+    // whose single-pass fold failed (the setup reads e only)
+    const int p = eb - ne;
+    if (FW.status[p]) fold_serial_replay(FA, FW, p, &s_J[0][0], kTaylorOrders * kSetupThreads);
+    return;
+  }
   const int n_end = min(P.N, P.ip + P.ib);
   const int xs = exp_shift(P), span = kMsThreads << xs, sub = 1 << xs;
   const int nb = (P.N + 2 + kMsThreads - 1) / kMsThreads;
@@ -2275,14 +2281,18 @@ __global__ __launch_bounds__(kMsThreads) void k_modesum_direct(DenseParams P, De
 }
 
 // Workgroups of k_modesum the device runs at once (CUs x occupancy), per precision.
-void launch_modesum(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
+void launch_modesum(const DenseParams &P, const DenseBuffers &B, const FoldArgs *replay, const FoldWork *fw,
+                    hipStream_t s) {
   const int nb = modesum_blocks(P), ne = expansion_blocks(P);
+  const FoldArgs FA = replay ? *replay : FoldArgs{};
+  const FoldWork FW = fw ? *fw : FoldWork{};
+  const int ng = ne + (replay ? 3 : 0);
   if (P.mixed) {
-    hipLaunchKernelGGL(k_modesum_setup<true>, dim3(ne), dim3(kSetupThreads), 0, s, P, B);
+    hipLaunchKernelGGL(k_modesum_setup<true>, dim3(ng), dim3(kSetupThreads), 0, s, P, B, FA, FW, ne);
     if (exp_shift(P) > 0) hipLaunchKernelGGL(k_modesum_split<true>, dim3(kSplitGrid), dim3(kSetupThreads), 0, s, P, B, ne);
     hipLaunchKernelGGL(k_modesum<true>, dim3(nb), dim3(kMsThreads), 0, s, P, B, nb);
   } else {
-    hipLaunchKernelGGL(k_modesum_setup<false>, dim3(ne), dim3(kSetupThreads), 0, s, P, B);
+    hipLaunchKernelGGL(k_modesum_setup<false>, dim3(ng), dim3(kSetupThreads), 0, s, P, B, FA, FW, ne);
     if (exp_shift(P) > 0)
       hipLaunchKernelGGL(k_modesum_split<false>, dim3(kSplitGrid), dim3(kSetupThreads), 0, s, P, B, ne);
     hipLaunchKernelGGL(k_modesum<false>, dim3(nb), dim3(kMsThreads), 0, s, P, B, nb);
