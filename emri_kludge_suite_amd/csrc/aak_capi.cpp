@@ -190,7 +190,9 @@ struct Lane {
   int *d_guardw = nullptr;    // [kGuardGrid + 1] (DenseBuffers::guard_work)
   int *d_flags = nullptr;     // [8]: non-positive Phidot count, fold fallbacks, ... (DenseBuffers::flags)
   bool flags_dirty = false;   // a waveform wrote d_flags and no final reduction has zeroed them since
-  FoldWork fw{};
+  FoldWork fw{};       // forward fold (and both directions of the three-pass form)
+  FoldWork fw_back{};  // backward direction of the single-pass fold
+  FoldReset fold_reset{};
   int fold_nblk_cap = 0;
   void *d_foldmem = nullptr;
   double2 *d_hf = nullptr;    // [2][nbins]
@@ -346,8 +348,8 @@ static int alloc_lane(aak_handle h, Lane &L) {
   const size_t o_E = carve(sizeof(double) * 3), o_ni = carve(sizeof(int) * 3 * nblk), o_nt = carve(sizeof(int) * 3 * nblk);
   const size_t o_items = carve(sizeof(FoldItem) * 3 * (size_t)nblk * kFoldMaxItems);
   const size_t o_ties = carve(sizeof(FoldTie) * 3 * (size_t)nblk * kFoldMaxTies);
-  const size_t o_st = carve(sizeof(int) * 3);
-  const size_t o_la = carve(sizeof(long long) * 6 * nblk), o_li = carve(sizeof(long long) * 3 * nblk);
+  const size_t o_st = carve(sizeof(int) * 6);
+  const size_t o_lw = carve(sizeof(long long) * 2 * 3 * (size_t)nblk * kLW);
   const size_t o_lc = carve(sizeof(unsigned) * 4);
   HIPCHK(hipMalloc(&L.d_foldmem, bytes));
   char *base = (char *)L.d_foldmem;
@@ -361,10 +363,15 @@ static int alloc_lane(aak_handle h, Lane &L) {
   L.fw.items = (FoldItem *)(base + o_items);
   L.fw.ties = (FoldTie *)(base + o_ties);
   L.fw.status = (int *)(base + o_st);
-  L.fw.look_agg = (long long *)(base + o_la);
-  L.fw.look_incl = (long long *)(base + o_li);
+  L.fw.look = (long long *)(base + o_lw);
   L.fw.look_ctr = (unsigned *)(base + o_lc);
   L.fw.fallbacks = L.d_flags + 1;
+  // the backward direction of the single-pass fold has words of its own
+  L.fw_back = L.fw;
+  L.fw_back.status = L.fw.status + 3;
+  L.fw_back.look = L.fw.look + 3 * (size_t)nblk * kLW;
+  L.fw_back.look_ctr = L.fw.look_ctr + 1;
+  L.fold_reset = FoldReset{L.fw.look, (int)(2 * 3 * (size_t)nblk * kLW), L.fw.status, L.fw.look_ctr, 0};
   const int nb = h->nbins;
   HIPCHK(hipMalloc(&L.d_hf, sizeof(double2) * 2 * (size_t)nb));
   const int nparts = std::max(2048, h->fused_fft ? fft_partials(h->fft) : 0);
@@ -527,24 +534,31 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
   HIPCHK(hipEventRecord(L.ev_rates[slot], L.stream));
   L.rates_recorded[slot] = true;
   h->mark(L, 1);
-  launch_phidot_guard(P, B, L.stream);
+  {
+    FoldReset R = L.fold_reset;
+    R.force = h->force_fold_replay;
+    launch_phidot_guard(P, B, R, L.stream);
+  }
   const int nfwd = P.last_fold - P.i0 + 1;
-  FoldWork fw = L.fw;
-  fw.force_replay = h->force_fold_replay;
+  FoldWork fws[2] = {L.fw, h->fold_mode == 0 ? L.fw_back : L.fw};
+  fws[0].force_replay = fws[1].force_replay = h->force_fold_replay;
   // the replay check of the last fold direction rides on the mode-sum setup
   // when one follows (the two directions share the fold's work arrays, so an
   // earlier direction keeps its own launch)
   FoldArgs deferred{};
   bool have_deferred = false;
+  int deferred_pass = 0;
   for (int pass = 0; pass < 2; pass++) {
     const int dir = pass == 0 ? 1 : -1;
     const int nsteps = pass == 0 ? nfwd : P.i0;
     if (nsteps <= 0) continue;
     const bool last = pass == 1 || P.i0 <= 0;
     const bool defer = last && !Pover && h->fold_mode == 0;
-    if (fold_exact(P, B, fw, L.fold_nblk_cap, dir, nsteps, h->fold_mode, defer ? &deferred : nullptr, L.stream))
+    if (fold_exact(P, B, fws[pass], L.fold_nblk_cap, dir, nsteps, h->fold_mode, defer ? &deferred : nullptr,
+                   L.stream))
       return fail(-7, "fold block capacity");
     have_deferred = defer;
+    deferred_pass = pass;
   }
   launch_fold_fill(P, B, L.stream);
   h->mark(L, 2);
@@ -552,7 +566,7 @@ static int enqueue_waveform(aak_handle h, Lane &L, const DenseParams &Pplan, con
     HIPCHK(hipGetLastError());
     return 0;
   }
-  launch_modesum(P, B, have_deferred ? &deferred : nullptr, &fw, L.stream);
+  launch_modesum(P, B, have_deferred ? &deferred : nullptr, &fws[deferred_pass], L.stream);
   h->mark(L, 3);
   HIPCHK(hipGetLastError());
   return 0;
@@ -1534,7 +1548,7 @@ int aak_test_phidot_guard(double *rate, int n, int i0, int guard_last) {
   if (st == hipSuccess) st = hipMemcpy(d_flags, flags, sizeof(flags), hipMemcpyHostToDevice);
   if (st == hipSuccess) st = hipMemcpy(d_gw, gw, sizeof(gw), hipMemcpyHostToDevice);
   if (st == hipSuccess) {
-    launch_phidot_guard(P, B, 0);
+    launch_phidot_guard(P, B, FoldReset{}, 0);
     st = hipGetLastError();
   }
   if (st == hipSuccess) st = hipMemcpy(rate, d_rate, sizeof(double) * n, hipMemcpyDeviceToHost);

@@ -154,6 +154,18 @@ struct FoldTie {
   long long P;  // inclusive run prefix at the tie (ties rounded down)
 };
 
+constexpr int kLW = 8;  // look-back words per block and phase of the single-pass fold
+
+// What k_guard clears before the folds of a waveform: the look-back words,
+// replay marks and tickets of both fold directions.
+struct FoldReset {
+  long long *words;
+  int nwords;
+  int *status;    // [2][3]
+  unsigned *ctr;  // [2]
+  int force;      // 1: every phase takes the serial replay (test hook)
+};
+
 // One direction of the three phase folds (kernel argument).
 struct FoldArgs {
   const double *rate[3];
@@ -169,9 +181,8 @@ struct FoldWork {
   int *nitems, *nties;                  // [3][nblk]
   FoldItem *items;                      // [3][nblk][kFoldMaxItems]
   FoldTie *ties;                        // [3][nblk][kFoldMaxTies]
-  int *status;                          // [3]: 0 ok, 1 overflow/fallback
-  long long *look_agg;                  // [3][nblk][2]: action of a one-run block (even, odd start)
-  long long *look_incl;                 // [3][nblk]: exact end value (bits)
+  int *status;                          // [3]: 0 ok, 1 overflow/fallback (one fold direction)
+  long long *look;                      // [3][nblk][kLW]: look-back words of the single-pass fold
   unsigned *look_ctr;                   // [1]: block tickets of the single-pass fold
   int *fallbacks;                       // serial replays since the last counter reset
   int force_replay;                     // test hook: 1 every phase takes the serial replay, 2 the middle block fails
@@ -211,7 +222,7 @@ void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
 int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, int nblk_cap, int dir, int nsteps,
                int mode, FoldArgs *defer_replay, hipStream_t s);
 void launch_fold_fill(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
-void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s);
+void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, const FoldReset &R, hipStream_t s);
 int modesum_blocks(const DenseParams &P);  // mode-sum blocks (256 samples each)
 int modesum_slots(const DenseParams &P);   // coefficient slots: expansion blocks + 256-sample blocks
 // replay (optional): a fold direction whose serial replay check the setup

@@ -192,13 +192,31 @@ void launch_rates(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
 // reads a final value.  The kernel returns at once when k_rates counted no
 // such sample, so it is always enqueued and needs no host round trip.
 constexpr int kGuardThreads = 1024;
+
+// Look-back words of the single-pass fold (k_fold_resolve), kLW per block and
+// phase, reset by k_guard for every waveform: the exact action of a pure
+// block (even and odd start) and its grid key, the exact end value, and the
+// approximate sum and L1 norm of the block's increments and their inclusive
+// prefixes.  "Not yet", "failed" and "not pure" are signalling-NaN patterns
+// that arithmetic never produces and no run total or key can equal.
+constexpr int kLwA0 = 0, kLwA1 = 1, kLwKey = 2, kLwIncl = 3, kLwAggS = 4, kLwAggL = 5, kLwIncS = 6, kLwIncL = 7;
+constexpr long long kLookEmpty = 0x7FF4A5A5A5A5A5A5LL;
+constexpr long long kLookFail = 0x7FF4A5A5A5A5A5A6LL;
+constexpr long long kLookNone = 0x7FF4A5A5A5A5A5A7LL;
 constexpr int kGuardPer = 32;  // consecutive samples per thread and chunk
 constexpr int kGuardSpinCap = 1 << 22;
 
 __device__ inline bool phidot_ok(double r) { return !(r <= 0.); }
 
-__global__ __launch_bounds__(kGuardThreads) void k_guard(DenseParams P, DenseBuffers B) {
-  if (B.flags[0] == 0) return;
+__global__ __launch_bounds__(kGuardThreads) void k_guard(DenseParams P, DenseBuffers B, FoldReset R) {
+  {
+    // the fold's look-back words, replay marks and tickets for this waveform
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+    for (int k = gt; k < R.nwords; k += gs) R.words[k] = kLookEmpty;
+    if (R.status && gt < 6) R.status[gt] = R.force == 1 ? 1 : 0;
+    if (R.ctr && gt < 2) R.ctr[gt] = 0;
+  }
+  if (P.guard_last <= P.i0 || B.flags[0] == 0) return;
   __shared__ int s_w[kGuardThreads / 64];
   __shared__ int s_t, s_carry;
   double *r = B.rate[1];
@@ -289,9 +307,9 @@ __global__ __launch_bounds__(kGuardThreads) void k_guard(DenseParams P, DenseBuf
   }
 }
 
-void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, hipStream_t s) {
-  if (P.guard_last <= P.i0) return;
-  hipLaunchKernelGGL(k_guard, dim3(kGuardGrid), dim3(kGuardThreads), 0, s, P, B);
+void launch_phidot_guard(const DenseParams &P, const DenseBuffers &B, const FoldReset &R, hipStream_t s) {
+  if (P.guard_last <= P.i0 && R.nwords == 0 && !R.status && !R.ctr) return;
+  hipLaunchKernelGGL(k_guard, dim3(kGuardGrid), dim3(kGuardThreads), 0, s, P, B, R);
 }
 
 // ------------------------------------------------------------------ folds
@@ -452,12 +470,6 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_sums(FoldArgs A, FoldWork
   }
 }
 
-// Look-back words of the single-pass fold (k_fold_resolve), reset by the plan
-constexpr long long kAggEmpty = (long long)0x8000000000000000ULL;  // not classified yet
-constexpr long long kAggNone = (long long)0x8000000000000001ULL;   // not a pure block
-constexpr long long kInclEmpty = 0x7FF4A5A5A5A5A5A5LL;             // end value not known yet
-constexpr long long kInclFail = 0x7FF4A5A5A5A5A5A6LL;              // the phase failed
-
 // approximate block start values and the error bound E
 __global__ __launch_bounds__(1024) void k_fold_plan(FoldArgs A, FoldWork W) {
   __shared__ double s_w[16], s_m[16];
@@ -503,13 +515,6 @@ __global__ __launch_bounds__(1024) void k_fold_plan(FoldArgs A, FoldWork W) {
     W.E[p] = ((double)A.nsteps + 2.0 * kFoldBlock) * ldexp(1.0, ex - 52);
     W.status[p] = W.force_replay == 1 ? 1 : 0;
   }
-  // look-back state of the single-pass resolve
-  for (int b = tid; b < A.nblk; b += 1024) {
-    W.look_agg[2 * (p * A.nblk + b)] = kAggEmpty;
-    W.look_agg[2 * (p * A.nblk + b) + 1] = kAggEmpty;
-    W.look_incl[p * A.nblk + b] = kInclEmpty;
-  }
-  if (p == 0 && tid == 0) *W.look_ctr = 0;
 }
 
 __device__ inline int binade_key(double a, double E) {
@@ -1047,6 +1052,14 @@ __device__ inline void look_store(long long *v, long long x) {
   __hip_atomic_store(v, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// a value every lane holds, kept in scalar registers
+__device__ inline double uniform_double(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // a 64-bit value of lane q, read by every lane of the wave
 __device__ inline long long lane_read64(long long v, int q) {
   const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffLL), q);
@@ -1074,11 +1087,16 @@ struct LookState {
   long long v;       // st 2: end value bits
   long long a0, a1;  // st 1: the totals for an even and an odd start
 };
-__device__ inline LookState look_state(const long long *agg, const long long *incl, int j) {
-  const long long iw = look_load(incl + j);  // all loads in flight at once
-  const long long w0 = look_load(agg + 2 * j), w1 = look_load(agg + 2 * j + 1);
-  if (iw != kInclEmpty) return {2, iw, 0, 0};
-  if (w0 != kAggEmpty && w0 != kAggNone && w1 != kAggEmpty) return {1, 0, w0, w1};
+__device__ inline LookState look_state(const long long *lw, int j, int &key) {
+  const long long *w = lw + (size_t)j * kLW;
+  const long long iw = look_load(w + kLwIncl);  // all loads in flight at once
+  const long long w0 = look_load(w + kLwA0), w1 = look_load(w + kLwA1), kw = look_load(w + kLwKey);
+  key = -1;
+  if (iw != kLookEmpty) return {2, iw, 0, 0};
+  if (w0 != kLookEmpty && w0 != kLookNone && w1 != kLookEmpty && kw != kLookEmpty) {
+    key = (int)kw;
+    return {1, 0, w0, w1};
+  }
   return {0, 0, 0, 0};
 }
 
@@ -1093,7 +1111,7 @@ __device__ inline void act_compose(long long f0, long long f1, long long &n0, lo
 // The value before a block from an exact end value w and the pure blocks
 // combined after it (key K, action c0/c1; K < 0: none).
 __device__ inline bool look_finish(long long w, int K, long long c0, long long c1, double &x_out) {
-  if (w == kInclFail) return false;
+  if (w == kLookFail) return false;
   double x = __longlong_as_double(w);
   if (K >= 0) {
     if (exact_key(x) != K) return false;
@@ -1111,11 +1129,8 @@ __device__ inline bool look_finish(long long w, int K, long long c0, long long c
 // Wave 0: exact value before block b's first step.  Returns false when the
 // phase failed (a failed predecessor, a failed binade check, a wait past the
 // cap).
-__device__ inline bool fold_look_back(const FoldArgs &A, const FoldWork &W, int p, int b, double E, double &x_out) {
+__device__ inline bool fold_look_back(const FoldArgs &A, const long long *lw, int p, int b, double &x_out) {
   const int lane = threadIdx.x & 63;
-  const long long *agg = W.look_agg + 2 * (size_t)p * A.nblk;
-  const long long *incl = W.look_incl + (size_t)p * A.nblk;
-  const double *xs = W.xstart + (size_t)p * A.nblk;
   int j = b - 1;              // nearest block not yet combined
   int K = -1;                 // binade key of the combined pure blocks
   long long c0 = 0, c1 = 0;   // their combined action
@@ -1125,10 +1140,7 @@ __device__ inline bool fold_look_back(const FoldArgs &A, const FoldWork &W, int 
     // block -1 stands for the start value x0
     LookState S = {2, __double_as_longlong(A.x0[p]), 0, 0};
     int key = -1;
-    if (jb >= 0) {
-      S = look_state(agg, incl, jb);
-      if (S.st == 1) key = binade_key(xs[jb], E);
-    }
+    if (jb >= 0) S = look_state(lw, jb, key);
     const int s0 = __shfl(S.st, 0, 64);
     const int k0 = __shfl(key, 0, 64);
     const int Kc = (K >= 0) ? K : (s0 == 1 ? k0 : -2);
@@ -1160,18 +1172,81 @@ __device__ inline bool fold_look_back(const FoldArgs &A, const FoldWork &W, int 
     // an end value is used at once, a newly published action is combined by
     // the next window, which starts at that block
     LookState T;
+    int tk;
     do {
       if (++spins > kLookSpinCap) return false;
       __builtin_amdgcn_s_sleep(1);
-      T = look_state(agg, incl, js);
+      T = look_state(lw, js, tk);
     } while (T.st == ss);
     if (T.st == 2) return look_finish(T.v, K, c0, c1, x_out);
     j = js;
   }
 }
 
+// Wave 0: the approximate value before block b (S) and |x0| plus the L1 norm
+// of the increments before it (L), from the sums every block publishes right
+// after loading its rates; the nearest block with published prefixes ends
+// the walk back.  The blocks waited on publish their sums without waiting
+// themselves, so the waits are short.  Returns false after a wait past the
+// cap.
+__device__ inline bool approx_look_back(const FoldArgs &A, const long long *lw, int p, int b, double &S, double &L) {
+  const int lane = threadIdx.x & 63;
+  int j = b - 1;
+  double s = 0.0, l = 0.0;
+  int spins = 0;
+  for (;;) {
+    const int jb = j - lane;
+    int st = 2;  // block -1 stands for the start value
+    double vs = A.x0[p], vl = fabs(A.x0[p]);
+    if (jb >= 0) {
+      const long long *w = lw + (size_t)jb * kLW;
+      const long long is = look_load(w + kLwIncS), il = look_load(w + kLwIncL);
+      const long long as = look_load(w + kLwAggS), al = look_load(w + kLwAggL);
+      if (is != kLookEmpty && il != kLookEmpty) {
+        vs = __longlong_as_double(is);
+        vl = __longlong_as_double(il);
+      } else if (as != kLookEmpty && al != kLookEmpty) {
+        st = 1;
+        vs = __longlong_as_double(as);
+        vl = __longlong_as_double(al);
+      } else {
+        st = 0;
+      }
+    }
+    const unsigned long long m2 = __ballot(st == 2), m0 = __ballot(st == 0);
+    const int f2 = m2 ? __builtin_ctzll(m2) : 64, f0 = m0 ? __builtin_ctzll(m0) : 64;
+    // lanes before the first unpublished block, or through the nearest prefix
+    const int upto = (f0 < f2) ? f0 : (f2 < 64 ? f2 + 1 : 64);
+    double ts = (lane < upto) ? vs : 0.0, tl = (lane < upto) ? vl : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ts += __shfl_xor(ts, o, 64);
+      tl += __shfl_xor(tl, o, 64);
+    }
+    s += ts;
+    l += tl;
+    if (f0 < f2) {
+      j -= f0;
+      for (;;) {
+        if (++spins > kLookSpinCap) return false;
+        __builtin_amdgcn_s_sleep(1);
+        const long long *w = lw + (size_t)j * kLW;
+        const long long as = look_load(w + kLwAggS), al = look_load(w + kLwAggL);
+        if (as != kLookEmpty && al != kLookEmpty) break;
+      }
+      continue;
+    }
+    if (f2 < 64) {
+      S = s;
+      L = l;
+      return true;
+    }
+    j -= 64;
+  }
+}
+
 template <int DIR>
-__global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, FoldWork W) {
+__global__ __launch_bounds__(kFoldThreads, 4) void k_fold_resolve(FoldArgs A, FoldWork W) {
   __shared__ double shd[kFoldThreads / 64];
   __shared__ long long shl[2 * (kFoldThreads / 64)];
   __shared__ int shi[kFoldThreads];
@@ -1180,6 +1255,7 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
   __shared__ FoldItem s_items[kFoldMaxItems];
   __shared__ FoldTie s_ties[kFoldMaxTies];
   __shared__ double s_x[kFoldMaxItems];
+  __shared__ double s_xs, s_E;
   __shared__ int s_pb[2], s_ok;
   const int tid = threadIdx.x;
   if (tid == 0) {
@@ -1188,9 +1264,9 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
     s_pb[1] = (int)(t / 3u);
   }
   __syncthreads();
-  const int p = s_pb[0], b = s_pb[1];
-  long long *aggw = W.look_agg + 2 * (size_t)p * A.nblk;
-  long long *inclw = W.look_incl + (size_t)p * A.nblk;
+  const int p = __builtin_amdgcn_readfirstlane(s_pb[0]), b = __builtin_amdgcn_readfirstlane(s_pb[1]);
+  long long *lw = W.look + (size_t)p * A.nblk * kLW;
+  long long *mw = lw + (size_t)b * kLW;  // this block's words
   // entry: a phase already marked (the replay hook) skips to the end
   if (tid == 0) s_ok = __hip_atomic_load(W.status + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
   __syncthreads();
@@ -1203,17 +1279,50 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
   bool next_starts = true;
   long long carry = 0;
   int item_base = 0, tie_base = 0;
-  const double E = W.E[p];
-  if (ok) {
-    stage_increments<DIR>(A, p, b, sd);
-    double tsum = 0.0;
+  // the increments, and the block's approximate sum and L1 norm, published at
+  // once for the approximate look-back of the blocks after it
+  stage_increments<DIR>(A, p, b, sd);
+  double tsum = 0.0, tl1 = 0.0;
 #pragma unroll
-    for (int j = 0; j < kFoldPerThread; j++) {
-      d[j] = sd[fold_lds_index(tid * kFoldPerThread + j)];
-      tsum += d[j];
+  for (int j = 0; j < kFoldPerThread; j++) {
+    const double v = sd[fold_lds_index(tid * kFoldPerThread + j)];
+    tsum += v;
+    tl1 += fabs(v);
+  }
+  double bsum, bl1;
+  const double excl = block_excl_sum(tsum, shd, &bsum);
+  (void)block_excl_sum(tl1, shd, &bl1);
+  if (tid == 0) {
+    look_store(mw + kLwAggL, __double_as_longlong(bl1));
+    look_store(mw + kLwAggS, __double_as_longlong(bsum));
+  }
+  // approximate start of the block, and the bound E on the distance between
+  // approximate and exact values in it: every value up to the block's end,
+  // either way, is at most U = |x0| + the L1 norm so far in magnitude, so
+  // each of at most nsteps + 2 kFoldBlock roundings on either path is at most
+  // half an ulp of U
+  if (tid < 64) {
+    double S = A.x0[p], L = fabs(A.x0[p]);
+    const bool aok = (b == 0) || approx_look_back(A, lw, p, b, S, L);
+    if (tid == 0) {
+      look_store(mw + kLwIncL, __double_as_longlong(L + bl1));
+      look_store(mw + kLwIncS, __double_as_longlong(S + bsum));
+      const double U = (L + bl1) * (1.0 + 0x1p-24);
+      int ex = 0;
+      frexp(U > 0.0 ? U : 1.0, &ex);
+      s_xs = S;
+      s_E = ((double)A.nsteps + 2.0 * kFoldBlock) * ldexp(1.0, ex - 52);
+      if (!aok) s_ok = 0;
     }
-    double dummy;
-    double a0 = W.xstart[p * A.nblk + b] + block_excl_sum(tsum, shd, &dummy);
+  }
+  __syncthreads();
+  ok = ok && s_ok != 0;
+  const double E = uniform_double(s_E);
+  if (ok) {
+    // the increments again from LDS (not held across the look-back)
+#pragma unroll
+    for (int j = 0; j < kFoldPerThread; j++) d[j] = sd[fold_lds_index(tid * kFoldPerThread + j)];
+    double a0 = uniform_double(s_xs) + excl;
     // keys: >= 0 binade grid, -1 exact replay, -2 beyond the end
 #pragma unroll
     for (int j = 0; j < kFoldPerThread; j++) {
@@ -1312,9 +1421,10 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
       s_pb[0] = nheads_tot;
       s_pb[1] = nties_tot;
       // a block that is one run publishes its action at once: the run total
-      // plus the ties rounded up, for an even and for an odd start
+      // plus the ties rounded up, for an even and for an odd start, and its
+      // grid key
       const bool pure = ok && nheads_tot == 1 && s_items[0].kind == 0;
-      long long a0 = kAggNone, a1 = kAggNone;
+      long long a0 = kLookNone, a1 = kLookNone;
       if (pure) {
         long long u0 = 0, u1 = 0;
         for (int t = 0; t < nties_tot; t++) {
@@ -1325,8 +1435,9 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
         a0 = s_items[0].R + u0;
         a1 = s_items[0].R + u1;
       }
-      look_store(aggw + 2 * b + 1, a1);
-      look_store(aggw + 2 * b, a0);
+      if (pure) look_store(mw + kLwKey, (long long)((s_items[0].k + 4096) * 2 + (s_items[0].sgn > 0 ? 1 : 0)));
+      look_store(mw + kLwA1, a1);
+      look_store(mw + kLwA0, a0);
     }
     __syncthreads();
   }
@@ -1358,7 +1469,7 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
       }
     }
     double x = 0.0;
-    bool wok = ok && (b == 0 ? (x = A.x0[p], true) : fold_look_back(A, W, p, b, E, x));
+    bool wok = ok && (b == 0 ? (x = A.x0[p], true) : fold_look_back(A, lw, p, b, x));
     if (W.force_replay == 2 && b == A.nblk / 2) wok = false;  // test hook
     if (wok && b == 0 && lane == 0) A.out[p][A.i0] = x;
     long long my_m = 0;
@@ -1390,10 +1501,10 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
     if (wok && !(x == x)) wok = false;  // a NaN phase takes the serial replay
     if (lane == 0) {
       if (wok) {
-        look_store(inclw + b, __double_as_longlong(x));
+        look_store(mw + kLwIncl, __double_as_longlong(x));
       } else {
         __hip_atomic_store(W.status + p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        look_store(inclw + b, kInclFail);
+        look_store(mw + kLwIncl, kLookFail);
       }
       s_ok = wok ? 1 : 0;
     }
@@ -1478,9 +1589,6 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
   A.dt = P.dt;
   A.nblk = (nsteps + kFoldBlock - 1) / kFoldBlock;
   if (A.nblk > nblk_cap) return -1;
-  if (dir > 0) hipLaunchKernelGGL(k_fold_sums<1>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
-  else hipLaunchKernelGGL(k_fold_sums<-1>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
-  hipLaunchKernelGGL(k_fold_plan, dim3(3), dim3(1024), 0, s, A, W);
   if (mode == 0) {
     if (dir > 0) hipLaunchKernelGGL(k_fold_resolve<1>, dim3(3 * A.nblk), dim3(kFoldThreads), 0, s, A, W);
     else hipLaunchKernelGGL(k_fold_resolve<-1>, dim3(3 * A.nblk), dim3(kFoldThreads), 0, s, A, W);
@@ -1490,6 +1598,9 @@ int fold_exact(const DenseParams &P, const DenseBuffers &B, const FoldWork &W, i
     else hipLaunchKernelGGL(k_fold_replay, dim3(3), dim3(kFoldThreads), 0, s, A, W);
     return 0;
   }
+  if (dir > 0) hipLaunchKernelGGL(k_fold_sums<1>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  else hipLaunchKernelGGL(k_fold_sums<-1>, dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
+  hipLaunchKernelGGL(k_fold_plan, dim3(3), dim3(1024), 0, s, A, W);
   if (dir > 0) hipLaunchKernelGGL((k_fold_local<false, 1>), dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   else hipLaunchKernelGGL((k_fold_local<false, -1>), dim3(A.nblk, 3), dim3(kFoldThreads), 0, s, A, W);
   hipLaunchKernelGGL(k_fold_chain, dim3(3), dim3(kChainThreads), 0, s, A, W);
