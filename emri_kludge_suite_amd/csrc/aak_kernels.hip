@@ -1246,7 +1246,7 @@ __device__ inline bool approx_look_back(const FoldArgs &A, const long long *lw, 
 }
 
 template <int DIR>
-__global__ __launch_bounds__(kFoldThreads, 4) void k_fold_resolve(FoldArgs A, FoldWork W) {
+__global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, FoldWork W) {
   __shared__ double shd[kFoldThreads / 64];
   __shared__ long long shl[2 * (kFoldThreads / 64)];
   __shared__ int shi[kFoldThreads];
