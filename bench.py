@@ -422,7 +422,7 @@ def main():
                                                "work (transcendentals at 20 FLOP), most of which the Taylor form skips; "
                                                "an effective rate, not a fraction of any peak"},
         "fold": dict(hbm_roof(48.0 * n_active, st_mx[1] / per_pass, "traffic_bytes_per_launch fold_stage"),
-                     kernels="k_guard + k_fold_sums + k_fold_plan + k_fold_resolve (the replay check rides on the mode-sum setup)",
+                     kernels="k_guard + k_fold_resolve (the replay check rides on the mode-sum setup)",
                      bytes_convention="48 B per active sample (three phases: rate read, phase written)"),
         "rates": dict(hbm_roof(40.0 * n_active, st_mx[0] / per_pass, "traffic_bytes_per_launch k_rates"),
                       kernels="k_rates", bytes_convention="40 B per active sample (e, three rates, amplitude written)"),
