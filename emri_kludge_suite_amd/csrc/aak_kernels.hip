@@ -1401,7 +1401,6 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
           s_items[item].sgn = (key[j] >= 0) ? ((key[j] & 1) ? 1 : -1) : 0;
           s_items[item].first = sl;
           s_items[item].tie_first = tcount;
-          s_items[item].d = d[j];
         }
         P += r[j];
         const bool last_of_run = (key[j] >= 0) && ((j + 1 < kFoldPerThread)
@@ -1454,7 +1453,9 @@ __global__ __launch_bounds__(kFoldThreads, 3) void k_fold_resolve(FoldArgs A, Fo
       it_kind = I.kind;
       it_k = I.k;
       it_sgn = I.sgn;
-      it_d = __double_as_longlong(I.d);
+      // a single step's increment, from the staged increments (sd holds them
+      // until the write)
+      it_d = __double_as_longlong(sd[fold_lds_index(I.first)]);
       if (I.kind == 0) {
         it_t0 = I.tie_first;
         it_t1 = (lane + 1 < ni) ? s_items[lane + 1].tie_first : nt;
