@@ -204,7 +204,9 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
       if (j_max == j_max_temp) j_max = j;
       eA[j + 1] = eA[j_max];
       vA[j + 1] = vA[j_max];
+      return true;
     }
+    return false;
   };
   for (int j = j0; j < j0 + steps - 1; j++) {
     edotm = edot;
@@ -237,13 +239,28 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
     polyfit2(Mc, dt_map, tr.M_map, points);
     polyfit2(Sc, dt_map, tr.s_map, points);
   }
+  // Clamp cycles.  After this loop's first clamp, j_max is fixed and every
+  // clamp at a step j resets (e, v) to the same state (eA[j_max], vA[j_max]);
+  // step j + 1 then advances from that state with the rates of step j as its
+  // previous rates.  So the steps after a clamp depend only on the bits of
+  // (edot, vdot) at the clamped step: when a clamp's rates equal those of an
+  // earlier clamp at step j', every later entry repeats the one j - j' before
+  // it (fixed step, pure rates, the same clamp decisions).  Sources that plunge
+  // long before the end of the evolution settle into such a cycle within a
+  // few clamps; the rates of the clamps seen so far are kept in a small
+  // open-addressed table.
+  constexpr int kCycleSlots = 1 << 12, kCycleMax = kCycleSlots / 2;
+  thread_local std::vector<std::pair<uint64_t, uint64_t>> cyc_key;
+  thread_local std::vector<int> cyc_j;
+  int cyc_n = 0;
+  bool cyc_on = false;
   for (int j = j0 + steps - 1; j < j_end; j++) {
     edotm = edot;
     vdotm = vdot;
     PN::ak_rates(vA[j], eA[j], coslam, mu, M0, S0, &edot, &vdot);
     eA[j + 1] = eA[j] + (1.5 * edot - .5 * edotm) * dt_large;
     vA[j + 1] = vA[j] + (1.5 * vdot - .5 * vdotm) * dt_large;
-    clamp_step(j);
+    const bool clamped = clamp_step(j);
     // Past a plunge the clamp resets (e, v) to the same point at every step:
     // once a step starts from the state it produces, with the rates it had
     // before, every later step repeats it exactly (fixed step, pure rates), so
@@ -255,24 +272,68 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
       }
       break;
     }
+    if (clamped && j_max != j_max_temp && cyc_n < kCycleMax) {
+      if (!cyc_on) {
+        cyc_key.assign(kCycleSlots, {0, 0});
+        cyc_j.assign(kCycleSlots, -1);
+        cyc_on = true;
+      }
+      uint64_t ke, kv;
+      std::memcpy(&ke, &edot, sizeof ke);
+      std::memcpy(&kv, &vdot, sizeof kv);
+      unsigned h = (unsigned)((ke * 0x9E3779B97F4A7C15ULL ^ kv) >> 40) & (kCycleSlots - 1);
+      while (cyc_j[h] >= 0 && !(cyc_key[h].first == ke && cyc_key[h].second == kv)) h = (h + 1) & (kCycleSlots - 1);
+      if (cyc_j[h] >= 0) {
+        const int period = j - cyc_j[h];
+        for (int k = j + 2; k <= j_end; k++) {
+          eA[k] = eA[k - period];
+          vA[k] = vA[k - period];
+        }
+        break;
+      }
+      cyc_key[h] = {ke, kv};
+      cyc_j[h] = j;
+      cyc_n++;
+    }
   }
   j_max_temp = j_max;
-  for (int j = j0; j <= j_end; j++) {
-    double dt;
-    if (j < j0 + steps) dt = dt_map[j - j0];
-    else dt = dt_map[steps - 1] + dt_large * (j - j0 - steps + 1);
-    const double dt2 = dt * dt;
-    tf[j] = dt;
-    ef[j] = eA[j] + ec[0] * dt + ec[1] * dt2;
-    vf[j] = vA[j] + vc[0] * dt + vc[1] * dt2;
-    Mf[j] = M0 + (Mc[0] * dt + Mc[1] * dt2) * kMsunSec;
-    Sf[j] = S0 + Sc[0] * dt + Sc[1] * dt2;
-    if (ef[j] < 0. || vf[j] < vf[std::max(j0, j - 1)]) {
-      if (j_max == j_max_temp) j_max = j - 1;
-      ef[j] = ef[j_max];
-      vf[j] = vf[j_max];
-      Mf[j] = Mf[j_max];
-      Sf[j] = Sf[j_max];
+  {
+    // Knot values.  Each knot's fitted values depend on that knot alone, so
+    // they are formed for every knot first (loops without branches, which
+    // vectorise); the clamp then runs in knot order from the first knot it
+    // touches, on the same values: before that knot it changes nothing.
+    double *__restrict tq = tf, *__restrict eq = ef, *__restrict vq = vf, *__restrict Mq = Mf, *__restrict Sq = Sf;
+    const double *__restrict eAq = eA, *__restrict vAq = vA;
+    const double ec0 = ec[0], ec1 = ec[1], vc0 = vc[0], vc1 = vc[1], Mc0 = Mc[0], Mc1 = Mc[1], Sc0 = Sc[0], Sc1 = Sc[1];
+    const int jh = std::min(j0 + steps - 1, j_end);  // last knot of the sparse head
+    for (int j = j0; j <= jh; j++) {
+      const double dt = dt_map[j - j0], dt2 = dt * dt;
+      tq[j] = dt;
+      eq[j] = eAq[j] + ec0 * dt + ec1 * dt2;
+      vq[j] = vAq[j] + vc0 * dt + vc1 * dt2;
+      Mq[j] = M0 + (Mc0 * dt + Mc1 * dt2) * kMsunSec;
+      Sq[j] = S0 + Sc0 * dt + Sc1 * dt2;
+    }
+    const double th = dt_map[steps - 1];
+    const int jb = j0 + steps - 1;
+    for (int j = jh + 1; j <= j_end; j++) {
+      const double dt = th + dt_large * (double)(j - jb), dt2 = dt * dt;
+      tq[j] = dt;
+      eq[j] = eAq[j] + ec0 * dt + ec1 * dt2;
+      vq[j] = vAq[j] + vc0 * dt + vc1 * dt2;
+      Mq[j] = M0 + (Mc0 * dt + Mc1 * dt2) * kMsunSec;
+      Sq[j] = S0 + Sc0 * dt + Sc1 * dt2;
+    }
+    int j = j0;
+    while (j <= j_end && !(ef[j] < 0. || vf[j] < vf[std::max(j0, j - 1)])) j++;
+    for (; j <= j_end; j++) {
+      if (ef[j] < 0. || vf[j] < vf[std::max(j0, j - 1)]) {
+        if (j_max == j_max_temp) j_max = j - 1;
+        ef[j] = ef[j_max];
+        vf[j] = vf[j_max];
+        Mf[j] = Mf[j_max];
+        Sf[j] = Sf[j_max];
+      }
     }
   }
   j_max_temp = j_max;

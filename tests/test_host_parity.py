@@ -111,6 +111,29 @@ def test_heavy_plunging_host_stage_bit_identical(L):
     assert done == 3
 
 
+CLAMP_CYCLE_SOURCES = [
+    # production-prior draws that plunge within a few percent of a long knot
+    # grid; past the plunge the clamped AK evolution climbs from the clamp
+    # state and falls back to it periodically (period > 1), which the product
+    # fills in from the first repeated clamp; the oracle runs every step
+    [0.5, 0.8053263365752357, 13.02269575452846, 0.3169443507894639, 122707.26286828033, 32.515375294234147,
+     4.867828697845539, 1.9909702715555997, 2.9346288941805714, 1.6807238079400315, 3.087183362292972,
+     2.5292079893135617, 3.233535495391245, 1.0],
+    [0.5, 0.09293423704679246, 7.375436404648140, 0.03683255387786658, 166902.5809156527, 29.647813206436222,
+     1.8930110446503920, 6.231121232928174, 3.951397327617074, 0.5099649652273917, 4.605533324478770,
+     0.6493725432204903, 5.930568302750496, 1.0],
+    [0.5, 0.5102705351537529, 8.074165353204982, 0.2773384640852461, 236184.32006748492, 33.94609678086813,
+     5.088971431664089, 0.8321169033334020, 5.851985312341005, 1.1741197989655192, 4.141312006222881,
+     1.2600595317379339, 1.7196746019819851, 1.0],
+]
+
+
+@pytest.mark.parametrize("k", range(len(CLAMP_CYCLE_SOURCES)))
+def test_clamp_cycle_host_stage_bit_identical(L, k):
+    K, info = _assert_same_plan(L, CLAMP_CYCLE_SOURCES[k], 2 ** 21, 15.0)
+    assert info[9] == 1 and K > 300_000
+
+
 def test_plunging_backint_host_stage_bit_identical(L):
     pars = [0.5, 0.9, 7.2, 0.3, 3e5, 30.0, 1.0, 2.0, 3.0, 1.1, 4.0, 0.7, 5.0, 1.0]
     K, info = _assert_same_plan(L, pars, 2 ** 22, 8.0, backint=True)
