@@ -198,6 +198,7 @@ struct Lane {
   double2 *d_hf = nullptr;    // [2][nbins]
   double2 *d_fy = nullptr;    // [2][N/2] four-step FFT scratch (fused FFT path)
   double *d_partials = nullptr, *d_like = nullptr;
+  double *d_fparts = nullptr;     // pass 2 with the final reduction: published pairs
   double *d_ms_coef = nullptr;  // mode-sum block Taylor coefficients (grown with nmodes)
   size_t ms_coef_cap = 0;
   int *d_ms_p = nullptr;
@@ -229,6 +230,7 @@ struct aak_handle_s {
   int nlanes = 2;                 // lanes used by batches (lane 0 serves single calls)
   int feed_depth = kFeedDepth;    // short plans kept ready or in progress for the GPU
   bool use_graphs = true;         // batches replay each lane's captured likelihood stage (AAK_GRAPHS=0: off)
+  bool fuse_final = true;         // pass 2 ends in the final reduction (AAK_FUSED_FINAL=0: separate launch)
   double *d_bessel = nullptr;     // c_m(k) table | log k!
   double2 *d_data = nullptr;      // [2][nbins] weighted data (fused FFT path: in its row order)
   double *d_w = nullptr;          // [2][nbins] weights (same order as d_data)
@@ -379,6 +381,10 @@ static int alloc_lane(aak_handle h, Lane &L) {
   HIPCHK(hipMalloc(&L.d_like, sizeof(double) * 2));
   if (h->fused_fft) {
     HIPCHK(hipMalloc(&L.d_fy, sizeof(double2) * 2 * (size_t)h->fft.M));
+    const int fp = fft_partials(h->fft);
+    HIPCHK(hipMalloc(&L.d_fparts, sizeof(double) * 2 * (size_t)fp));
+    launch_fft_final_reset(L.d_fparts, fp, L.stream);
+    HIPCHK(hipGetLastError());
     L.ready = true;
     return 0;
   }
@@ -418,7 +424,7 @@ static void free_lane(Lane &L) {
   if (L.plan) rocfft_plan_destroy(L.plan);
   if (L.exec) rocfft_execution_info_destroy(L.exec);
   void *ptrs[] = {L.d_table[0], L.d_table[1], L.d_dense, L.d_h, L.d_t, L.d_flags, L.d_guardw, L.d_foldmem,
-                  L.d_hf, L.d_fy, L.d_partials, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_slot,
+                  L.d_hf, L.d_fy, L.d_partials, L.d_fparts, L.d_like, L.d_work, L.d_ms_coef, L.d_ms_p, L.d_ms_slot,
                   L.d_ms_list, L.d_ms_split, L.d_ms_eref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -599,6 +605,16 @@ static int lane_like_graph(aak_handle h, Lane &L) {
 // clearing launch (Lane::flags_dirty).
 static int enqueue_likelihood(aak_handle h, Lane &L, double *out_dev, bool keep_spectrum, const int *flags) {
   int *clear = (flags == L.d_flags) ? L.d_flags : nullptr;
+  if (h->fused_fft && h->fuse_final && !keep_spectrum) {
+    // two launches: the second pass's last workgroup writes the template's row
+    launch_fft_pass1(h->fft, L.d_h, L.d_h + h_stride(h->N), L.d_fy, L.stream);
+    h->mark(L, 4);
+    launch_fft_pass2_final(h->fft, L.d_fy, L.d_fparts, FftFinal{out_dev, flags, clear}, L.stream);
+    h->mark(L, 5);
+    HIPCHK(hipGetLastError());
+    if (clear) L.flags_dirty = false;
+    return 0;
+  }
   if (h->fused_fft && h->use_graphs && !keep_spectrum && !h->profiling && clear) {
     int rc = lane_like_graph(h, L);
     if (rc) return rc;
@@ -838,6 +854,7 @@ int aak_create(double T_fit, int init_length, int length, double init_dt, double
   h->N = length;
   h->nbins = length / 2 + 1;
   if (const char *env = std::getenv("AAK_GRAPHS")) h->use_graphs = std::atoi(env) != 0;
+  if (const char *env = std::getenv("AAK_FUSED_FINAL")) h->fuse_final = std::atoi(env) != 0;
   if (const char *env = std::getenv("AAK_LANES")) {
     const int v = std::atoi(env);
     h->nlanes = v < 1 ? 1 : (v > kMaxLanes ? kMaxLanes : v);

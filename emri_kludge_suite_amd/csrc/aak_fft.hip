@@ -238,9 +238,45 @@ __global__ __launch_bounds__(col_cols<LR>() * fft_threads<LR>()) void k_fft_cols
 // ---------------------------------------------------------------- pass 2
 // One workgroup per row pair, half of it per row: block 0 takes rows 0 and R/2
 // (each its own mirror), block p > 0 rows p and R - p.
-template <int LC>
+constexpr int kFinalLanes = 256;       // lanes of the final reduction's order
+constexpr int kFinalSpinCap = 1 << 20;  // polls of one missing pair before it reads as NaN
+
+constexpr int kFinalMaxPer = 8;         // pairs per lane: F.R <= 2048
+
+__device__ inline long long final_peek(const double *p) {
+  return __hip_atomic_load(reinterpret_cast<const long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The value of a published word, first read as v (polled again while it still
+// reads kFinalEmpty); the word is returned to kFinalEmpty.
+__device__ inline double final_take(double *p, long long v) {
+  long long *w = reinterpret_cast<long long *>(p);
+  int spins = 0;
+  while (v == kFinalEmpty) {
+    if (++spins > kFinalSpinCap) {
+      v = 0x7FF8000000000000LL;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __hip_atomic_store(w, kFinalEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double(v);
+}
+
+__device__ inline void final_put(double *p, double x) {
+  __hip_atomic_store(reinterpret_cast<long long *>(p), __double_as_longlong(x), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_fft_final_reset(double *partials, int nwords) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x)
+    reinterpret_cast<long long *>(partials)[i] = kFinalEmpty;
+}
+
+template <int LC, bool FINAL>
 __global__ __launch_bounds__(2 * fft_threads<LC>()) void k_fft_rows(FftArgs F, const double2 *Y, double *partials,
-                                                                    double2 *hw_out) {
+                                                                    double2 *hw_out, FftFinal fin) {
   constexpr int RSD = LC + LC / 16, NT = fft_threads<LC>();
   __shared__ double2 s[2 * RSD];
   __shared__ double s_red[2 * NT / 64][2];
@@ -328,9 +364,66 @@ __global__ __launch_bounds__(2 * fft_threads<LC>()) void k_fft_rows(FftArgs F, c
       b += s_red[w][1];
     }
     const int q = ch * (F.R / 2) + blk;
-    partials[2 * q] = a;
-    partials[2 * q + 1] = b;
+    if (FINAL) {
+      final_put(partials + 2 * q, a);
+      final_put(partials + 2 * q + 1, b);
+    } else {
+      partials[2 * q] = a;
+      partials[2 * q + 1] = b;
+    }
   }
+  // The workgroup dispatched last reduces: every other one was dispatched
+  // before it, so every pair it waits for is on its way.  (LDS of the FFT is
+  // reused where it is large enough, the FFT being done.)
+  if (!FINAL || blk != (int)gridDim.x - 1 || ch != (int)gridDim.y - 1) return;
+  constexpr int BS = 2 * NT;
+  constexpr bool kReuse = 4 * RSD >= 2 * kFinalLanes;
+  __shared__ double s_fin[kReuse ? 1 : 2 * kFinalLanes];
+  double *sd = kReuse ? reinterpret_cast<double *>(s) : s_fin, *sh = sd + kFinalLanes;
+  // every word of the thread's lanes is requested before the first is used
+  constexpr int kVL = (kFinalLanes + BS - 1) / BS;  // lanes per thread
+  long long w[kVL][2 * kFinalMaxPer];
+#pragma unroll
+  for (int v = 0; v < kVL; v++)
+#pragma unroll
+    for (int k = 0; k < kFinalMaxPer; k++) {
+      const int q = tid + v * BS + k * kFinalLanes;
+      if (tid + v * BS < kFinalLanes && q < F.R) {
+        w[v][2 * k] = final_peek(partials + 2 * q);
+        w[v][2 * k + 1] = final_peek(partials + 2 * q + 1);
+      }
+    }
+#pragma unroll
+  for (int v = 0; v < kVL; v++) {
+    const int t = tid + v * BS;
+    if (t >= kFinalLanes) break;
+    double pd = 0.0, ph = 0.0;
+#pragma unroll
+    for (int k = 0; k < kFinalMaxPer; k++) {
+      const int q = t + k * kFinalLanes;
+      if (q < F.R) {
+        pd += final_take(partials + 2 * q, w[v][2 * k]);
+        ph += final_take(partials + 2 * q + 1, w[v][2 * k + 1]);
+      }
+    }
+    sd[t] = pd;
+    sh[t] = ph;
+  }
+  __syncthreads();
+  for (int o = kFinalLanes / 2; o > 0; o >>= 1) {
+    for (int t = tid; t < o; t += BS) {
+      sd[t] += sd[t + o];
+      sh[t] += sh[t + o];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const bool bad = fin.flags && fin.flags[3] != 0;
+    fin.out[0] = bad ? __builtin_nan("") : 4.0 * sd[0];
+    fin.out[1] = bad ? __builtin_nan("") : 4.0 * sh[0];
+  }
+  __syncthreads();
+  if (fin.clear && tid < 8) fin.clear[tid] = 0;
 }
 
 // ---------------------------------------------------------------- launch
@@ -341,8 +434,14 @@ static void launch_cols(const FftArgs &F, const double *h0, const double *h1, do
 }
 
 template <int LC>
-static void launch_rows(const FftArgs &F, const double2 *Y, double *partials, double2 *hw, hipStream_t s) {
-  hipLaunchKernelGGL(k_fft_rows<LC>, dim3(F.R / 2, 2), dim3(2 * fft_threads<LC>()), 0, s, F, Y, partials, hw);
+static void launch_rows(const FftArgs &F, const double2 *Y, double *partials, double2 *hw, const FftFinal *fin,
+                        hipStream_t s) {
+  if (fin)
+    hipLaunchKernelGGL((k_fft_rows<LC, true>), dim3(F.R / 2, 2), dim3(2 * fft_threads<LC>()), 0, s, F, Y, partials,
+                       nullptr, *fin);
+  else
+    hipLaunchKernelGGL((k_fft_rows<LC, false>), dim3(F.R / 2, 2), dim3(2 * fft_threads<LC>()), 0, s, F, Y, partials,
+                       hw, FftFinal{});
 }
 
 bool fft_supported(int N, int *R, int *C) {
@@ -424,18 +523,31 @@ void launch_fft_pass1(const FftArgs &F, const double *h0, const double *h1, doub
   }
 }
 
-void launch_fft_pass2(const FftArgs &F, const double2 *Y, double *partials, double2 *hw_out, hipStream_t s) {
+static void launch_pass2(const FftArgs &F, const double2 *Y, double *partials, double2 *hw_out, const FftFinal *fin,
+                         hipStream_t s) {
   switch (F.C) {
-    case 8: launch_rows<8>(F, Y, partials, hw_out, s); break;
-    case 16: launch_rows<16>(F, Y, partials, hw_out, s); break;
-    case 32: launch_rows<32>(F, Y, partials, hw_out, s); break;
-    case 64: launch_rows<64>(F, Y, partials, hw_out, s); break;
-    case 128: launch_rows<128>(F, Y, partials, hw_out, s); break;
-    case 256: launch_rows<256>(F, Y, partials, hw_out, s); break;
-    case 512: launch_rows<512>(F, Y, partials, hw_out, s); break;
-    case 1024: launch_rows<1024>(F, Y, partials, hw_out, s); break;
-    default: launch_rows<2048>(F, Y, partials, hw_out, s); break;
+    case 8: launch_rows<8>(F, Y, partials, hw_out, fin, s); break;
+    case 16: launch_rows<16>(F, Y, partials, hw_out, fin, s); break;
+    case 32: launch_rows<32>(F, Y, partials, hw_out, fin, s); break;
+    case 64: launch_rows<64>(F, Y, partials, hw_out, fin, s); break;
+    case 128: launch_rows<128>(F, Y, partials, hw_out, fin, s); break;
+    case 256: launch_rows<256>(F, Y, partials, hw_out, fin, s); break;
+    case 512: launch_rows<512>(F, Y, partials, hw_out, fin, s); break;
+    case 1024: launch_rows<1024>(F, Y, partials, hw_out, fin, s); break;
+    default: launch_rows<2048>(F, Y, partials, hw_out, fin, s); break;
   }
+}
+
+void launch_fft_pass2(const FftArgs &F, const double2 *Y, double *partials, double2 *hw_out, hipStream_t s) {
+  launch_pass2(F, Y, partials, hw_out, nullptr, s);
+}
+
+void launch_fft_pass2_final(const FftArgs &F, const double2 *Y, double *partials, const FftFinal &fin, hipStream_t s) {
+  launch_pass2(F, Y, partials, nullptr, &fin, s);
+}
+
+void launch_fft_final_reset(double *partials, int npairs, hipStream_t s) {
+  hipLaunchKernelGGL(k_fft_final_reset, dim3(8), dim3(256), 0, s, partials, 2 * npairs);
 }
 
 }  // namespace aak

@@ -35,4 +35,22 @@ void fft_bind_tables(FftArgs *F, const double2 *tw);
 void launch_fft_pass1(const FftArgs &F, const double *h0, const double *h1, double2 *Y, hipStream_t s);
 void launch_fft_pass2(const FftArgs &F, const double2 *Y, double *partials, double2 *hw_out, hipStream_t s);
 
+// Pass 2 with the final reduction folded in: each workgroup publishes its
+// pair in `partials` (words that read kFinalEmpty until written); the one
+// dispatched last collects them and sums them in the order of the separate
+// final reduction (k_dot_final: 256 lanes, lane t adds pairs t, t + 256, ...
+// in turn, then a halving tree), writes out[0..1] = 4 x the sums (NaN when
+// flags[3] is set), zeroes clear[0..7] (flags and clear may be null) and
+// returns every word to kFinalEmpty.  A pair that never arrives within the
+// spin cap reads as NaN.  launch_fft_final_reset brings a new buffer to that
+// state.
+struct FftFinal {
+  double *out;
+  const int *flags;
+  int *clear;
+};
+constexpr long long kFinalEmpty = 0x7FF4A5A5A5A5A5B1LL;  // a signalling NaN: no arithmetic result
+void launch_fft_pass2_final(const FftArgs &F, const double2 *Y, double *partials, const FftFinal &fin, hipStream_t s);
+void launch_fft_final_reset(double *partials, int npairs, hipStream_t s);
+
 }  // namespace aak
