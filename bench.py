@@ -429,7 +429,7 @@ def main():
         "inner_product": {"bound": "hbm", "achieved": ip_bytes / (ip_ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                           "frac": ip_bytes / (ip_ms * 1e-3) / 8e12, "bytes_per_launch": ip_bytes, "ms": ip_ms,
                           "kernels": "k_fft_cols (column FFTs x twiddles) + k_fft_rows (row FFTs, real-spectrum split, "
-                                     "weighting, both sums) + k_dot_final",
+                                     "weighting, both sums; its last workgroup adds the partial sums)",
                           "scratch_bytes_per_launch": 32 * N_SAMPLES},
     }
     del g

@@ -274,6 +274,59 @@ __global__ void k_fft_final_reset(double *partials, int nwords) {
     reinterpret_cast<long long *>(partials)[i] = kFinalEmpty;
 }
 
+// The final reduction, run by one workgroup of BS threads with LDS scratch
+// sd (2 kFinalLanes doubles): collects the R published pairs and sums them in
+// the order of k_dot_final.
+template <int BS>
+__device__ void final_collect(int R, double *partials, const FftFinal &fin, double *sd) {
+  const int tid = threadIdx.x;
+  double *sh = sd + kFinalLanes;
+  // every word of the thread's lanes is requested before the first is used
+  constexpr int kVL = (kFinalLanes + BS - 1) / BS;  // lanes per thread
+  long long w[kVL][2 * kFinalMaxPer];
+#pragma unroll
+  for (int v = 0; v < kVL; v++)
+#pragma unroll
+    for (int k = 0; k < kFinalMaxPer; k++) {
+      const int q = tid + v * BS + k * kFinalLanes;
+      if (tid + v * BS < kFinalLanes && q < R) {
+        w[v][2 * k] = final_peek(partials + 2 * q);
+        w[v][2 * k + 1] = final_peek(partials + 2 * q + 1);
+      }
+    }
+#pragma unroll
+  for (int v = 0; v < kVL; v++) {
+    const int t = tid + v * BS;
+    if (t >= kFinalLanes) break;
+    double pd = 0.0, ph = 0.0;
+#pragma unroll
+    for (int k = 0; k < kFinalMaxPer; k++) {
+      const int q = t + k * kFinalLanes;
+      if (q < R) {
+        pd += final_take(partials + 2 * q, w[v][2 * k]);
+        ph += final_take(partials + 2 * q + 1, w[v][2 * k + 1]);
+      }
+    }
+    sd[t] = pd;
+    sh[t] = ph;
+  }
+  __syncthreads();
+  for (int o = kFinalLanes / 2; o > 0; o >>= 1) {
+    for (int t = tid; t < o; t += BS) {
+      sd[t] += sd[t + o];
+      sh[t] += sh[t + o];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const bool bad = fin.flags && fin.flags[3] != 0;
+    fin.out[0] = bad ? __builtin_nan("") : 4.0 * sd[0];
+    fin.out[1] = bad ? __builtin_nan("") : 4.0 * sh[0];
+  }
+  __syncthreads();
+  if (fin.clear && tid < 8) fin.clear[tid] = 0;
+}
+
 template <int LC, bool FINAL>
 __global__ __launch_bounds__(2 * fft_threads<LC>()) void k_fft_rows(FftArgs F, const double2 *Y, double *partials,
                                                                     double2 *hw_out, FftFinal fin) {
@@ -281,6 +334,16 @@ __global__ __launch_bounds__(2 * fft_threads<LC>()) void k_fft_rows(FftArgs F, c
   __shared__ double2 s[2 * RSD];
   __shared__ double s_red[2 * NT / 64][2];
   const int ch = blockIdx.y, blk = blockIdx.x, tid = threadIdx.x;
+  if (FINAL && blk == F.R / 2) {
+    // the extra column of workgroups: (R/2, 1), dispatched after every row
+    // workgroup, collects their pairs, so each pair it waits for is on its
+    // way; (R/2, 0) has nothing to do.  LDS of the FFT serves as scratch where
+    // it is large enough.
+    constexpr bool kReuse = 4 * RSD >= 2 * kFinalLanes;
+    __shared__ double s_fin[kReuse ? 1 : 2 * kFinalLanes];
+    if (ch == 1) final_collect<2 * NT>(F.R, partials, fin, kReuse ? reinterpret_cast<double *>(s) : s_fin);
+    return;
+  }
   const int half = tid / NT;
   const int A = blk, B = blk == 0 ? F.R / 2 : F.R - blk;
   const double2 *Yc = Y + (size_t)ch * F.M;
@@ -372,58 +435,6 @@ __global__ __launch_bounds__(2 * fft_threads<LC>()) void k_fft_rows(FftArgs F, c
       partials[2 * q + 1] = b;
     }
   }
-  // The workgroup dispatched last reduces: every other one was dispatched
-  // before it, so every pair it waits for is on its way.  (LDS of the FFT is
-  // reused where it is large enough, the FFT being done.)
-  if (!FINAL || blk != (int)gridDim.x - 1 || ch != (int)gridDim.y - 1) return;
-  constexpr int BS = 2 * NT;
-  constexpr bool kReuse = 4 * RSD >= 2 * kFinalLanes;
-  __shared__ double s_fin[kReuse ? 1 : 2 * kFinalLanes];
-  double *sd = kReuse ? reinterpret_cast<double *>(s) : s_fin, *sh = sd + kFinalLanes;
-  // every word of the thread's lanes is requested before the first is used
-  constexpr int kVL = (kFinalLanes + BS - 1) / BS;  // lanes per thread
-  long long w[kVL][2 * kFinalMaxPer];
-#pragma unroll
-  for (int v = 0; v < kVL; v++)
-#pragma unroll
-    for (int k = 0; k < kFinalMaxPer; k++) {
-      const int q = tid + v * BS + k * kFinalLanes;
-      if (tid + v * BS < kFinalLanes && q < F.R) {
-        w[v][2 * k] = final_peek(partials + 2 * q);
-        w[v][2 * k + 1] = final_peek(partials + 2 * q + 1);
-      }
-    }
-#pragma unroll
-  for (int v = 0; v < kVL; v++) {
-    const int t = tid + v * BS;
-    if (t >= kFinalLanes) break;
-    double pd = 0.0, ph = 0.0;
-#pragma unroll
-    for (int k = 0; k < kFinalMaxPer; k++) {
-      const int q = t + k * kFinalLanes;
-      if (q < F.R) {
-        pd += final_take(partials + 2 * q, w[v][2 * k]);
-        ph += final_take(partials + 2 * q + 1, w[v][2 * k + 1]);
-      }
-    }
-    sd[t] = pd;
-    sh[t] = ph;
-  }
-  __syncthreads();
-  for (int o = kFinalLanes / 2; o > 0; o >>= 1) {
-    for (int t = tid; t < o; t += BS) {
-      sd[t] += sd[t + o];
-      sh[t] += sh[t + o];
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    const bool bad = fin.flags && fin.flags[3] != 0;
-    fin.out[0] = bad ? __builtin_nan("") : 4.0 * sd[0];
-    fin.out[1] = bad ? __builtin_nan("") : 4.0 * sh[0];
-  }
-  __syncthreads();
-  if (fin.clear && tid < 8) fin.clear[tid] = 0;
 }
 
 // ---------------------------------------------------------------- launch
@@ -437,8 +448,8 @@ template <int LC>
 static void launch_rows(const FftArgs &F, const double2 *Y, double *partials, double2 *hw, const FftFinal *fin,
                         hipStream_t s) {
   if (fin)
-    hipLaunchKernelGGL((k_fft_rows<LC, true>), dim3(F.R / 2, 2), dim3(2 * fft_threads<LC>()), 0, s, F, Y, partials,
-                       nullptr, *fin);
+    hipLaunchKernelGGL((k_fft_rows<LC, true>), dim3(F.R / 2 + 1, 2), dim3(2 * fft_threads<LC>()), 0, s, F, Y,
+                       partials, nullptr, *fin);
   else
     hipLaunchKernelGGL((k_fft_rows<LC, false>), dim3(F.R / 2, 2), dim3(2 * fft_threads<LC>()), 0, s, F, Y, partials,
                        hw, FftFinal{});

@@ -35,10 +35,10 @@ void fft_bind_tables(FftArgs *F, const double2 *tw);
 void launch_fft_pass1(const FftArgs &F, const double *h0, const double *h1, double2 *Y, hipStream_t s);
 void launch_fft_pass2(const FftArgs &F, const double2 *Y, double *partials, double2 *hw_out, hipStream_t s);
 
-// Pass 2 with the final reduction folded in: each workgroup publishes its
-// pair in `partials` (words that read kFinalEmpty until written); the one
-// dispatched last collects them and sums them in the order of the separate
-// final reduction (k_dot_final: 256 lanes, lane t adds pairs t, t + 256, ...
+// Pass 2 with the final reduction folded in: each row workgroup publishes its
+// pair in `partials` (words that read kFinalEmpty until written); one extra
+// workgroup, dispatched after all of them, collects the pairs and sums them
+// in the order of the separate final reduction (k_dot_final: 256 lanes, lane t adds pairs t, t + 256, ...
 // in turn, then a halving tree), writes out[0..1] = 4 x the sums (NaN when
 // flags[3] is set), zeroes clear[0..7] (flags and clear may be null) and
 // returns every word to kFinalEmpty.  A pair that never arrives within the
