@@ -722,6 +722,11 @@ static void worker_main(aak_handle h, int id) {
     WaveConfig cfg = h->cfg;
     if (nmax_rows) cfg.n_max = nmax_rows[i];
     const int rc = plan_safe(cfg, sp, &plan, &err, &sink);
+    if (rc == 0) {
+      // likelihoods need the spline only up to the end of the signal
+      plan.P.full_dense = 0;
+      trim_table_to_signal(&plan);
+    }
     slot->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
     slot->P = plan.P;
     slot->tab_n = plan.tab_n;

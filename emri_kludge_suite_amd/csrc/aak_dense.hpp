@@ -23,6 +23,8 @@ constexpr int kTableArrays = 8;
 //   j >= j0+steps      dt_map[steps-1] + dt_large * (j - j0 - steps + 1)
 // exactly as the host evolution computes them (AAK.cc:114-121, 224-231), so
 // host and device agree bit for bit without shipping K times.
+constexpr int kRatesBlock = 256;  // samples per k_rates workgroup
+
 struct KnotGrid {
   int K;           // knots in the table
   int head_off;    // knot index of dt_map[0]  (j0 - j_start)

@@ -594,6 +594,23 @@ void fill_table(const KnotSet &ks, const SparseTrajectory &tr, double *tab, Knot
   grid->t_head_last = tr.dt_map[ks.steps - 1];
 }
 
+void trim_table_to_signal(WaveformPlan *plan) {
+  DenseParams &P = plan->P;
+  const long long last = ((long long)(P.last_fold + 1) / kRatesBlock) * kRatesBlock + kRatesBlock - 1;
+  if (last >= P.N) return;  // sample N (time t_last, the last knot) is evaluated
+  const int i_max = (int)std::max(last, 1LL);
+  const KnotGrid g = P.grid;
+  double *tab = plan->tab;
+  const double *head = tab + (size_t)kTableArrays * g.K;
+  const int Kn = knot_index(g, head, (double)(i_max - P.i0) * P.dt) + 2;
+  if (Kn >= g.K) return;
+  for (int a = 1; a < kTableArrays; a++) std::memmove(tab + (size_t)a * Kn, tab + (size_t)a * g.K, sizeof(double) * Kn);
+  std::memmove(tab + (size_t)kTableArrays * Kn, head, sizeof(double) * g.steps);
+  P.grid.K = Kn;
+  P.grid.flat = 0;
+  plan->tab_n = (size_t)kTableArrays * Kn + g.steps;
+}
+
 double table_eval(const double *tab, const KnotGrid &g, int array, double t) {
   const int K = g.K;
   const double *head = tab + (size_t)kTableArrays * K;

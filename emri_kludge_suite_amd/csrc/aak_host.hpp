@@ -105,6 +105,13 @@ int plunge_buffer(const WaveConfig &cfg, const KnotSet &ks, const double *tab, c
                   int *ip, int *ib, double *tp, std::string *err);
 int plan_waveform(const WaveConfig &cfg, const SourceParams &sp, WaveformPlan *plan, std::string *err,
                   TableSink *sink = nullptr);
+// For a plan whose device stage skips the samples past the signal
+// (DenseParams::full_dense = 0, as in likelihood batches): the knots past
+// the interval of the last sample that stage evaluates (the rest of the
+// last rates block of kRatesBlock samples after last_fold + 1) are dropped
+// from the table, in place.  Every evaluated sample reads the same interval
+// and values, so the results are unchanged.
+void trim_table_to_signal(WaveformPlan *plan);
 double omega_phi(double v, double e, double cosiota, double s, double M);
 
 }  // namespace aak

@@ -33,7 +33,7 @@
 namespace aak {
 
 // ------------------------------------------------------------------ rates
-constexpr int kRatesThreads = 256;
+constexpr int kRatesThreads = kRatesBlock;
 constexpr int kStageCap = 128;  // knot intervals staged in LDS per block (else direct evaluation)
 
 __device__ inline int knot_search(const double *x, int lo, int hi, double t) {

@@ -74,6 +74,28 @@ def test_batch_equals_single_calls():
 
 
 @pytest.mark.timeout(300)
+def test_batch_with_trimmed_tables_equals_single_calls():
+    # sources that plunge early into clamp cycles: the batch uploads their
+    # tables only up to the last sample it evaluates, single calls upload
+    # them whole (every sample is evaluated there); the rows must agree
+    from test_host_parity import CLAMP_CYCLE_SOURCES
+    N, dt = 2 ** 21, 15.0
+    print("case batch-trimmed-tables N=2^21 dt=15 n_max=20", flush=True)
+    g = _handle_with_data(N, dt, 20, seed=6)
+    P = np.array(CLAMP_CYCLE_SOURCES + [list(_prior(np.random.default_rng(7), 1)[0])])
+    like, st = g.LikelihoodBatch(P, return_status=True)
+    assert np.all(st[:3] == 0), st
+    for i in range(len(P)):
+        if st[i] < 0:
+            continue
+        single = g.WaveformThroughLikelihood(*P[i])
+        if i < 3:
+            inf = g.info()
+            assert inf["i_plunge"] + inf["i_buffer"] < N // 2, inf
+        assert np.array_equal(like[i], single), (i, like[i], single)
+
+
+@pytest.mark.timeout(300)
 def test_batch_repeatable_and_thread_count_independent():
     print("case batch-repeat N=2^20 dt=15 n_max=12 seed 5", flush=True)
     g = _handle_with_data(N2, DT2, 12, seed=4)
