@@ -326,13 +326,27 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
     }
     int j = j0;
     while (j <= j_end && !(ef[j] < 0. || vf[j] < vf[std::max(j0, j - 1)])) j++;
-    for (; j <= j_end; j++) {
+    for (; j <= j_end && j_max == j_max_temp; j++) {
       if (ef[j] < 0. || vf[j] < vf[std::max(j0, j - 1)]) {
-        if (j_max == j_max_temp) j_max = j - 1;
+        j_max = j - 1;
         ef[j] = ef[j_max];
         vf[j] = vf[j_max];
         Mf[j] = Mf[j_max];
         Sf[j] = Sf[j_max];
+      }
+    }
+    // j_max is fixed from here on, so is the clamp state: selects instead of
+    // branches (the clamps past a plunge follow no pattern a predictor learns)
+    if (j <= j_end) {
+      const double Ec = ef[j_max], Vc = vf[j_max], Mc_ = Mf[j_max], Sc_ = Sf[j_max];
+      double vprev = vf[std::max(j0, j - 1)];
+      for (; j <= j_end; j++) {
+        const bool c = ef[j] < 0. || vf[j] < vprev;
+        ef[j] = c ? Ec : ef[j];
+        vf[j] = c ? Vc : vf[j];
+        Mf[j] = c ? Mc_ : Mf[j];
+        Sf[j] = c ? Sc_ : Sf[j];
+        vprev = vf[j];
       }
     }
   }
@@ -356,20 +370,47 @@ int build_knots(const WaveConfig &cfg, const double *par, double M_phys, double 
   // margin dwarfs every rounding), so neither test below can pass and the pow
   // is skipped; elsewhere p_j is formed exactly as pn_p forms it.
   const double L3 = LSO_max * LSO_max * LSO_max * (1.0 + 1e-9);
-  for (int j = j0; j <= j_max_temp; j++) {
-    if (j > j_max) break;
-    const double x = PN::omega_phi(vf[j], ef[j], coslam, Sf[j], Mf[j]) * M_phys * kMsunSec;
-    const double ome = 1. - ef[j] * ef[j];
-    if (x > 0. && ome > 0. && ome * ome * ome >= L3 * (x * x)) continue;
-    const double pj = ome * pow(x, -2. / 3.);
-    if (pj < LSO_min && j_max == j_max_temp) {
-      j_min = std::max(j - j_RR, j0);
-      j_max = std::min(j + 1, j_max_temp);
+  // x and the skip test of a chunk of knots first (independent per knot, so
+  // the loop vectorises), then the scan itself in knot order
+  constexpr int kScanChunk = 64;
+  double xs[kScanChunk];
+  bool skip[kScanChunk];
+  bool scan_done = false;
+  for (int jc = j0; jc <= j_max_temp && !scan_done; jc += kScanChunk) {
+    const int m = std::min(kScanChunk, j_max_temp - jc + 1);
+    // PN::omega_phi with its orientation chosen once: (cos i, s) when
+    // cos i > 0, else (-cos i, -s)
+    const bool pro = coslam > 0;
+    const double Y = pro ? coslam : -coslam;
+    for (int u = 0; u < m; u++) {
+      const int j = jc + u;
+      const double q = pro ? Sf[j] : -Sf[j];
+      const double x = PN::dphidm(vf[j], ef[j], Y, q) / PN::dtdm(vf[j], ef[j], Y, q) / Mf[j] * M_phys * kMsunSec;
+      const double ome = 1. - ef[j] * ef[j];
+      xs[u] = x;
+      skip[u] = (x > 0.) & (ome > 0.) & (ome * ome * ome >= L3 * (x * x));
     }
-    if (j > j0 && (j - j0) % j_RR == 0 && pj < LSO_max && j_max == j_max_temp) {
-      j_min = std::max(j - j_RR, j0);
-      if (unstable(j, &bad)) j_max = std::min(j + 1, j_max_temp);
-      if (bad) break;
+    for (int u = 0; u < m; u++) {
+      const int j = jc + u;
+      if (j > j_max) {
+        scan_done = true;
+        break;
+      }
+      if (skip[u]) continue;
+      const double ome = 1. - ef[j] * ef[j];
+      const double pj = ome * pow(xs[u], -2. / 3.);
+      if (pj < LSO_min && j_max == j_max_temp) {
+        j_min = std::max(j - j_RR, j0);
+        j_max = std::min(j + 1, j_max_temp);
+      }
+      if (j > j0 && (j - j0) % j_RR == 0 && pj < LSO_max && j_max == j_max_temp) {
+        j_min = std::max(j - j_RR, j0);
+        if (unstable(j, &bad)) j_max = std::min(j + 1, j_max_temp);
+        if (bad) {
+          scan_done = true;
+          break;
+        }
+      }
     }
   }
   while (!bad && j_max - j_min > 1) {
